@@ -1,0 +1,240 @@
+/*
+ * dlg.h -- C ABI of the MI355X semi-naive fixpoint engine (libdlg.so).
+ *
+ * The reference (harp-lab/MNMGDatalog) has no library boundary: each program
+ * #includes the .cu files of common/.  This header exports the same ROLES as plain C entry
+ * points (plain pointers and sizes, no C++ or torch types), so C/C++ drivers,
+ * ctypes harnesses or any FFI can bind them.  Each declaration cites the
+ * reference interface it replaces.
+ *
+ * Tuples.  A tuple is the reference's `struct Entity {int key; int value;}`
+ * (common/utils.cu:8-11) stored as one uint64_t:
+ *     t = ((uint64_t)((uint32_t)key ^ 0x80000000u) << 32) | ((uint32_t)value ^ 0x80000000u)
+ * so unsigned 64-bit order equals the reference's signed lexicographic
+ * (key, value) order `set_cmp` (common/utils.cu:115-124).
+ *
+ * Errors.  Every call returns a dlg_status; the library never exits the
+ * process (the reference's checkCuda() exits, common/error_handler.cu:1-8).
+ * dlg_last_error() gives the message of the last failure on a context.
+ *
+ * Threading.  A context owns one HIP stream and a device memory pool.  A
+ * context is used by one host thread at a time; several contexts may share a
+ * device.  Entry points that return sizes synchronise their stream.
+ */
+#ifndef DLG_H
+#define DLG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dlg_status {
+    DLG_OK = 0,
+    DLG_INVALID_INPUT = 1,   /* bad argument, malformed file, reserved id            */
+    DLG_OUT_OF_MEMORY = 2,   /* device allocation failed or HBM budget exceeded      */
+    DLG_OVERFLOW = 3,        /* a count exceeds what the engine represents (2^32-1 rows per buffer) */
+    DLG_DEVICE_ERROR = 4,    /* HIP runtime error                                    */
+    DLG_IO_ERROR = 5,        /* file open/read/write failure                         */
+    DLG_REFUSED = 6          /* serving: admission refused (predicted need > budget) */
+} dlg_status;
+
+typedef struct dlg_ctx dlg_ctx;
+typedef struct dlg_hash dlg_hash;
+
+/* ---------------------------------------------------------------- context */
+/* Replaces the per-program set-up in benchmark() (tc.cu:42-60: device query,
+ * warm_up_kernel, block 512 / grid 32*SMs). */
+dlg_status dlg_ctx_create(int device, dlg_ctx **out);
+void dlg_ctx_destroy(dlg_ctx *ctx);
+const char *dlg_last_error(const dlg_ctx *ctx);
+dlg_status dlg_ctx_sync(dlg_ctx *ctx);
+/* Device and library facts: CU count, HBM bytes, LDS bytes per block, arch name. */
+dlg_status dlg_device_info(dlg_ctx *ctx, int32_t *num_cus, int64_t *hbm_bytes,
+                           int32_t *lds_per_block, char *arch_name, int32_t arch_len);
+
+/* Per-kernel event timing on the context's stream (off by default).  While on,
+ * every launch is bracketed by hipEvents and its algorithmic bytes recorded. */
+dlg_status dlg_profiling(dlg_ctx *ctx, int enable);
+/* Kernel ids for dlg_kernel_stats(); DLG_K_COUNT entries. */
+enum {
+    DLG_K_PACK = 0, DLG_K_HASH_BUILD, DLG_K_JOIN_COUNT, DLG_K_SCAN, DLG_K_JOIN_EXPAND,
+    DLG_K_SORT_HIST, DLG_K_SORT_UPSWEEP, DLG_K_SORT_DOWNSWEEP, DLG_K_SELECT, DLG_K_MERGE,
+    DLG_K_WCC_RELAX, DLG_K_WCC_FINAL, DLG_K_HSET_INSERT, DLG_K_MISC, DLG_K_COUNT
+};
+const char *dlg_kernel_name(int kernel_id);
+/* Accumulated since the last reset: launches, summed duration (ms), summed
+ * algorithmic bytes.  Returns DLG_INVALID_INPUT for a bad id. */
+dlg_status dlg_kernel_stats(dlg_ctx *ctx, int kernel_id, int64_t *launches, double *total_ms,
+                            double *alg_bytes);
+dlg_status dlg_kernel_stats_reset(dlg_ctx *ctx);
+
+/* ---------------------------------------------------------- device memory */
+/* Buffers handed across the ABI.  The reference's callee-allocates /
+ * caller-frees convention (get_join returns a cudaMalloc'd buffer, freed by
+ * the caller at tc.cu:226) maps to dlg_dev_alloc / dlg_dev_free. */
+dlg_status dlg_dev_alloc(dlg_ctx *ctx, size_t bytes, void **d_ptr);
+dlg_status dlg_dev_free(dlg_ctx *ctx, void *d_ptr);
+dlg_status dlg_copy_h2d(dlg_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+dlg_status dlg_copy_d2h(dlg_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+
+/* ------------------------------------------------------- tuple conversion */
+typedef enum dlg_orient {
+    DLG_ORIENT_KV = 0,   /* tuple (col0, col1): create_entity_ar, get_int_ar_from_entity_ar   */
+    DLG_ORIENT_VK = 1    /* tuple (col1, col0): create_entity_ar_reverse, reverse_t_full,
+                            get_reverse_int_ar_from_entity_ar                                   */
+} dlg_orient;
+/* int32 rows -> tuples.  Replaces create_entity_ar / create_entity_ar_reverse
+ * (common/kernels.cu:245-265). */
+dlg_status dlg_pack_rows(dlg_ctx *ctx, const int32_t *d_rows, int64_t n, int orient,
+                         uint64_t *d_tuples);
+/* tuples -> int32 rows.  Replaces reverse_t_full / get_int_ar_from_entity_ar /
+ * get_reverse_int_ar_from_entity_ar (common/kernels.cu:268-299). */
+dlg_status dlg_unpack_rows(dlg_ctx *ctx, const uint64_t *d_tuples, int64_t n, int orient,
+                           int32_t *d_rows);
+
+/* ------------------------------------------------------------- hash table */
+/* Replaces get_hash_table(grid, block, relation, n, &rows, &secs)
+ * (common/hash_table.cu:24-55).  Builds an open-addressed table of packed
+ * 64-bit (key, row-group) slots over a copy of the relation grouped by key,
+ * linear probing, slots = next_pow2(ceil(#distinct keys / load_factor)).
+ * Multimap semantics are kept: every row of `d_rel` (duplicates included)
+ * joins once.  load_factor <= 0 selects the tuned default.  n == 0 gives an
+ * empty table whose joins are empty, as in the reference (hash_table.cu:30-35). */
+dlg_status dlg_hash_build(dlg_ctx *ctx, const uint64_t *d_rel, int64_t n, double load_factor,
+                          dlg_hash **out);
+dlg_status dlg_hash_info(const dlg_hash *h, int64_t *slots, int64_t *distinct_keys,
+                         int64_t *rows);
+void dlg_hash_free(dlg_ctx *ctx, dlg_hash *h);
+
+/* ------------------------------------------------------------------- join */
+typedef enum dlg_proj {
+    DLG_PROJ_HV_PV = 0,  /* (ht.value, probe.value): get_join_result_entity (common/join.cu:38-41) */
+    DLG_PROJ_PV_HV = 1   /* (probe.value, ht.value): the same join followed by reverse_entity_ar
+                            (sg.cu:252-253, 300-301) */
+} dlg_proj;
+/* Replaces get_join(grid, block, ht, ht_rows, probe, n, &out_n, &secs)
+ * (common/join.cu:51-87): count pass, wavefront/block prefix scan, then a
+ * load-balanced materialize pass.  *d_out is allocated by the library (NULL
+ * when *out_n == 0) and released with dlg_dev_free. */
+dlg_status dlg_join(dlg_ctx *ctx, const dlg_hash *h, const uint64_t *d_probe, int64_t n,
+                    int proj, uint64_t **d_out, int64_t *out_n);
+
+/* ------------------------------------------------- relational operators */
+/* thrust::sort(set_cmp) + thrust::unique(is_equal) (tc.cu:250-253 and every
+ * site listed in the header of engine.hip).  In place; *out_n = distinct count. */
+dlg_status dlg_sort_unique(dlg_ctx *ctx, uint64_t *d_data, int64_t n, int64_t *out_n);
+/* thrust::sort(set_cmp) alone, in place. */
+dlg_status dlg_sort(dlg_ctx *ctx, uint64_t *d_data, int64_t n);
+/* thrust::set_difference(a, b, set_cmp) of sorted duplicate-free inputs
+ * (tc.cu:260-263).  d_out may alias d_a. */
+dlg_status dlg_set_difference(dlg_ctx *ctx, const uint64_t *d_a, int64_t na, const uint64_t *d_b,
+                              int64_t nb, uint64_t *d_out, int64_t *out_n);
+/* thrust::merge(a, b, set_cmp) of sorted inputs into d_out (na + nb slots)
+ * (tc.cu:275-279).  Equal elements: a's copy first, as a stable merge does. */
+dlg_status dlg_merge(dlg_ctx *ctx, const uint64_t *d_a, int64_t na, const uint64_t *d_b,
+                     int64_t nb, uint64_t *d_out);
+/* Exclusive prefix sum of uint32 counts into uint64 offsets; *total = sum.
+ * Replaces thrust::reduce + thrust::exclusive_scan in get_join (common/join.cu:72-74). */
+dlg_status dlg_exclusive_scan(dlg_ctx *ctx, const uint32_t *d_in, int64_t n, uint64_t *d_out,
+                              uint64_t *total);
+
+/* ------------------------------------------------------- whole programs */
+/* Tuning knobs (the reference hard-codes them: block 512, grid 32*SMs,
+ * load factor 0.6 -- tc.cu:58-60, common/hash_table.cu:37).  Zero fields
+ * select the tuned defaults. */
+typedef struct dlg_opts {
+    double load_factor;      /* hash table load factor, (0, 0.95]                      */
+    int32_t block_size;      /* threads per block for the streaming kernels (64..1024) */
+    int32_t radix_bits;      /* radix digit width 4..8                                 */
+    int32_t strategy;        /* 0 auto, 1 sorted full relation (sort/diff/merge),
+                                2 hash-set full relation (insert-dedup)                */
+    int32_t max_iterations;  /* 0 = run to the fixpoint; >0 stops early (sampling)     */
+    int32_t keep_output;     /* 1: copy the output rows to host (dlg_result.rows)      */
+    int32_t reserved[6];
+} dlg_opts;
+
+/* Phase timers in the reference's CSV order (tc.cu:413-420). */
+enum {
+    DLG_PH_JOIN = 0, DLG_PH_BUFPREP, DLG_PH_COMM, DLG_PH_DEDUP, DLG_PH_MERGE, DLG_PH_INIT,
+    DLG_PH_HASHTABLE, DLG_PH_FINAL, DLG_PH_CLEAR, DLG_PH_FILEIO, DLG_PH_COUNT
+};
+
+typedef struct dlg_result {
+    int32_t iterations;      /* as the reference counts them                        */
+    int64_t output_size;     /* |TC|, |SG|, |single join output|, #components (WCC) */
+    int64_t secondary;       /* WCC: nodes in the largest component                 */
+    int64_t nodes;           /* WCC: distinct nodes                                  */
+    int64_t raw_join_rows;   /* sum of |J| over all joins (before dedup)             */
+    int64_t input_rows;
+    int32_t *rows;           /* host rows in the reference's file order, or NULL    */
+    int64_t n_rows;
+    double device_seconds;   /* upload-to-resident-result wall time on the device    */
+    double phase_seconds[DLG_PH_COUNT];
+} dlg_result;
+
+/* Whole-program entry points, the benchmark() role of tc.cu:42-423,
+ * sg.cu:40-500, wcc.cu:58-581 and single_join.cu:39-396 minus MPI and file
+ * I/O.  Input rows are host int32 pairs (the reference's file layout); the
+ * *_device variants take rows already resident in device memory and time
+ * only device work.  Results go to `res`; free with dlg_result_free. */
+dlg_status dlg_run_tc(dlg_ctx *ctx, const int32_t *h_rows, int64_t n, const dlg_opts *o,
+                      dlg_result *res);
+dlg_status dlg_run_sg(dlg_ctx *ctx, const int32_t *h_rows, int64_t n, const dlg_opts *o,
+                      dlg_result *res);
+dlg_status dlg_run_wcc(dlg_ctx *ctx, const int32_t *h_rows, int64_t n, const dlg_opts *o,
+                       dlg_result *res);
+dlg_status dlg_run_single_join(dlg_ctx *ctx, const int32_t *h_rows, int64_t n,
+                               const dlg_opts *o, dlg_result *res);
+dlg_status dlg_run_tc_device(dlg_ctx *ctx, const int32_t *d_rows, int64_t n, const dlg_opts *o,
+                             dlg_result *res);
+dlg_status dlg_run_sg_device(dlg_ctx *ctx, const int32_t *d_rows, int64_t n, const dlg_opts *o,
+                             dlg_result *res);
+dlg_status dlg_run_wcc_device(dlg_ctx *ctx, const int32_t *d_rows, int64_t n, const dlg_opts *o,
+                              dlg_result *res);
+dlg_status dlg_run_single_join_device(dlg_ctx *ctx, const int32_t *d_rows, int64_t n,
+                                      const dlg_opts *o, dlg_result *res);
+void dlg_result_free(dlg_result *res);
+
+/* ------------------------------------------------------- input formats */
+/* Replaces parallel_read (common/parallel_io.cu:5-41) on one rank: raw
+ * little-endian int32 pairs, rows = filesize / 8 (trailing bytes ignored).
+ * *h_rows is malloc'd; free with dlg_host_free. */
+dlg_status dlg_read_rows(const char *path, int32_t **h_rows, int64_t *n);
+/* Replaces parallel_write (common/parallel_io.cu:110-126) on one rank; the
+ * file is truncated first (the reference never truncates, so a stale longer
+ * file kept its tail). */
+dlg_status dlg_write_rows(const char *path, const int32_t *h_rows, int64_t n);
+/* Replaces parallel_generate (common/parallel_io.cu:79-108) on rank 0:
+ * srand(1); v[i] = rand() % range + 1 for i < 2*rows. */
+dlg_status dlg_generate_rand_rows(int64_t rows, int32_t range, int32_t **h_rows);
+void dlg_host_free(void *p);
+
+/* Synthetic workloads (deterministic: SplitMix64-seeded PCG32, seed per
+ * instance).  All return malloc'd int32 rows freed with dlg_host_free. */
+/* Directed graph with power-law out-degree P(d) ~ d^-alpha, d in [1, n-1],
+ * uniform destinations, no self-loops, exactly m distinct edges. */
+dlg_status dlg_gen_powerlaw(int32_t n, int64_t m, double alpha, uint64_t seed,
+                            int32_t **h_rows, int64_t *n_rows);
+/* side x side grid, each right/down lattice edge kept with probability p and
+ * stored in both directions, plus round(shortcut_frac * kept) shortcut edges
+ * between vertices within Manhattan distance max_dist (both directions). */
+dlg_status dlg_gen_grid(int32_t side, double p, double shortcut_frac, int32_t max_dist,
+                        uint64_t seed, int32_t **h_rows, int64_t *n_rows);
+/* cells^3 structured hex mesh, each cell split into 5 tetrahedra with
+ * parity-alternating orientation; tetrahedron edges directed low -> high id. */
+dlg_status dlg_gen_tetmesh(int32_t cells, int32_t **h_rows, int64_t *n_rows);
+/* Erdos-Renyi directed graph: n vertices, m distinct edges, no self-loops. */
+dlg_status dlg_gen_erdos_renyi(int32_t n, int64_t m, uint64_t seed, int32_t **h_rows,
+                               int64_t *n_rows);
+/* R-MAT(a, b, c, 1-a-b-c) on 2^scale vertices, m distinct edges, no self-loops
+ * (when `undirected`, each kept edge is also stored reversed). */
+dlg_status dlg_gen_rmat(int32_t scale, int64_t m, double a, double b, double c, uint64_t seed,
+                        int undirected, int32_t **h_rows, int64_t *n_rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DLG_H */

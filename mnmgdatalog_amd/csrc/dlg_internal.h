@@ -1,0 +1,321 @@
+// dlg_internal.h -- runtime shared by the HIP kernels and the C ABI of libdlg.so.
+//
+// One dlg_ctx = one device, one HIP stream (default priority), one caching
+// device-memory pool and an optional per-launch event profiler.  Errors are
+// C++ exceptions inside the library and become dlg_status at the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/dlg.h"
+
+namespace dlg {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+typedef int64_t i64;
+
+constexpr int WAVE = 64;          // CDNA wavefront width
+constexpr u64 EMPTY_SLOT = ~0ull; // hash-table empty marker
+
+struct Error {
+    dlg_status status;
+    std::string msg;
+};
+
+[[noreturn]] void fail(dlg_status s, const std::string &msg);
+void check_hip(hipError_t e, const char *what, const char *file, int line);
+#define HIPCHK(call) ::dlg::check_hip((call), #call, __FILE__, __LINE__)
+
+// ------------------------------------------------------------------ pool
+// Caching allocator.  Sizes round up to 2^k * {1, 1.25, 1.5, 1.75} (waste
+// under 25 %); freed blocks are kept per size class and reused by later
+// requests on the same stream, so fixpoint loops do no hipMalloc once warm.
+class Pool {
+public:
+    explicit Pool(int device) : device_(device) {}
+    ~Pool();
+    void *get(size_t bytes);
+    void put(void *p);
+    void trim();                      // release cached blocks to the runtime
+    size_t live_bytes() const { return live_bytes_; }
+    size_t cached_bytes() const { return cached_bytes_; }
+    size_t peak_bytes() const { return peak_bytes_; }
+    void set_budget(size_t b) { budget_ = b; }   // 0 = no limit besides the device
+    size_t budget() const { return budget_; }
+    static size_t round_size(size_t bytes);
+
+private:
+    int device_;
+    std::map<size_t, std::vector<void *>> free_;
+    std::unordered_map<void *, size_t> live_;
+    size_t live_bytes_ = 0, cached_bytes_ = 0, peak_bytes_ = 0, budget_ = 0;
+};
+
+// ------------------------------------------------------------- profiler
+struct KStat {
+    i64 launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+
+struct ProfRec {
+    int kid;
+    hipEvent_t a, b;
+    double bytes;
+};
+
+}  // namespace dlg
+
+struct dlg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dlg::Pool *pool = nullptr;
+    std::string last_error;
+    int num_cus = 256;
+    int64_t hbm_bytes = 0;
+    int lds_per_block = 65536;
+    std::string arch;
+    // tuned defaults
+    double load_factor = 0.5;
+    int block_size = 256;
+    int radix_bits = 8;
+    // profiler
+    bool prof = false;
+    dlg::KStat kstat[DLG_K_COUNT];
+    std::vector<dlg::ProfRec> pending;
+    std::vector<hipEvent_t> event_free;
+    hipEvent_t cur_a = nullptr;
+    // pinned host scalar area for size read-backs
+    uint64_t *h_scalars = nullptr;
+
+    void prof_begin();
+    void prof_end(int kid, double bytes);
+    void prof_flush();
+    void sync();
+    // read `count` u64 scalars from device memory into host (synchronises)
+    void read_scalars(const uint64_t *d, int count, uint64_t *h);
+};
+
+namespace dlg {
+
+// Device buffer owned by a context's pool.  Move-only.
+template <typename T>
+struct DBuf {
+    dlg_ctx *ctx = nullptr;
+    T *p = nullptr;
+    size_t cap = 0;  // elements
+
+    DBuf() = default;
+    explicit DBuf(dlg_ctx *c) : ctx(c) {}
+    DBuf(dlg_ctx *c, size_t n) : ctx(c) { ensure(n); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : ctx(o.ctx), p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            ctx = o.ctx; p = o.p; cap = o.cap;
+            o.p = nullptr; o.cap = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    // Make room for n elements; contents are NOT preserved when it grows.
+    void ensure(size_t n) {
+        if (n <= cap && p) return;
+        release();
+        size_t bytes = (n ? n : 1) * sizeof(T);
+        p = static_cast<T *>(ctx->pool->get(bytes));
+        cap = Pool::round_size(bytes) / sizeof(T);
+    }
+    void release() {
+        if (p && ctx) ctx->pool->put(p);
+        p = nullptr;
+        cap = 0;
+    }
+    void swap(DBuf &o) {
+        std::swap(ctx, o.ctx); std::swap(p, o.p); std::swap(cap, o.cap);
+    }
+};
+
+// ------------------------------------------------------------- launches
+inline unsigned grid_for(u64 n, u64 per_block) {
+    u64 g = (n + per_block - 1) / per_block;
+    return (unsigned)(g ? g : 1);
+}
+
+#define DLG_LAUNCH(ctx, kid, bytes, kern, grid, block, shmem, ...)                        \
+    do {                                                                                  \
+        if ((ctx)->prof) (ctx)->prof_begin();                                             \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shmem, (ctx)->stream, __VA_ARGS__); \
+        HIPCHK(hipGetLastError());                                                        \
+        if ((ctx)->prof) (ctx)->prof_end(kid, (double)(bytes));                           \
+    } while (0)
+
+// ------------------------------------------------------------- device helpers
+__device__ __forceinline__ u32 fmix32(u32 h) {
+    // MurmurHash3 finaliser (Austin Appleby, public domain), unsigned form.
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+__device__ __forceinline__ u32 tkey(u64 t) { return (u32)(t >> 32); }
+__device__ __forceinline__ u32 tval(u64 t) { return (u32)t; }
+__device__ __forceinline__ u64 tmake(u32 k, u32 v) { return ((u64)k << 32) | v; }
+__device__ __forceinline__ u64 lanemask_lt() {
+    return (1ull << (threadIdx.x & (WAVE - 1))) - 1ull;
+}
+
+// Block-wide exclusive scan (blockDim.x a multiple of 64, <= 1024).
+// `scratch` holds >= blockDim.x/64 + 1 elements of T in LDS.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T *scratch, T *total) {
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    const int nw = blockDim.x / WAVE;
+    T inc = v;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        T t = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc += t;
+    }
+    if (lane == WAVE - 1) scratch[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T s = 0;
+        for (int w = 0; w < nw; w++) {
+            T c = scratch[w];
+            scratch[w] = s;
+            s += c;
+        }
+        scratch[nw] = s;
+    }
+    __syncthreads();
+    T r = scratch[wave] + inc - v;
+    if (total) *total = scratch[nw];
+    __syncthreads();
+    return r;
+}
+
+// Binary searches over sorted u64 arrays in global memory.
+__device__ __forceinline__ u64 lower_bound_u64(const u64 *a, u64 lo, u64 hi, u64 x) {
+    while (lo < hi) {
+        u64 mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ u64 upper_bound_u64(const u64 *a, u64 lo, u64 hi, u64 x) {
+    while (lo < hi) {
+        u64 mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------- primitives (k_prims.hip)
+// Exclusive scan of u32 counts into u64 offsets; returns the total (synchronises).
+u64 scan_u32_u64(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out);
+// Same without reading the total back; total left at d_total (device).
+void scan_u32_u64_async(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *d_total);
+// Stable LSD radix sort of u64 keys; result in `data`.  `tmp` is scratch of >= n.
+void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits = 0);
+// Compaction modes.
+enum SelectMode {
+    SEL_UNIQUE = 0,         // keep first of each run of equal elements
+    SEL_UNIQUE_NODIAG = 1,  // ... and drop key == value
+    SEL_UNIQUE_NOT_IN = 2,  // ... and drop elements present in sorted `other`
+    SEL_NOT_IN = 3,         // drop elements present in sorted `other`
+    SEL_KEY_HEAD = 4,       // output INDEX of first element of each equal-key run
+};
+// Writes kept elements (or indices for SEL_KEY_HEAD, as u32) to out; returns count.
+u64 select(dlg_ctx *ctx, int mode, const u64 *in, u64 n, const u64 *other, u64 n_other,
+           void *out);
+// Merge of sorted a, b into out (a first on ties).
+void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out);
+void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out);
+void unpack_rows(dlg_ctx *ctx, const u64 *t, u64 n, int orient, int32_t *rows);
+void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out);  // 2n tuples: kv, vk
+void fill_u64(dlg_ctx *ctx, u64 *p, u64 n, u64 v);
+void fill_u32(dlg_ctx *ctx, u32 *p, u64 n, u32 v);
+void copy_u64(dlg_ctx *ctx, u64 *dst, const u64 *src, u64 n);
+
+// ------------------------------------------------------------- index + join (k_join.hip)
+struct Index {
+    DBuf<u64> rows;      // relation sorted by (key, value); duplicates kept
+    DBuf<u32> gstart;    // start row of each key group, plus gstart[G] = n
+    DBuf<u64> slots;     // open-addressed (key << 32 | group) slots, EMPTY_SLOT = free
+    u64 n = 0, groups = 0, nslots = 0;
+    explicit Index(dlg_ctx *c) : rows(c), gstart(c), slots(c) {}
+};
+// Build from a device relation (copied); `presorted` skips the sort.
+void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double load_factor,
+                 bool presorted);
+// Count phase: per probe, the matching group's row start and count.  Returns |J|.
+struct JoinScratch {
+    DBuf<u32> cnt, pstart;
+    DBuf<u64> offs;
+    explicit JoinScratch(dlg_ctx *c) : cnt(c), pstart(c), offs(c) {}
+};
+u64 join_count(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js);
+void join_expand(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, const JoinScratch &js,
+                 u64 total, int proj, u64 *out);
+// Dense lookup of every value of `ix.rows` as a key: out[i] = group id (u32) or ~0u.
+void index_lookup_values(dlg_ctx *ctx, const Index &ix, u32 *out);
+
+// ------------------------------------------------------------- WCC (k_wcc.hip)
+struct WccState {
+    u64 V = 0;
+    DBuf<u32> L, mark, fnode, flabel, nnode, cnt, pstart, counter;
+    DBuf<u64> offs;
+    explicit WccState(dlg_ctx *c)
+        : L(c), mark(c), fnode(c), flabel(c), nnode(c), cnt(c), pstart(c), counter(c), offs(c) {}
+};
+void wcc_setup(dlg_ctx *ctx, WccState &s, u64 V);
+// One round over frontier of size nf; returns the new frontier size.
+u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf, u32 round);
+// #components, largest component size and the (label, 0) tuples in `out`.
+u64 wcc_finalize(dlg_ctx *ctx, WccState &s, const Index &ix, DBuf<u64> &out, i64 *largest);
+
+}  // namespace dlg
+
+// Exception -> dlg_status translation used by every C ABI entry point.
+#define API_BEGIN(ctx)       \
+    try {                    \
+        if (ctx) (void)hipSetDevice((ctx)->device);
+#define API_END(ctx)                                                   \
+    }                                                                  \
+    catch (const dlg::Error &e) {                                      \
+        if (ctx) (ctx)->last_error = e.msg;                            \
+        return e.status;                                               \
+    }                                                                  \
+    catch (const std::bad_alloc &) {                                   \
+        if (ctx) (ctx)->last_error = "host allocation failed";         \
+        return DLG_OUT_OF_MEMORY;                                      \
+    }                                                                  \
+    return DLG_OK;
+// Same, running `cleanup` before an error status is returned.
+#define API_END_CLEANUP(ctx, cleanup)                                  \
+    }                                                                  \
+    catch (const dlg::Error &e) {                                      \
+        cleanup;                                                       \
+        if (ctx) (ctx)->last_error = e.msg;                            \
+        return e.status;                                               \
+    }                                                                  \
+    catch (const std::bad_alloc &) {                                   \
+        cleanup;                                                       \
+        if (ctx) (ctx)->last_error = "host allocation failed";         \
+        return DLG_OUT_OF_MEMORY;                                      \
+    }                                                                  \
+    return DLG_OK;
+

@@ -1,0 +1,525 @@
+// engine.hip -- semi-naive fixpoint loops (TC, SG, WCC, single join) and the
+// operator-level C ABI of libdlg.so.
+//
+// Every relation is a device array of packed u64 tuples (see dlg.h); all
+// work runs on the context's stream; buffers come from the context's pool.
+// The loops keep the reference's relational steps and iteration counting:
+//   TC  tc.cu:132-314    Δ0 = unique(E reversed); F = Δ0; repeat
+//                        { J = Δ ⋈ E; N = unique(J) \ F; F = F ∪ N; it++ } until N = ∅
+//   SG  sg.cu:117-390    base = unique(E ⋈ E) minus (x,x); repeat
+//                        { T = unique(Δ ⋈ E reversed); J = T ⋈ E reversed; N = unique(J) \ F; ... }
+//   WCC wcc.cu:140-485   min-label propagation, stop when two consecutive |Δ| are equal
+//   SJ  single_join.cu:100-283  unique((R⁻¹) ⋈ R) over deduplicated R, R⁻¹
+// Full relation F: sorted array; N is found by one fused compaction
+// ("first of an equal run AND not found in F"), then F' = merge(F, N).
+#include "dlg_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+
+namespace dlg {
+
+// Phase timing with events on the context stream: mark(p) starts phase p.
+struct PhaseClock {
+    dlg_ctx *ctx;
+    std::vector<std::pair<int, hipEvent_t>> marks;
+    explicit PhaseClock(dlg_ctx *c) : ctx(c) {}
+    ~PhaseClock() {
+        for (auto &m : marks) ctx->event_free.push_back(m.second);
+    }
+    void mark(int ph) {
+        if (ctx->event_free.empty()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            ctx->event_free.push_back(e);
+        }
+        hipEvent_t e = ctx->event_free.back();
+        ctx->event_free.pop_back();
+        HIPCHK(hipEventRecord(e, ctx->stream));
+        marks.push_back({ph, e});
+    }
+    void finish(double *out) {
+        mark(-1);
+        HIPCHK(hipEventSynchronize(marks.back().second));
+        for (size_t i = 1; i < marks.size(); i++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, marks[i - 1].second, marks[i].second));
+            if (marks[i - 1].first >= 0) out[marks[i - 1].first] += ms * 1e-3;
+        }
+    }
+};
+
+struct Opts {
+    double lf;
+    int bits;
+    int max_it;
+    bool keep;
+};
+
+static Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
+    Opts r{ctx->load_factor, ctx->radix_bits, 0, true};
+    if (o) {
+        if (o->load_factor > 0) r.lf = o->load_factor;
+        if (o->radix_bits > 0) r.bits = o->radix_bits;
+        r.max_it = o->max_iterations;
+        r.keep = o->keep_output != 0;
+        if (o->load_factor < 0 || o->load_factor > 0.95) fail(DLG_INVALID_INPUT, "load_factor out of (0, 0.95]");
+        if (o->radix_bits != 0 && (o->radix_bits < 4 || o->radix_bits > 8))
+            fail(DLG_INVALID_INPUT, "radix_bits out of 4..8");
+    }
+    return r;
+}
+
+// sort + unique of buf[0..n) into out; returns distinct count.  buf is clobbered.
+static u64 sort_unique_into(dlg_ctx *ctx, DBuf<u64> &buf, u64 n, DBuf<u64> &tmp, DBuf<u64> &out,
+                            int mode, const u64 *other, u64 n_other, int bits) {
+    if (n == 0) return 0;
+    radix_sort(ctx, buf.p, n, tmp, bits);
+    out.ensure(n);
+    return select(ctx, mode, buf.p, n, other, n_other, out.p);
+}
+
+// J = probe ⋈ index into `out` (resized); returns |J|.
+static u64 join_into(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, int proj,
+                     JoinScratch &js, DBuf<u64> &out) {
+    u64 total = join_count(ctx, ix, probe, n, js);
+    if (total) {
+        out.ensure(total);
+        join_expand(ctx, ix, probe, n, js, total, proj, out.p);
+    }
+    return total;
+}
+
+static void download_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, dlg_result *res) {
+    res->n_rows = (int64_t)n;
+    res->rows = (int32_t *)std::malloc(sizeof(int32_t) * 2 * (n ? n : 1));
+    if (!res->rows) fail(DLG_OUT_OF_MEMORY, "host allocation of result rows");
+    if (!n) return;
+    DBuf<int32_t> r(ctx, 2 * n);
+    unpack_rows(ctx, d, n, orient, r.p);
+    HIPCHK(hipMemcpyAsync(res->rows, r.p, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+}
+
+static void check_n(int64_t n, const void *rows) {
+    if (n < 0) fail(DLG_INVALID_INPUT, "negative row count");
+    if (n > 0 && !rows) fail(DLG_INVALID_INPUT, "null rows with n > 0");
+    if ((u64)n >= 0x7fffffffull) fail(DLG_OVERFLOW, "more than 2^31-2 input rows");
+}
+
+typedef std::chrono::steady_clock Clock;
+static double secs(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+}
+
+// ------------------------------------------------------------------ TC
+void run_tc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+    PhaseClock pc(ctx);
+    auto t0 = Clock::now();
+    pc.mark(DLG_PH_INIT);
+    DBuf<u64> edge(ctx, n), delta(ctx, n), J(ctx, n), full(ctx), full2(ctx), N(ctx), tmp(ctx);
+    pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, edge.p);   // e(src, dst) keyed by src
+    pack_rows(ctx, d_rows, n, DLG_ORIENT_VK, J.p);      // Δ0 rows (dst, src)
+    pc.mark(DLG_PH_HASHTABLE);
+    Index ix(ctx);
+    index_build(ctx, ix, edge.p, n, o.lf, false);      // duplicates kept, as the reference
+    edge.release();
+    pc.mark(DLG_PH_DEDUP);
+    u64 nd = sort_unique_into(ctx, J, n, tmp, delta, SEL_UNIQUE, nullptr, 0, o.bits);
+    pc.mark(DLG_PH_MERGE);
+    full.ensure(nd);
+    copy_u64(ctx, full.p, delta.p, nd);
+    u64 nfull = nd;
+    JoinScratch js(ctx);
+    int it = 0;
+    i64 raw = 0;
+    while (true) {
+        pc.mark(DLG_PH_JOIN);
+        u64 nj = join_into(ctx, ix, delta.p, nd, DLG_PROJ_HV_PV, js, J);   // (z, x)
+        raw += (i64)nj;
+        pc.mark(DLG_PH_DEDUP);
+        u64 nn = 0;
+        if (nj) {
+            radix_sort(ctx, J.p, nj, tmp, o.bits);
+            pc.mark(DLG_PH_MERGE);
+            N.ensure(nj);
+            nn = select(ctx, SEL_UNIQUE_NOT_IN, J.p, nj, full.p, nfull, N.p);
+        }
+        pc.mark(DLG_PH_MERGE);
+        if (nn) {
+            full2.ensure(nfull + nn);
+            merge(ctx, full.p, nfull, N.p, nn, full2.p);
+            full.swap(full2);
+            nfull += nn;
+        }
+        it++;
+        if (nn == 0) break;                         // |F| unchanged (tc.cu:294-300)
+        delta.swap(N);
+        nd = nn;
+        if (o.max_it > 0 && it >= o.max_it) break;
+    }
+    pc.mark(DLG_PH_FINAL);
+    ctx->sync();
+    res->device_seconds = secs(t0, Clock::now());
+    res->iterations = it;
+    res->output_size = (int64_t)nfull;
+    res->raw_join_rows = raw;
+    if (o.keep) download_rows(ctx, full.p, nfull, DLG_ORIENT_VK, res);   // (src, dst)
+    pc.finish(res->phase_seconds);
+}
+
+// ------------------------------------------------------------------ SG
+void run_sg(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+    PhaseClock pc(ctx);
+    auto t0 = Clock::now();
+    pc.mark(DLG_PH_INIT);
+    DBuf<u64> edge(ctx), J(ctx, n), T(ctx), delta(ctx), full(ctx), full2(ctx), N(ctx), tmp(ctx);
+    pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, J.p);
+    pc.mark(DLG_PH_DEDUP);
+    u64 ne = sort_unique_into(ctx, J, n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
+    pc.mark(DLG_PH_HASHTABLE);
+    Index ix(ctx);
+    index_build(ctx, ix, edge.p, ne, o.lf, true);
+    JoinScratch js(ctx);
+    i64 raw = 0;
+    pc.mark(DLG_PH_JOIN);
+    u64 nb = join_into(ctx, ix, edge.p, ne, DLG_PROJ_HV_PV, js, J);   // (y, x)
+    raw += (i64)nb;
+    pc.mark(DLG_PH_DEDUP);
+    u64 nfull = sort_unique_into(ctx, J, nb, tmp, full, SEL_UNIQUE_NODIAG, nullptr, 0, o.bits);
+    pc.mark(DLG_PH_MERGE);
+    delta.ensure(nfull);
+    copy_u64(ctx, delta.p, full.p, nfull);
+    u64 nd = nfull;
+    int it = 0;
+    while (true) {
+        pc.mark(DLG_PH_JOIN);
+        u64 n1 = join_into(ctx, ix, delta.p, nd, DLG_PROJ_PV_HV, js, J);  // (b, x)
+        raw += (i64)n1;
+        pc.mark(DLG_PH_DEDUP);
+        u64 nt = sort_unique_into(ctx, J, n1, tmp, T, SEL_UNIQUE, nullptr, 0, o.bits);
+        pc.mark(DLG_PH_JOIN);
+        u64 n2 = join_into(ctx, ix, T.p, nt, DLG_PROJ_PV_HV, js, J);      // (x, y)
+        raw += (i64)n2;
+        pc.mark(DLG_PH_DEDUP);
+        u64 nn = 0;
+        if (n2) {
+            radix_sort(ctx, J.p, n2, tmp, o.bits);
+            pc.mark(DLG_PH_MERGE);
+            N.ensure(n2);
+            nn = select(ctx, SEL_UNIQUE_NOT_IN, J.p, n2, full.p, nfull, N.p);
+        }
+        pc.mark(DLG_PH_MERGE);
+        if (nn) {
+            full2.ensure(nfull + nn);
+            merge(ctx, full.p, nfull, N.p, nn, full2.p);
+            full.swap(full2);
+            nfull += nn;
+        }
+        it++;
+        if (nn == 0) break;                         // sg.cu:376-389
+        delta.swap(N);
+        nd = nn;
+        if (o.max_it > 0 && it >= o.max_it) break;
+    }
+    pc.mark(DLG_PH_FINAL);
+    ctx->sync();
+    res->device_seconds = secs(t0, Clock::now());
+    res->iterations = it;
+    res->output_size = (int64_t)nfull;
+    res->raw_join_rows = raw;
+    if (o.keep) download_rows(ctx, full.p, nfull, DLG_ORIENT_KV, res);   // (x, y)
+    pc.finish(res->phase_seconds);
+}
+
+// ------------------------------------------------------------------ single join
+void run_sj(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+    PhaseClock pc(ctx);
+    auto t0 = Clock::now();
+    pc.mark(DLG_PH_INIT);
+    DBuf<u64> a(ctx, n), R(ctx), Rr(ctx), J(ctx), U(ctx), tmp(ctx);
+    pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, a.p);
+    pc.mark(DLG_PH_DEDUP);
+    u64 nr = sort_unique_into(ctx, a, n, tmp, R, SEL_UNIQUE, nullptr, 0, o.bits);
+    pc.mark(DLG_PH_INIT);
+    pack_rows(ctx, d_rows, n, DLG_ORIENT_VK, a.p);
+    pc.mark(DLG_PH_DEDUP);
+    u64 nrr = sort_unique_into(ctx, a, n, tmp, Rr, SEL_UNIQUE, nullptr, 0, o.bits);
+    a.release();
+    pc.mark(DLG_PH_HASHTABLE);
+    Index ix(ctx);
+    index_build(ctx, ix, R.p, nr, o.lf, true);
+    pc.mark(DLG_PH_JOIN);
+    JoinScratch js(ctx);
+    u64 nj = join_into(ctx, ix, Rr.p, nrr, DLG_PROJ_HV_PV, js, J);   // (c, a)
+    pc.mark(DLG_PH_DEDUP);
+    u64 nu = sort_unique_into(ctx, J, nj, tmp, U, SEL_UNIQUE, nullptr, 0, o.bits);
+    pc.mark(DLG_PH_FINAL);
+    ctx->sync();
+    res->device_seconds = secs(t0, Clock::now());
+    res->iterations = 1;
+    res->output_size = (int64_t)nu;
+    res->raw_join_rows = (i64)nj;
+    if (o.keep) download_rows(ctx, U.p, nu, DLG_ORIENT_VK, res);   // (a, c)
+    pc.finish(res->phase_seconds);
+}
+
+// ------------------------------------------------------------------ WCC
+void run_wcc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+    PhaseClock pc(ctx);
+    auto t0 = Clock::now();
+    pc.mark(DLG_PH_INIT);
+    DBuf<u64> sym(ctx, 2 * n), edge(ctx), tmp(ctx), out(ctx);
+    pack_sym_rows(ctx, d_rows, n, sym.p);                  // E ∪ E⁻¹ (wcc.cu:140-160)
+    pc.mark(DLG_PH_DEDUP);
+    u64 ne = sort_unique_into(ctx, sym, 2 * n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
+    sym.release();
+    tmp.release();
+    pc.mark(DLG_PH_HASHTABLE);
+    Index ix(ctx);
+    index_build(ctx, ix, edge.p, ne, o.lf, true);
+    edge.release();
+    u64 V = ix.groups;
+    DBuf<u32> nbr(ctx, ne ? ne : 1);
+    index_lookup_values(ctx, ix, nbr.p);                   // neighbour dense ids
+    pc.mark(DLG_PH_INIT);
+    WccState s(ctx);
+    wcc_setup(ctx, s, V);
+    u64 nf = V;
+    u64 prev = V;                                          // wcc.cu:236-242
+    int it = 0;
+    while (true) {
+        pc.mark(DLG_PH_JOIN);
+        u64 cur = nf ? wcc_round(ctx, s, ix, nbr.p, nf, (u32)(it + 1)) : 0;
+        it++;
+        nf = cur;
+        if (cur == prev) break;                            // wcc.cu:366-383
+        prev = cur;
+        if (o.max_it > 0 && it >= o.max_it) break;
+    }
+    pc.mark(DLG_PH_FINAL);
+    i64 largest = -1;
+    u64 ncomp = wcc_finalize(ctx, s, ix, out, &largest);
+    ctx->sync();
+    res->device_seconds = secs(t0, Clock::now());
+    res->iterations = it;
+    res->output_size = (int64_t)ncomp;
+    res->secondary = largest;
+    res->nodes = (int64_t)V;
+    if (o.keep) download_rows(ctx, out.p, ncomp, DLG_ORIENT_KV, res);   // (label, 0)
+    pc.finish(res->phase_seconds);
+}
+
+}  // namespace dlg
+
+// ====================================================================== C ABI
+using namespace dlg;
+
+struct dlg_hash {
+    Index ix;
+    explicit dlg_hash(dlg_ctx *c) : ix(c) {}
+};
+
+typedef void (*runner_t)(dlg_ctx *, const int32_t *, u64, const Opts &, dlg_result *);
+
+static dlg_status run_device(dlg_ctx *ctx, runner_t fn, const int32_t *d_rows, int64_t n,
+                             const dlg_opts *o, dlg_result *res) {
+    if (!ctx || !res) return DLG_INVALID_INPUT;
+    std::memset(res, 0, sizeof(*res));
+    API_BEGIN(ctx)
+    check_n(n, d_rows);
+    Opts op = read_opts(ctx, o);
+    res->input_rows = n;
+    fn(ctx, d_rows, (u64)n, op, res);
+    API_END(ctx)
+}
+
+static dlg_status run_host(dlg_ctx *ctx, runner_t fn, const int32_t *h_rows, int64_t n,
+                           const dlg_opts *o, dlg_result *res) {
+    if (!ctx || !res) return DLG_INVALID_INPUT;
+    std::memset(res, 0, sizeof(*res));
+    API_BEGIN(ctx)
+    check_n(n, h_rows);
+    Opts op = read_opts(ctx, o);
+    res->input_rows = n;
+    auto t0 = Clock::now();
+    DBuf<int32_t> d(ctx, 2 * (size_t)(n ? n : 1));
+    if (n)
+        HIPCHK(hipMemcpyAsync(d.p, h_rows, sizeof(int32_t) * 2 * (size_t)n, hipMemcpyHostToDevice,
+                              ctx->stream));
+    ctx->sync();
+    double up = secs(t0, Clock::now());
+    fn(ctx, d.p, (u64)n, op, res);
+    res->phase_seconds[DLG_PH_INIT] += up;
+    API_END(ctx)
+}
+
+extern "C" {
+
+dlg_status dlg_run_tc(dlg_ctx *c, const int32_t *h, int64_t n, const dlg_opts *o, dlg_result *r) {
+    return run_host(c, run_tc, h, n, o, r);
+}
+dlg_status dlg_run_sg(dlg_ctx *c, const int32_t *h, int64_t n, const dlg_opts *o, dlg_result *r) {
+    return run_host(c, run_sg, h, n, o, r);
+}
+dlg_status dlg_run_wcc(dlg_ctx *c, const int32_t *h, int64_t n, const dlg_opts *o, dlg_result *r) {
+    return run_host(c, run_wcc, h, n, o, r);
+}
+dlg_status dlg_run_single_join(dlg_ctx *c, const int32_t *h, int64_t n, const dlg_opts *o,
+                               dlg_result *r) {
+    return run_host(c, run_sj, h, n, o, r);
+}
+dlg_status dlg_run_tc_device(dlg_ctx *c, const int32_t *d, int64_t n, const dlg_opts *o,
+                             dlg_result *r) {
+    return run_device(c, run_tc, d, n, o, r);
+}
+dlg_status dlg_run_sg_device(dlg_ctx *c, const int32_t *d, int64_t n, const dlg_opts *o,
+                             dlg_result *r) {
+    return run_device(c, run_sg, d, n, o, r);
+}
+dlg_status dlg_run_wcc_device(dlg_ctx *c, const int32_t *d, int64_t n, const dlg_opts *o,
+                              dlg_result *r) {
+    return run_device(c, run_wcc, d, n, o, r);
+}
+dlg_status dlg_run_single_join_device(dlg_ctx *c, const int32_t *d, int64_t n, const dlg_opts *o,
+                                      dlg_result *r) {
+    return run_device(c, run_sj, d, n, o, r);
+}
+
+void dlg_result_free(dlg_result *r) {
+    if (!r) return;
+    std::free(r->rows);
+    r->rows = nullptr;
+    r->n_rows = 0;
+}
+
+dlg_status dlg_pack_rows(dlg_ctx *c, const int32_t *d_rows, int64_t n, int orient, uint64_t *d_t) {
+    if (!c || n < 0 || (n && (!d_rows || !d_t)) || (orient != 0 && orient != 1))
+        return DLG_INVALID_INPUT;
+    API_BEGIN(c)
+    pack_rows(c, d_rows, (u64)n, orient, d_t);
+    c->sync();
+    API_END(c)
+}
+
+dlg_status dlg_unpack_rows(dlg_ctx *c, const uint64_t *d_t, int64_t n, int orient, int32_t *d_rows) {
+    if (!c || n < 0 || (n && (!d_rows || !d_t)) || (orient != 0 && orient != 1))
+        return DLG_INVALID_INPUT;
+    API_BEGIN(c)
+    unpack_rows(c, d_t, (u64)n, orient, d_rows);
+    c->sync();
+    API_END(c)
+}
+
+dlg_status dlg_hash_build(dlg_ctx *c, const uint64_t *d_rel, int64_t n, double lf, dlg_hash **out) {
+    if (!c || !out || n < 0 || (n && !d_rel) || lf > 0.95) return DLG_INVALID_INPUT;
+    *out = nullptr;
+    dlg_hash *h = nullptr;
+    API_BEGIN(c)
+    h = new dlg_hash(c);
+    index_build(c, h->ix, d_rel, (u64)n, lf, false);
+    c->sync();
+    *out = h;
+    h = nullptr;
+    API_END_CLEANUP(c, delete h)
+}
+
+dlg_status dlg_hash_info(const dlg_hash *h, int64_t *slots, int64_t *keys, int64_t *rows) {
+    if (!h) return DLG_INVALID_INPUT;
+    if (slots) *slots = (int64_t)h->ix.nslots;
+    if (keys) *keys = (int64_t)h->ix.groups;
+    if (rows) *rows = (int64_t)h->ix.n;
+    return DLG_OK;
+}
+
+void dlg_hash_free(dlg_ctx *c, dlg_hash *h) {
+    if (!h) return;
+    if (c) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    delete h;
+}
+
+dlg_status dlg_join(dlg_ctx *c, const dlg_hash *h, const uint64_t *d_probe, int64_t n, int proj,
+                    uint64_t **d_out, int64_t *out_n) {
+    if (!c || !h || !d_out || !out_n || n < 0 || (n && !d_probe) ||
+        (proj != DLG_PROJ_HV_PV && proj != DLG_PROJ_PV_HV))
+        return DLG_INVALID_INPUT;
+    *d_out = nullptr;
+    *out_n = 0;
+    API_BEGIN(c)
+    JoinScratch js(c);
+    u64 total = join_count(c, h->ix, d_probe, (u64)n, js);
+    if (total) {
+        u64 *o = (u64 *)c->pool->get(total * sizeof(u64));
+        join_expand(c, h->ix, d_probe, (u64)n, js, total, proj, o);
+        c->sync();
+        *d_out = o;
+    }
+    *out_n = (int64_t)total;
+    API_END(c)
+}
+
+dlg_status dlg_sort(dlg_ctx *c, uint64_t *d, int64_t n) {
+    if (!c || n < 0 || (n && !d)) return DLG_INVALID_INPUT;
+    API_BEGIN(c)
+    DBuf<u64> tmp(c);
+    radix_sort(c, d, (u64)n, tmp);
+    c->sync();
+    API_END(c)
+}
+
+dlg_status dlg_sort_unique(dlg_ctx *c, uint64_t *d, int64_t n, int64_t *out_n) {
+    if (!c || !out_n || n < 0 || (n && !d)) return DLG_INVALID_INPUT;
+    *out_n = 0;
+    API_BEGIN(c)
+    if (n) {
+        DBuf<u64> tmp(c), out(c, (size_t)n);
+        radix_sort(c, d, (u64)n, tmp);
+        u64 k = select(c, SEL_UNIQUE, d, (u64)n, nullptr, 0, out.p);
+        copy_u64(c, d, out.p, k);
+        c->sync();
+        *out_n = (int64_t)k;
+    }
+    API_END(c)
+}
+
+dlg_status dlg_set_difference(dlg_ctx *c, const uint64_t *a, int64_t na, const uint64_t *b,
+                              int64_t nb, uint64_t *out, int64_t *out_n) {
+    if (!c || !out_n || na < 0 || nb < 0 || (na && (!a || !out)) || (nb && !b))
+        return DLG_INVALID_INPUT;
+    *out_n = 0;
+    API_BEGIN(c)
+    if (na) {
+        DBuf<u64> o(c, (size_t)na);
+        u64 k = select(c, SEL_NOT_IN, a, (u64)na, b, (u64)nb, o.p);
+        copy_u64(c, out, o.p, k);
+        c->sync();
+        *out_n = (int64_t)k;
+    }
+    API_END(c)
+}
+
+dlg_status dlg_merge(dlg_ctx *c, const uint64_t *a, int64_t na, const uint64_t *b, int64_t nb,
+                     uint64_t *out) {
+    if (!c || na < 0 || nb < 0 || (na && !a) || (nb && !b) || ((na || nb) && !out))
+        return DLG_INVALID_INPUT;
+    API_BEGIN(c)
+    merge(c, a, (u64)na, b, (u64)nb, out);
+    c->sync();
+    API_END(c)
+}
+
+dlg_status dlg_exclusive_scan(dlg_ctx *c, const uint32_t *in, int64_t n, uint64_t *out,
+                              uint64_t *total) {
+    if (!c || n < 0 || (n && (!in || !out))) return DLG_INVALID_INPUT;
+    API_BEGIN(c)
+    u64 t = n ? scan_u32_u64(c, in, (u64)n, out) : 0;
+    if (total) *total = t;
+    API_END(c)
+}
+
+}  // extern "C"

@@ -1,0 +1,177 @@
+// k_join.hip -- hash index build and the two-pass hash join.
+//
+// Reference roles: get_hash_table + build_hash_table_entity
+// (common/hash_table.cu:1-55) and get_join + get_join_result_size_entity +
+// get_join_result_entity (common/join.cu:1-87).
+//
+// Layout in HBM (one index):
+//   rows[n]     the relation sorted by (key, value), duplicates kept (multimap)
+//   gstart[G+1] first row of each distinct key ("group"), gstart[G] = n
+//   slots[S]    open-addressed table, S = next_pow2(ceil(G / load_factor)),
+//               slot = (key << 32) | group, linear probing, EMPTY = ~0
+// The reference stores one (key, value) slot per ROW and walks the whole run
+// of equal keys in both join passes; here one slot per distinct KEY is found
+// once per probe and the matching values are a contiguous range of `rows`,
+// so the materialize pass reads them coalesced and the multiset of join rows
+// is exactly the reference's (one output per relation row with an equal key).
+//
+// Join, per probe tuple (k, v):
+//   count   : slot lookup -> (row start, count)                 [join_count_k]
+//   offsets : exclusive scan of counts (u32 -> u64)             [scan]
+//   expand  : one thread per OUTPUT row (load-balanced): binary search of the
+//             output index in the offsets, then out = (rows[start+j].value, v)
+//             or its reverse                                     [join_expand_k]
+#include "dlg_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace dlg {
+
+__device__ __forceinline__ u32 ix_find(const u64 *slots, u64 mask, u32 key) {
+    u64 p = fmix32(key) & mask;
+    while (true) {
+        u64 s = slots[p];
+        if (s == EMPTY_SLOT) return 0xffffffffu;
+        if (tkey(s) == key) return tval(s);
+        p = (p + 1) & mask;
+    }
+}
+
+__global__ void set_last_k(u32 *gstart, u64 g, u32 n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) gstart[g] = n;
+}
+
+__global__ void hash_insert_k(const u64 *rows, const u32 *gstart, u64 G, u64 *slots, u64 mask) {
+    u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; g < G; g += stride) {
+        u32 key = tkey(rows[gstart[g]]);
+        u64 v = tmake(key, (u32)g);
+        u64 p = fmix32(key) & mask;
+        while (true) {
+            unsigned long long prev =
+                atomicCAS((unsigned long long *)&slots[p], (unsigned long long)EMPTY_SLOT,
+                          (unsigned long long)v);
+            if (prev == EMPTY_SLOT) break;
+            p = (p + 1) & mask;
+        }
+    }
+}
+
+void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double lf, bool presorted) {
+    ix.n = n;
+    ix.groups = 0;
+    ix.nslots = 0;
+    if (n == 0) return;
+    if (n >= 0xffffffffull) fail(DLG_OVERFLOW, "index: relation larger than 2^32-2 rows");
+    if (!(lf > 0.0)) lf = ctx->load_factor;
+    lf = std::min(0.95, std::max(0.05, lf));
+    ix.rows.ensure(n);
+    if (rel != ix.rows.p) copy_u64(ctx, ix.rows.p, rel, n);
+    if (!presorted) {
+        DBuf<u64> tmp(ctx);
+        radix_sort(ctx, ix.rows.p, n, tmp);
+    }
+    ix.gstart.ensure(n + 1);
+    u64 G = select(ctx, SEL_KEY_HEAD, ix.rows.p, n, nullptr, 0, ix.gstart.p);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0, set_last_k, 1, 64, 0, ix.gstart.p, G, (u32)n);
+    ix.groups = G;
+    u64 want = (u64)std::ceil((double)G / lf);
+    u64 S = 64;
+    while (S < want || S <= G) S <<= 1;
+    ix.nslots = S;
+    ix.slots.ensure(S);
+    fill_u64(ctx, ix.slots.p, S, EMPTY_SLOT);
+    unsigned g = (unsigned)std::min<u64>((G + 255) / 256, (u64)ctx->num_cus * 16);
+    DLG_LAUNCH(ctx, DLG_K_HASH_BUILD, 8.0 * S + 12.0 * G + 8.0 * G, hash_insert_k, g, 256, 0,
+               (const u64 *)ix.rows.p, (const u32 *)ix.gstart.p, G, ix.slots.p, S - 1);
+}
+
+__global__ __launch_bounds__(256) void join_count_k(const u64 *probe, u64 n, const u64 *slots,
+                                                    u64 mask, const u32 *gstart, u32 *cnt,
+                                                    u32 *pstart) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        u32 g = ix_find(slots, mask, tkey(probe[i]));
+        u32 c = 0, s = 0;
+        if (g != 0xffffffffu) {
+            s = gstart[g];
+            c = gstart[g + 1] - s;
+        }
+        cnt[i] = c;
+        pstart[i] = s;
+    }
+}
+
+u64 join_count(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js) {
+    if (n == 0 || ix.n == 0) return 0;
+    js.cnt.ensure(n);
+    js.pstart.ensure(n);
+    js.offs.ensure(n + 1);
+    unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 32);
+    // probe read + one slot line per probe + gstart pair + count/start writes
+    DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 8.0 * n + 8.0 * n + 8.0 * n, join_count_k, g, 256,
+               0, probe, n, (const u64 *)ix.slots.p, ix.nslots - 1, (const u32 *)ix.gstart.p,
+               js.cnt.p, js.pstart.p);
+    return scan_u32_u64(ctx, js.cnt.p, n, js.offs.p);
+}
+
+constexpr int EX_T = 256, EX_I = 8, EX_TILE = EX_T * EX_I;
+
+template <int PROJ>
+__global__ __launch_bounds__(EX_T) void join_expand_k(const u64 *probe, u64 np, const u32 *pstart,
+                                                      const u64 *offs, u64 total, const u64 *rows,
+                                                      u64 *out) {
+    __shared__ u64 s_lo, s_hi;
+    const u64 base = (u64)blockIdx.x * EX_TILE;
+    if (threadIdx.x == 0) {
+        u64 last = (base + EX_TILE < total ? base + EX_TILE : total) - 1;
+        u64 lo = upper_bound_u64(offs, 0, np, base) - 1;
+        s_lo = lo;
+        s_hi = upper_bound_u64(offs, lo, np, last);
+    }
+    __syncthreads();
+    const u64 lo = s_lo, hi = s_hi;
+#pragma unroll
+    for (int r = 0; r < EX_I; r++) {
+        u64 o = base + (u64)r * EX_T + threadIdx.x;
+        if (o < total) {
+            u64 i = upper_bound_u64(offs, lo, hi, o) - 1;
+            u32 hv = tval(rows[(u64)pstart[i] + (o - offs[i])]);
+            u32 pv = tval(probe[i]);
+            out[o] = PROJ == DLG_PROJ_HV_PV ? tmake(hv, pv) : tmake(pv, hv);
+        }
+    }
+}
+
+void join_expand(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, const JoinScratch &js,
+                 u64 total, int proj, u64 *out) {
+    if (total == 0) return;
+    unsigned g = grid_for(total, EX_TILE);
+    double bytes = 24.0 * total;  // relation value read, output write, probe/offset reads amortised
+    if (proj == DLG_PROJ_PV_HV)
+        DLG_LAUNCH(ctx, DLG_K_JOIN_EXPAND, bytes, join_expand_k<DLG_PROJ_PV_HV>, g, EX_T, 0, probe,
+                   n, (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total,
+                   (const u64 *)ix.rows.p, out);
+    else
+        DLG_LAUNCH(ctx, DLG_K_JOIN_EXPAND, bytes, join_expand_k<DLG_PROJ_HV_PV>, g, EX_T, 0, probe,
+                   n, (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total,
+                   (const u64 *)ix.rows.p, out);
+}
+
+__global__ void lookup_values_k(const u64 *rows, u64 n, const u64 *slots, u64 mask, u32 *out) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = ix_find(slots, mask, tval(rows[i]));
+}
+
+void index_lookup_values(dlg_ctx *ctx, const Index &ix, u32 *out) {
+    if (ix.n == 0) return;
+    unsigned g = (unsigned)std::min<u64>((ix.n + 255) / 256, (u64)ctx->num_cus * 32);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * ix.n, lookup_values_k, g, 256, 0, (const u64 *)ix.rows.p,
+               ix.n, (const u64 *)ix.slots.p, ix.nslots - 1, out);
+}
+
+}  // namespace dlg

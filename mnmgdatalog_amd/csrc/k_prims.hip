@@ -1,0 +1,594 @@
+// k_prims.hip -- data-parallel primitives for the fixpoint engine on gfx950.
+//
+// Replaces the Thrust calls of the reference (sort/unique/set_difference/
+// merge/reduce/exclusive_scan; sites in tc.cu:177-283, sg.cu:150-368,
+// wcc.cu:163-463, single_join.cu:165-264, common/join.cu:72-74) with
+// hand-written kernels:
+//   * exclusive scan       reduce -> scan of block sums -> apply
+//   * radix sort           LSD, reduce-then-scan per digit, wave-match ranking
+//                          (64-lane ballots), LDS staging for coalesced scatter;
+//                          digits run over a compacted key (hi - min_hi, lo - min_lo)
+//                          so only the bits that vary are sorted
+//   * select (compaction)  unique / unique minus diagonal / unique minus a sorted
+//                          set / set difference / key-run heads; flags are
+//                          ballot words, offsets are a block-sum scan
+//   * merge                rank-by-binary-search scatter (a first on ties)
+#include "dlg_internal.h"
+
+#include <algorithm>
+
+namespace dlg {
+
+// =============================================================== fills / copies
+__global__ void fill_u64_k(u64 *p, u64 n, u64 v) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+__global__ void fill_u32_k(u32 *p, u64 n, u32 v) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+__global__ void copy_u64_k(u64 *dst, const u64 *src, u64 n) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+static unsigned stream_grid(dlg_ctx *ctx, u64 n) {
+    u64 g = (n + 255) / 256;
+    u64 cap = (u64)ctx->num_cus * 16;
+    if (g > cap) g = cap;
+    return (unsigned)(g ? g : 1);
+}
+
+void fill_u64(dlg_ctx *ctx, u64 *p, u64 n, u64 v) {
+    if (!n) return;
+    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, fill_u64_k, stream_grid(ctx, n), 256, 0, p, n, v);
+}
+void fill_u32(dlg_ctx *ctx, u32 *p, u64 n, u32 v) {
+    if (!n) return;
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * n, fill_u32_k, stream_grid(ctx, n), 256, 0, p, n, v);
+}
+void copy_u64(dlg_ctx *ctx, u64 *dst, const u64 *src, u64 n) {
+    if (!n) return;
+    DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * n, copy_u64_k, stream_grid(ctx, n), 256, 0, dst, src, n);
+}
+
+// =============================================================== pack / unpack
+// Tuple t = ((col_a ^ 0x80000000) << 32) | (col_b ^ 0x80000000): u64 order is
+// the signed lexicographic order of the reference's set_cmp.
+__global__ void pack_k(const int2 *rows, u64 n, int orient, u64 *out) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        int2 r = rows[i];
+        u32 a = (u32)r.x ^ 0x80000000u, b = (u32)r.y ^ 0x80000000u;
+        out[i] = orient ? tmake(b, a) : tmake(a, b);
+    }
+}
+__global__ void pack_sym_k(const int2 *rows, u64 n, u64 *out) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        int2 r = rows[i];
+        u32 a = (u32)r.x ^ 0x80000000u, b = (u32)r.y ^ 0x80000000u;
+        out[i] = tmake(a, b);
+        out[n + i] = tmake(b, a);
+    }
+}
+__global__ void unpack_k(const u64 *t, u64 n, int orient, int2 *rows) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        u64 x = t[i];
+        int k = (int)(tkey(x) ^ 0x80000000u), v = (int)(tval(x) ^ 0x80000000u);
+        rows[i] = orient ? make_int2(v, k) : make_int2(k, v);
+    }
+}
+
+void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out) {
+    if (!n) return;
+    DLG_LAUNCH(ctx, DLG_K_PACK, 16.0 * n, pack_k, stream_grid(ctx, n), 256, 0,
+               (const int2 *)rows, n, orient, out);
+}
+void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out) {
+    if (!n) return;
+    DLG_LAUNCH(ctx, DLG_K_PACK, 24.0 * n, pack_sym_k, stream_grid(ctx, n), 256, 0,
+               (const int2 *)rows, n, out);
+}
+void unpack_rows(dlg_ctx *ctx, const u64 *t, u64 n, int orient, int32_t *rows) {
+    if (!n) return;
+    DLG_LAUNCH(ctx, DLG_K_PACK, 16.0 * n, unpack_k, stream_grid(ctx, n), 256, 0, t, n, orient,
+               (int2 *)rows);
+}
+
+// =============================================================== scan
+constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
+
+__global__ __launch_bounds__(SCAN_T) void scan_reduce_k(const u32 *in, u64 n, u64 *bsum) {
+    __shared__ u64 sc[SCAN_T / WAVE + 1];
+    u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_I;
+    u64 s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; i++)
+        if (base + i < n) s += in[base + i];
+    u64 tot;
+    block_exclusive_scan<u64>(s, sc, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// One block scans the block sums in place (exclusive); total to *d_total.
+__global__ __launch_bounds__(1024) void scan_blocks_k(u64 *bsum, u64 nb, u64 *d_total) {
+    __shared__ u64 sc[1024 / WAVE + 1];
+    u64 carry = 0;
+    for (u64 base = 0; base < nb; base += 1024) {
+        u64 i = base + threadIdx.x;
+        u64 v = i < nb ? bsum[i] : 0;
+        u64 tot;
+        u64 ex = block_exclusive_scan<u64>(v, sc, &tot);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_T) void scan_apply_k(const u32 *in, u64 n, const u64 *bpre,
+                                                       u64 *out) {
+    __shared__ u64 sc[SCAN_T / WAVE + 1];
+    u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_I;
+    u32 v[SCAN_I];
+    u64 s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; i++) {
+        v[i] = base + i < n ? in[base + i] : 0;
+        s += v[i];
+    }
+    u64 ex = block_exclusive_scan<u64>(s, sc, nullptr) + bpre[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_I; i++) {
+        if (base + i < n) out[base + i] = ex;
+        ex += v[i];
+    }
+}
+
+void scan_u32_u64_async(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *d_total) {
+    u64 nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 0) nb = 1;
+    DBuf<u64> bsum(ctx, nb);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 4.0 * n + 8.0 * nb, scan_reduce_k, (unsigned)nb, SCAN_T, 0, in, n,
+               bsum.p);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb, d_total);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 12.0 * n, scan_apply_k, (unsigned)nb, SCAN_T, 0, in, n, bsum.p,
+               out);
+}
+
+u64 scan_u32_u64(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out) {
+    DBuf<u64> tot(ctx, 1);
+    scan_u32_u64_async(ctx, in, n, out, tot.p);
+    u64 h = 0;
+    ctx->read_scalars(tot.p, 1, &h);
+    return h;
+}
+
+// Exclusive scan of u32 counts into u32 offsets (radix digit counts; sums < 2^32).
+__global__ __launch_bounds__(SCAN_T) void scan_reduce32_k(const u32 *in, u64 n, u64 *bsum) {
+    __shared__ u64 sc[SCAN_T / WAVE + 1];
+    u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_I;
+    u64 s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; i++)
+        if (base + i < n) s += in[base + i];
+    u64 tot;
+    block_exclusive_scan<u64>(s, sc, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(SCAN_T) void scan_apply32_k(u32 *data, u64 n, const u64 *bpre) {
+    __shared__ u64 sc[SCAN_T / WAVE + 1];
+    u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_I;
+    u32 v[SCAN_I];
+    u64 s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; i++) {
+        v[i] = base + i < n ? data[base + i] : 0;
+        s += v[i];
+    }
+    u64 ex = block_exclusive_scan<u64>(s, sc, nullptr) + bpre[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_I; i++) {
+        if (base + i < n) data[base + i] = (u32)ex;
+        ex += v[i];
+    }
+}
+static void scan_u32_inplace(dlg_ctx *ctx, u32 *data, u64 n, DBuf<u64> &bsum) {
+    u64 nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 0) nb = 1;
+    bsum.ensure(nb);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 4.0 * n, scan_reduce32_k, (unsigned)nb, SCAN_T, 0, data, n,
+               bsum.p);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb,
+               (u64 *)nullptr);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 8.0 * n, scan_apply32_k, (unsigned)nb, SCAN_T, 0, data, n,
+               bsum.p);
+}
+
+// =============================================================== radix sort
+constexpr int RS_T = 256, RS_I = 16, RS_TILE = RS_T * RS_I, RS_WAVES = RS_T / WAVE;
+
+struct RadixKey {
+    u32 min_hi, min_lo;
+    u32 vb;     // bits of (lo - min_lo)
+    u32 shift;  // digit position in the compacted key
+};
+
+__device__ __forceinline__ u32 radix_digit(u64 x, const RadixKey &k, u32 mask) {
+    u64 hi = (u64)(tkey(x) - k.min_hi);
+    u64 lo = (u64)(tval(x) - k.min_lo);
+    u64 c = (hi << k.vb) | lo;  // vb <= 32, (hi bits + vb) <= 64
+    return (u32)(c >> k.shift) & mask;
+}
+
+// min/max of the key and value halves: out[0]=min_hi out[1]=max_hi out[2]=min_lo out[3]=max_lo
+__global__ void minmax_init_k(u32 *out) {
+    if (threadIdx.x == 0) {
+        out[0] = 0xffffffffu; out[1] = 0u; out[2] = 0xffffffffu; out[3] = 0u;
+    }
+}
+__global__ __launch_bounds__(256) void minmax_k(const u64 *in, u64 n, u32 *out) {
+    u32 mnh = 0xffffffffu, mxh = 0, mnl = 0xffffffffu, mxl = 0;
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        u64 x = in[i];
+        u32 h = tkey(x), l = tval(x);
+        mnh = min(mnh, h); mxh = max(mxh, h); mnl = min(mnl, l); mxl = max(mxl, l);
+    }
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        mnh = min(mnh, (u32)__shfl_xor((int)mnh, d, WAVE));
+        mxh = max(mxh, (u32)__shfl_xor((int)mxh, d, WAVE));
+        mnl = min(mnl, (u32)__shfl_xor((int)mnl, d, WAVE));
+        mxl = max(mxl, (u32)__shfl_xor((int)mxl, d, WAVE));
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        atomicMin(&out[0], mnh); atomicMax(&out[1], mxh);
+        atomicMin(&out[2], mnl); atomicMax(&out[3], mxl);
+    }
+}
+
+template <int BITS>
+__global__ __launch_bounds__(RS_T) void rs_upsweep_k(const u64 *in, u64 n, RadixKey key,
+                                                     u32 *counts, u32 nb) {
+    constexpr int BINS = 1 << BITS;
+    __shared__ u32 h[RS_WAVES][BINS];
+    for (int i = threadIdx.x; i < RS_WAVES * BINS; i += RS_T) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const int wave = threadIdx.x / WAVE;
+    u64 base = (u64)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_I; r++) {
+        u64 idx = base + (u64)r * RS_T + threadIdx.x;
+        if (idx < n) atomicAdd(&h[wave][radix_digit(in[idx], key, BINS - 1)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < BINS; d += RS_T) {
+        u32 s = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) s += h[w][d];
+        counts[(u64)d * nb + blockIdx.x] = s;
+    }
+}
+
+template <int BITS>
+__global__ __launch_bounds__(RS_T) void rs_downsweep_k(const u64 *in, u64 *out, u64 n,
+                                                       RadixKey key, const u32 *offs, u32 nb) {
+    constexpr int BINS = 1 << BITS;
+    __shared__ u64 stage[RS_TILE];
+    __shared__ u32 wcnt[RS_WAVES][BINS];
+    __shared__ u32 bstart[BINS];
+    __shared__ u32 gofs[BINS];
+    __shared__ u32 sc[RS_T / WAVE + 1];
+
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    for (int i = threadIdx.x; i < RS_WAVES * BINS; i += RS_T) (&wcnt[0][0])[i] = 0;
+    for (int d = threadIdx.x; d < BINS; d += RS_T) gofs[d] = offs[(u64)d * nb + blockIdx.x];
+    __syncthreads();
+
+    const u64 tile0 = (u64)blockIdx.x * RS_TILE;
+    const u64 wbase = tile0 + (u64)wave * (WAVE * RS_I);
+    const u64 lt = lanemask_lt();
+    u64 keys[RS_I];
+    u32 pk[RS_I];  // digit | (rank within wave's digit run << 8)
+
+#pragma unroll
+    for (int r = 0; r < RS_I; r++) {
+        u64 idx = wbase + (u64)r * WAVE + lane;
+        bool valid = idx < n;
+        u64 x = valid ? in[idx] : 0ull;
+        u32 d = radix_digit(x, key, BINS - 1);
+        u64 m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < BITS; b++) {
+            bool bit = (d >> b) & 1u;
+            u64 bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        u32 before = __popcll(m & lt);
+        u32 cnt = wcnt[wave][d];
+        // The lowest lane of each digit group publishes the new running count;
+        // every lane of the wave has read the old value in the same LDS op.
+        if (valid && before == 0) wcnt[wave][d] = cnt + (u32)__popcll(m);
+        keys[r] = x;
+        pk[r] = d | ((cnt + before) << 8);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // per digit: exclusive prefix over waves, then over digits
+    u32 tot = 0;
+    if (threadIdx.x < BINS) {
+        u32 s = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            u32 c = wcnt[w][threadIdx.x];
+            wcnt[w][threadIdx.x] = s;
+            s += c;
+        }
+        tot = s;
+    }
+    u32 ex = block_exclusive_scan<u32>(tot, sc, nullptr);
+    if (threadIdx.x < BINS) bstart[threadIdx.x] = ex;
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < RS_I; r++) {
+        u64 idx = wbase + (u64)r * WAVE + lane;
+        if (idx < n) {
+            u32 d = pk[r] & 0xffu;
+            stage[bstart[d] + wcnt[wave][d] + (pk[r] >> 8)] = keys[r];
+        }
+    }
+    __syncthreads();
+
+    u64 rem = n - tile0;
+    u32 tn = rem < (u64)RS_TILE ? (u32)rem : (u32)RS_TILE;
+    for (u32 i = threadIdx.x; i < tn; i += RS_T) {
+        u64 x = stage[i];
+        u32 d = radix_digit(x, key, BINS - 1);
+        out[(u64)gofs[d] + (i - bstart[d])] = x;
+    }
+}
+
+static int bit_width(u32 v) { return v ? 32 - __builtin_clz(v) : 0; }
+
+template <int BITS>
+static void radix_passes(dlg_ctx *ctx, u64 *&src, u64 *&dst, u64 n, RadixKey key, int total_bits) {
+    constexpr int BINS = 1 << BITS;
+    u32 nb = (u32)((n + RS_TILE - 1) / RS_TILE);
+    DBuf<u32> counts(ctx, (size_t)BINS * nb);
+    DBuf<u64> bsum(ctx);
+    for (int shift = 0; shift < total_bits; shift += BITS) {
+        key.shift = (u32)shift;
+        DLG_LAUNCH(ctx, DLG_K_SORT_UPSWEEP, 8.0 * n + 4.0 * BINS * nb, rs_upsweep_k<BITS>, nb,
+                   RS_T, 0, (const u64 *)src, n, key, counts.p, nb);
+        scan_u32_inplace(ctx, counts.p, (u64)BINS * nb, bsum);
+        DLG_LAUNCH(ctx, DLG_K_SORT_DOWNSWEEP, 16.0 * n + 4.0 * BINS * nb, rs_downsweep_k<BITS>,
+                   nb, RS_T, 0, (const u64 *)src, dst, n, key, (const u32 *)counts.p, nb);
+        std::swap(src, dst);
+    }
+}
+
+void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits) {
+    if (n < 2) return;
+    if (n > 0xffffffffull) fail(DLG_OVERFLOW, "radix_sort: more than 2^32-1 tuples in one buffer");
+    if (bits <= 0) bits = ctx->radix_bits;
+    if (bits < 4) bits = 4;
+    if (bits > 8) bits = 8;
+    DBuf<u32> mm(ctx, 4);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 0.0, minmax_init_k, 1, 64, 0, mm.p);
+    unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 8);
+    DLG_LAUNCH(ctx, DLG_K_SORT_HIST, 8.0 * n, minmax_k, g, 256, 0, (const u64 *)data, n, mm.p);
+    u32 h[4];
+    HIPCHK(hipMemcpyAsync(ctx->h_scalars, mm.p, 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(h, ctx->h_scalars, 16);
+    RadixKey key;
+    key.min_hi = h[0];
+    key.min_lo = h[2];
+    key.vb = (u32)bit_width(h[3] - h[2]);
+    key.shift = 0;
+    int total_bits = (int)key.vb + bit_width(h[1] - h[0]);
+    if (total_bits == 0) return;  // all equal
+    tmp.ensure(n);
+    u64 *src = data, *dst = tmp.p;
+    switch (bits) {
+        case 4: radix_passes<4>(ctx, src, dst, n, key, total_bits); break;
+        case 5: radix_passes<5>(ctx, src, dst, n, key, total_bits); break;
+        case 6: radix_passes<6>(ctx, src, dst, n, key, total_bits); break;
+        case 7: radix_passes<7>(ctx, src, dst, n, key, total_bits); break;
+        default: radix_passes<8>(ctx, src, dst, n, key, total_bits); break;
+    }
+    if (src != data) copy_u64(ctx, data, src, n);
+}
+
+// =============================================================== select
+constexpr int SEL_T = 256, SEL_I = 8, SEL_TILE = SEL_T * SEL_I;
+
+__device__ __forceinline__ bool found_sorted(const u64 *a, u64 lo, u64 hi, u64 x) {
+    u64 p = lower_bound_u64(a, lo, hi, x);
+    return p < hi && a[p] == x;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(SEL_T) void select_count_k(const u64 *in, u64 n, const u64 *other,
+                                                        u64 no, u64 *flags, u64 *bsum) {
+    __shared__ u64 s_lo, s_hi;
+    __shared__ u64 sc[SEL_T / WAVE + 1];
+    const u64 base = (u64)blockIdx.x * SEL_TILE;
+    constexpr bool NEED_OTHER = (MODE == SEL_UNIQUE_NOT_IN || MODE == SEL_NOT_IN);
+    if (NEED_OTHER) {
+        if (threadIdx.x == 0) {
+            u64 last = base + SEL_TILE - 1 < n ? base + SEL_TILE - 1 : n - 1;
+            u64 lo = lower_bound_u64(other, 0, no, in[base]);
+            s_lo = lo;
+            s_hi = upper_bound_u64(other, lo, no, in[last]);
+        }
+        __syncthreads();
+    }
+    u64 c = 0;
+#pragma unroll
+    for (int r = 0; r < SEL_I; r++) {
+        u64 idx = base + (u64)r * SEL_T + threadIdx.x;
+        bool keep = false;
+        if (idx < n) {
+            u64 x = in[idx];
+            if (MODE == SEL_KEY_HEAD) {
+                keep = idx == 0 || tkey(in[idx - 1]) != tkey(x);
+            } else if (MODE == SEL_NOT_IN) {
+                keep = !found_sorted(other, s_lo, s_hi, x);
+            } else {
+                keep = idx == 0 || in[idx - 1] != x;
+                if (MODE == SEL_UNIQUE_NODIAG) keep = keep && tkey(x) != tval(x);
+                if (MODE == SEL_UNIQUE_NOT_IN && keep) keep = !found_sorted(other, s_lo, s_hi, x);
+            }
+        }
+        u64 m = __ballot(keep);
+        u64 w0 = base + (u64)r * SEL_T + (threadIdx.x & ~(WAVE - 1));
+        if ((threadIdx.x & (WAVE - 1)) == 0 && w0 < n) flags[w0 / WAVE] = m;
+        c += keep ? 1 : 0;
+    }
+    u64 tot;
+    block_exclusive_scan<u64>(c, sc, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+template <bool OUT_INDEX>
+__global__ __launch_bounds__(SEL_T) void select_scatter_k(const u64 *in, u64 n, const u64 *flags,
+                                                          const u64 *bpre, void *out) {
+    __shared__ u32 wc[SEL_I][SEL_T / WAVE];
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    const u64 base = (u64)blockIdx.x * SEL_TILE;
+    u64 words[SEL_I];
+#pragma unroll
+    for (int r = 0; r < SEL_I; r++) {
+        u64 w0 = base + (u64)r * SEL_T + (u64)wave * WAVE;
+        words[r] = w0 < n ? flags[w0 / WAVE] : 0ull;
+        if (lane == 0) wc[r][wave] = (u32)__popcll(words[r]);
+    }
+    __syncthreads();
+    u64 pre = bpre[blockIdx.x];
+    const u64 lt = lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < SEL_I; r++) {
+        u32 p = 0;
+        for (int rr = 0; rr < SEL_I; rr++)
+            for (int ww = 0; ww < SEL_T / WAVE; ww++)
+                if (rr < r || (rr == r && ww < wave)) p += wc[rr][ww];
+        if ((words[r] >> lane) & 1ull) {
+            u64 idx = base + (u64)r * SEL_T + threadIdx.x;
+            u64 pos = pre + p + (u64)__popcll(words[r] & lt);
+            if (OUT_INDEX) ((u32 *)out)[pos] = (u32)idx;
+            else ((u64 *)out)[pos] = in[idx];
+        }
+    }
+}
+
+u64 select(dlg_ctx *ctx, int mode, const u64 *in, u64 n, const u64 *other, u64 n_other,
+           void *out) {
+    if (n == 0) return 0;
+    if ((mode == SEL_NOT_IN || mode == SEL_UNIQUE_NOT_IN) && n_other == 0)
+        mode = mode == SEL_NOT_IN ? -1 : SEL_UNIQUE;
+    if (mode == -1) {  // nothing to remove
+        if (out != (const void *)in) copy_u64(ctx, (u64 *)out, in, n);
+        return n;
+    }
+    u64 nb = (n + SEL_TILE - 1) / SEL_TILE;
+    DBuf<u64> flags(ctx, (n + WAVE - 1) / WAVE + 1);
+    DBuf<u64> bsum(ctx, nb);
+    DBuf<u64> tot(ctx, 1);
+    double rb = 8.0 * n + 8.0 * ((n + 63) / 64);
+    switch (mode) {
+        case SEL_UNIQUE:
+            DLG_LAUNCH(ctx, DLG_K_SELECT, rb, select_count_k<SEL_UNIQUE>, (unsigned)nb, SEL_T, 0,
+                       in, n, other, n_other, flags.p, bsum.p);
+            break;
+        case SEL_UNIQUE_NODIAG:
+            DLG_LAUNCH(ctx, DLG_K_SELECT, rb, select_count_k<SEL_UNIQUE_NODIAG>, (unsigned)nb,
+                       SEL_T, 0, in, n, other, n_other, flags.p, bsum.p);
+            break;
+        case SEL_UNIQUE_NOT_IN:
+            DLG_LAUNCH(ctx, DLG_K_SELECT, rb, select_count_k<SEL_UNIQUE_NOT_IN>, (unsigned)nb,
+                       SEL_T, 0, in, n, other, n_other, flags.p, bsum.p);
+            break;
+        case SEL_NOT_IN:
+            DLG_LAUNCH(ctx, DLG_K_SELECT, rb, select_count_k<SEL_NOT_IN>, (unsigned)nb, SEL_T, 0,
+                       in, n, other, n_other, flags.p, bsum.p);
+            break;
+        case SEL_KEY_HEAD:
+            DLG_LAUNCH(ctx, DLG_K_SELECT, rb, select_count_k<SEL_KEY_HEAD>, (unsigned)nb, SEL_T,
+                       0, in, n, other, n_other, flags.p, bsum.p);
+            break;
+        default: fail(DLG_INVALID_INPUT, "select: bad mode");
+    }
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb, tot.p);
+    u64 total = 0;
+    ctx->read_scalars(tot.p, 1, &total);
+    if (total == 0) return 0;
+    if (mode == SEL_KEY_HEAD)
+        DLG_LAUNCH(ctx, DLG_K_SELECT, 8.0 * ((n + 63) / 64) + 12.0 * total,
+                   select_scatter_k<true>, (unsigned)nb, SEL_T, 0, in, n, flags.p, bsum.p, out);
+    else
+        DLG_LAUNCH(ctx, DLG_K_SELECT, 8.0 * ((n + 63) / 64) + 16.0 * total,
+                   select_scatter_k<false>, (unsigned)nb, SEL_T, 0, in, n, flags.p, bsum.p, out);
+    return total;
+}
+
+// =============================================================== merge
+constexpr int MG_T = 256, MG_I = 8, MG_TILE = MG_T * MG_I;
+
+// Scatter one side of a merge: element x of `src` goes to i + rank(x in oth),
+// rank = #oth < x for side a (a first on ties) and #oth <= x for side b.
+template <bool SIDE_A>
+__global__ __launch_bounds__(MG_T) void merge_side_k(const u64 *src, u64 ns, const u64 *oth, u64 no,
+                                                     u64 *out) {
+    __shared__ u64 s_lo, s_hi;
+    const u64 base = (u64)blockIdx.x * MG_TILE;
+    if (threadIdx.x == 0) {
+        u64 last = base + MG_TILE - 1 < ns ? base + MG_TILE - 1 : ns - 1;
+        u64 lo, hi;
+        if (SIDE_A) {
+            lo = lower_bound_u64(oth, 0, no, src[base]);
+            hi = lower_bound_u64(oth, lo, no, src[last]);
+        } else {
+            lo = upper_bound_u64(oth, 0, no, src[base]);
+            hi = upper_bound_u64(oth, lo, no, src[last]);
+        }
+        s_lo = lo;
+        s_hi = hi;
+    }
+    __syncthreads();
+    const u64 lo = s_lo, hi = s_hi;
+#pragma unroll
+    for (int r = 0; r < MG_I; r++) {
+        u64 i = base + (u64)r * MG_T + threadIdx.x;
+        if (i < ns) {
+            u64 x = src[i];
+            u64 rk = lo == hi ? lo
+                              : (SIDE_A ? lower_bound_u64(oth, lo, hi, x)
+                                        : upper_bound_u64(oth, lo, hi, x));
+            out[i + rk] = x;
+        }
+    }
+}
+
+void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out) {
+    if (na)
+        DLG_LAUNCH(ctx, DLG_K_MERGE, 16.0 * na, merge_side_k<true>, grid_for(na, MG_TILE), MG_T, 0,
+                   a, na, b, nb, out);
+    if (nb)
+        DLG_LAUNCH(ctx, DLG_K_MERGE, 16.0 * nb, merge_side_k<false>, grid_for(nb, MG_TILE), MG_T,
+                   0, b, nb, a, na, out);
+}
+
+}  // namespace dlg

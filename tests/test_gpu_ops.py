@@ -1,0 +1,150 @@
+"""Operator-level parity of the HIP kernels (through the C ABI) against numpy.
+
+Covers the roles of the reference's Thrust calls and join:
+sort / sort+unique / set_difference / merge / exclusive_scan, and
+get_hash_table + get_join (multiset of join rows, both projections)."""
+import numpy as np
+import pytest
+
+import mnmgdatalog_amd as dlg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dlg.Ctx(0)
+    yield c
+    c.close()
+
+
+def rand_tuples(rng, n, kbits=32, vbits=32, neg=False):
+    k = rng.integers(0, 2 ** kbits, size=n, dtype=np.uint64)
+    v = rng.integers(0, 2 ** vbits, size=n, dtype=np.uint64)
+    if neg:
+        k ^= np.uint64(0x80000000)
+    return (k << np.uint64(32)) | v
+
+
+SORT_CASES = [(0, 32, 32), (1, 32, 32), (2, 3, 3), (1000, 17, 17), (4097, 32, 32), (65536, 1, 0),
+              (100_003, 20, 12), (1_000_000, 32, 32), (2_500_000, 17, 17), (300_000, 0, 5)]
+
+
+@pytest.mark.parametrize("n,kb,vb", SORT_CASES)
+def test_sort(ctx, n, kb, vb):
+    rng = np.random.default_rng(n + kb)
+    t = rand_tuples(rng, n, kb, vb)
+    assert np.array_equal(ctx.sort(t), np.sort(t))
+
+
+def test_sort_all_equal_and_presorted(ctx):
+    t = np.full(50_000, 0x1234567800000001, np.uint64)
+    assert np.array_equal(ctx.sort(t), t)
+    s = np.sort(rand_tuples(np.random.default_rng(5), 70_000))
+    assert np.array_equal(ctx.sort(s), s)
+    assert np.array_equal(ctx.sort(s[::-1].copy()), s)
+
+
+@pytest.mark.parametrize("n,kb,vb", SORT_CASES)
+def test_sort_unique(ctx, n, kb, vb):
+    rng = np.random.default_rng(7 * n + vb)
+    t = rand_tuples(rng, n, kb, vb)
+    assert np.array_equal(ctx.sort_unique(t), np.unique(t))
+
+
+@pytest.mark.parametrize("bits", [4, 5, 6, 7, 8])
+def test_sort_digit_widths_via_program(ctx, bits):
+    rng = np.random.default_rng(bits)
+    rows = rng.integers(0, 3000, size=(40_000, 2)).astype(np.int32)
+    r = ctx.run("single_join", rows, radix_bits=bits)
+    t = dlg.pack(rows)
+    R = np.unique(t)
+    # brute force of the single join with numpy: (c, a) for R(a, b), R(b, c)
+    a, b = R >> np.uint64(32), R & np.uint64(0xffffffff)
+    order = np.argsort(a, kind="stable")
+    ka = a[order]
+    lo, hi = np.searchsorted(ka, b, "left"), np.searchsorted(ka, b, "right")
+    cnt = hi - lo
+    rep = np.repeat(np.arange(len(R)), cnt)
+    idx = np.concatenate([np.arange(l, h) for l, h in zip(lo, hi)]) if cnt.sum() else np.zeros(0, int)
+    c = b[order][idx]
+    out = np.unique((c << np.uint64(32)) | a[rep])
+    assert r["output_size"] == len(out)
+    assert np.array_equal(dlg.pack(r["rows"], reverse=True), out)
+
+
+@pytest.mark.parametrize("na,nb,overlap", [(0, 10, 0.0), (10, 0, 0.0), (1000, 1000, 0.5),
+                                          (200_000, 3_000, 0.3), (3_000, 200_000, 0.9),
+                                          (1_000_000, 1_000_000, 0.1)])
+def test_set_difference_and_merge(ctx, na, nb, overlap):
+    rng = np.random.default_rng(na + 3 * nb)
+    a = np.unique(rand_tuples(rng, na, 20, 20))
+    b = np.unique(np.concatenate([rand_tuples(rng, nb, 20, 20),
+                                  rng.choice(a, size=int(overlap * min(len(a), nb)), replace=False)
+                                  if len(a) else np.zeros(0, np.uint64)]))
+    d = ctx.set_difference(a, b)
+    assert np.array_equal(d, np.setdiff1d(a, b))
+    m = ctx.merge(a, b)
+    assert np.array_equal(m, np.sort(np.concatenate([a, b])))
+
+
+@pytest.mark.parametrize("n", [1, 100, 2048, 2049, 1_000_000, 5_000_001])
+def test_exclusive_scan(ctx, n):
+    rng = np.random.default_rng(n)
+    c = rng.integers(0, 5000, size=n, dtype=np.uint32)
+    offs, tot = ctx.exclusive_scan(c)
+    ref = np.concatenate([[0], np.cumsum(c.astype(np.uint64))[:-1]])
+    assert tot == int(c.astype(np.uint64).sum())
+    assert np.array_equal(offs, ref)
+
+
+def numpy_join(rel, probe, proj):
+    rk, rv = rel >> np.uint64(32), rel & np.uint64(0xffffffff)
+    pk, pv = probe >> np.uint64(32), probe & np.uint64(0xffffffff)
+    order = np.argsort(rk, kind="stable")
+    sk, sv = rk[order], rv[order]
+    lo, hi = np.searchsorted(sk, pk, "left"), np.searchsorted(sk, pk, "right")
+    cnt = hi - lo
+    rep = np.repeat(np.arange(len(probe)), cnt)
+    idx = np.concatenate([np.arange(l, h) for l, h in zip(lo, hi)]) if cnt.sum() else np.zeros(0, int)
+    hv, pvv = sv[idx], pv[rep]
+    out = (hv << np.uint64(32)) | pvv if proj == dlg.PROJ_HV_PV else (pvv << np.uint64(32)) | hv
+    return np.sort(out)
+
+
+@pytest.mark.parametrize("nrel,nprobe,keys,lf", [(0, 100, 50, 0.0), (100, 0, 50, 0.0), (1, 1, 1, 0.0),
+                                                (5000, 7000, 300, 0.6), (50_000, 20_000, 40_000, 0.3),
+                                                (200_000, 100_000, 2_000, 0.9), (30_000, 30_000, 30, 0.5)])
+@pytest.mark.parametrize("proj", [dlg.PROJ_HV_PV, dlg.PROJ_PV_HV])
+def test_hash_join_multiset(ctx, nrel, nprobe, keys, lf, proj):
+    rng = np.random.default_rng(nrel + nprobe + keys)
+    k = rng.integers(0, keys, size=nrel).astype(np.uint64) ^ np.uint64(0x80000000)
+    rel = (k << np.uint64(32)) | rng.integers(0, 2 ** 32, size=nrel, dtype=np.uint64)
+    if nrel > 10:
+        rel[:5] = rel[5:10]                                   # duplicate relation rows
+    pk = rng.integers(0, 2 * keys, size=nprobe).astype(np.uint64) ^ np.uint64(0x80000000)
+    probe = (pk << np.uint64(32)) | rng.integers(0, 2 ** 32, size=nprobe, dtype=np.uint64)
+    got, info = ctx.hash_join(rel, probe, proj, lf)
+    assert np.array_equal(np.sort(got), numpy_join(rel, probe, proj))
+    assert info["rows"] == nrel
+    if nrel:
+        assert info["keys"] == len(np.unique(k))
+        assert info["slots"] & (info["slots"] - 1) == 0 and info["slots"] > info["keys"]
+
+
+def test_hash_join_extreme_ids(ctx):
+    ids = np.array([-2**31, -1, 0, 2**31 - 1, -2], np.int32)
+    rows = np.array([(a, b) for a in ids for b in ids], np.int32)
+    rel, probe = dlg.pack(rows), dlg.pack(rows, reverse=True)
+    got, _ = ctx.hash_join(rel, probe)
+    assert np.array_equal(np.sort(got), numpy_join(rel, probe, dlg.PROJ_HV_PV))
+
+
+def test_invalid_arguments_report_status(ctx):
+    import ctypes
+    k = ctypes.c_int64()
+    s = ctx.L.dlg_sort_unique(ctx.h, None, 5, ctypes.byref(k))
+    assert s == dlg.DLG_INVALID_INPUT
+    with pytest.raises(dlg.DlgError) as e:
+        ctx.run("tc", np.zeros((3, 2), np.int32), load_factor=2.0)
+    assert e.value.status == dlg.DLG_INVALID_INPUT

@@ -1,0 +1,250 @@
+"""Program-level parity on the GPU: TC, SG, WCC and the single join through
+the C ABI (and the CLI executables) against the CPU oracle -- bit-exact rows
+and equal iteration counts -- and against the reference's published counts at
+full size."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import mnmgdatalog_amd as dlg
+from conftest import DATA, read_bin
+
+pytestmark = pytest.mark.gpu
+
+SMALL = ["hipc_2019.bin", "data_5.bin", "data_10.bin", "data_22.bin", "dummy.bin", "tc_paper.bin",
+         "paper.bin", "quiz3.bin", "data_7035.bin", "data_23874.bin"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dlg.Ctx(0)
+    yield c
+    c.close()
+
+
+def assert_same(got, want, secondary=False):
+    assert got["iterations"] == want["iterations"]
+    assert got["output_size"] == want["output_size"]
+    if secondary:
+        assert got["secondary"] == want["secondary"]
+    assert got["rows"].shape == want["rows"].shape
+    assert np.array_equal(got["rows"], want["rows"])
+
+
+def rand_rows(seed, n, m, neg=False, self_loops=True, dup=True):
+    rng = np.random.default_rng(seed)
+    r = rng.integers(-n if neg else 0, n, size=(m, 2)).astype(np.int32)
+    if not self_loops:
+        r = r[r[:, 0] != r[:, 1]]
+    if dup and len(r) > 4:
+        r = np.concatenate([r, r[:4]])
+    return r
+
+
+# ------------------------------------------------------------------ fixtures vs oracle
+@pytest.mark.parametrize("name", SMALL)
+def test_tc_fixture(ctx, orc, name):
+    rows = read_bin(name)
+    assert_same(ctx.run("tc", rows), orc.tc(rows))
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_sg_fixture(ctx, orc, name):
+    rows = read_bin(name)
+    assert_same(ctx.run("sg", rows), orc.sg(rows))
+
+
+@pytest.mark.parametrize("name", SMALL + ["data_214078.bin", "data_223001.bin", "data_88234.bin"])
+def test_wcc_fixture(ctx, orc, name):
+    rows = read_bin(name)
+    assert_same(ctx.run("wcc", rows), orc.wcc(rows), secondary=True)
+
+
+@pytest.mark.parametrize("name,its,cc,largest", [
+    ("paper.bin", 4, 2, 3), ("dummy.bin", 4, 3, 3), ("data_214078.bin", 12, 547, 56739),
+    ("data_223001.bin", 306, 17, 115921)])
+def test_wcc_published(ctx, name, its, cc, largest):
+    r = ctx.run("wcc", read_bin(name), keep_output=False)
+    assert (r["iterations"], r["output_size"], r["secondary"]) == (its, cc, largest)
+
+
+# ------------------------------------------------------------------ random graphs vs oracle
+@pytest.mark.parametrize("seed", range(8))
+def test_tc_random(ctx, orc, seed):
+    rows = rand_rows(seed, 60 + 40 * seed, 90 + 60 * seed, neg=seed % 2 == 1)
+    assert_same(ctx.run("tc", rows), orc.tc(rows))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_sg_random(ctx, orc, seed):
+    rows = rand_rows(100 + seed, 50 + 30 * seed, 70 + 40 * seed, neg=seed % 2 == 0)
+    assert_same(ctx.run("sg", rows), orc.sg(rows))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_wcc_random(ctx, orc, seed):
+    rows = rand_rows(200 + seed, 400 * (seed + 1), 300 * (seed + 1), neg=seed % 3 == 0)
+    assert_same(ctx.run("wcc", rows), orc.wcc(rows), secondary=True)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_single_join_random(ctx, orc, seed):
+    rng = np.random.default_rng(300 + seed)
+    rows = rng.integers(-50 if seed % 2 else 1, 400, size=(3000 * (seed + 1), 2)).astype(np.int32)
+    assert_same(ctx.run("single_join", rows), orc.single_join(rows))
+
+
+@pytest.mark.parametrize("prog", ["tc", "sg", "wcc", "single_join"])
+def test_extreme_ids(ctx, orc, prog):
+    ids = np.array([-2**31, -2**31 + 1, -1, 0, 1, 2**31 - 2, 2**31 - 1], np.int32)
+    rng = np.random.default_rng(9)
+    rows = ids[rng.integers(0, len(ids), size=(40, 2))]
+    want = getattr(orc, prog)(rows)
+    assert_same(ctx.run(prog, rows), want, secondary=prog == "wcc")
+
+
+@pytest.mark.parametrize("prog,its,size,second", [("tc", 1, 0, 0), ("sg", 1, 0, 0), ("wcc", 1, 0, -1),
+                                                  ("single_join", 1, 0, 0)])
+def test_empty_input(ctx, prog, its, size, second):
+    r = ctx.run(prog, np.zeros((0, 2), np.int32))
+    assert (r["iterations"], r["output_size"], r["secondary"]) == (its, size, second)
+
+
+@pytest.mark.parametrize("lf,bits", [(0.3, 4), (0.6, 6), (0.9, 8), (0.05, 7)])
+def test_tuning_does_not_change_results(ctx, orc, lf, bits):
+    rows = read_bin("data_23874.bin")
+    want = orc.tc(rows)
+    assert_same(ctx.run("tc", rows, load_factor=lf, radix_bits=bits), want)
+
+
+def test_repeated_runs_reuse_pool(ctx, orc):
+    rows = read_bin("data_7035.bin")
+    want = orc.sg(rows)
+    for _ in range(3):
+        assert_same(ctx.run("sg", rows), want)
+
+
+# ------------------------------------------------------------------ configs
+def test_config_powerlaw_tc(ctx, orc):
+    rows = dlg.gen_powerlaw(3000, 7000, 2.1, 0x4D4E4D47)
+    got = ctx.run("tc", rows)
+    want = orc.tc(rows)
+    assert_same(got, want)
+    assert want["output_size"] == 7_778_066 and want["iterations"] == 43
+
+
+def test_config_tetmesh_sg_small(ctx, orc):
+    rows = dlg.gen_tetmesh(6)
+    assert_same(ctx.run("sg", rows), orc.sg(rows))
+
+
+def test_config_grid_wcc(ctx, orc):
+    rows = dlg.gen_grid(400, 0.55, 0.01, 20, 0x4D4E4D47)
+    assert_same(ctx.run("wcc", rows), orc.wcc(rows), secondary=True)
+
+
+# ------------------------------------------------------------------ published counts, full size
+@pytest.mark.parametrize("name,its,size", [
+    ("data_223001.bin", 287, 80_498_014),   # logs/small-merged-job-output.txt:108 (SF.cedge)
+    ("data_163734.bin", 188, 156_120_489),  # rough/compares.md:19 (fe_body)
+])
+def test_tc_published_full(ctx, name, its, size):
+    r = ctx.run("tc", read_bin(name))
+    assert (r["iterations"], r["output_size"]) == (its, size)
+    rows = r["rows"]
+    key = rows[:, 1].astype(np.int64) * (1 << 32) + rows[:, 0].astype(np.int64)
+    assert np.all(np.diff(key) > 0)                                # sorted by (dst, src), no duplicates
+
+    def member(x, y):
+        k = y.astype(np.int64) * (1 << 32) + x.astype(np.int64)
+        p = np.searchsorted(key, k)
+        return (p < len(key)) & (key[np.minimum(p, len(key) - 1)] == k)
+
+    e = read_bin(name)
+    assert member(e[:, 0], e[:, 1]).all()                          # E is in TC
+    # closure on a sample: tc(x, y), e(y, z) => tc(x, z)
+    rng = np.random.default_rng(0)
+    smp = rows[rng.integers(0, len(rows), size=20000)]
+    es = e[np.argsort(e[:, 0], kind="stable")]
+    lo, hi = np.searchsorted(es[:, 0], smp[:, 1], "left"), np.searchsorted(es[:, 0], smp[:, 1], "right")
+    xs = np.repeat(smp[:, 0], hi - lo)
+    zs = es[np.concatenate([np.arange(a, b) for a, b in zip(lo, hi)]), 1]
+    assert member(xs, zs).all()
+
+
+@pytest.mark.parametrize("name,its,size", [
+    ("data_88234.bin", 13, 15_018_986),    # logs/small-merged-job-output.txt:269 (ego-Facebook)
+    ("data_51971.bin", 9, 74_618_689),     # logs/small-merged-job-output.txt:250 (CA-HepTh)
+])
+def test_sg_published_full(ctx, name, its, size):
+    r = ctx.run("sg", read_bin(name), keep_output=False)
+    assert (r["iterations"], r["output_size"]) == (its, size)
+
+
+@pytest.mark.parametrize("rows,rng_,out", [
+    (10_000_000, 1_000_000, 99_998_429),   # logs/single-join-small-job-output.txt:34
+    (10_000_000, 500_000, 199_915_594),    # :58
+    (10_000_000, 100_000, 950_725_901),    # :10
+])
+def test_single_join_published(ctx, rows, rng_, out):
+    r = ctx.run("single_join", dlg.generate_rand_rows(rows, rng_), keep_output=False)
+    assert r["output_size"] == out
+    if rng_ == 1_000_000:
+        assert r["raw_join_rows"] == 100_003_467
+
+
+def test_single_join_10m_rows_equal_oracle(ctx, orc):
+    rows = dlg.generate_rand_rows(10_000_000, 1_000_000)
+    got = ctx.run("single_join", rows)
+    want = orc.single_join(rows, threads=16)
+    assert_same(got, want)
+
+
+# ------------------------------------------------------------------ CLI executables
+def run_cli(prog, *args):
+    p = subprocess.run([os.path.join(dlg.BIN_DIR, prog), *map(str, args)], capture_output=True,
+                       text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    return p.stdout.strip().splitlines()
+
+
+@pytest.mark.parametrize("prog,suffix", [("tc", "_tc.bin"), ("sg", "_sg.bin"), ("wcc", "_cc.bin")])
+@pytest.mark.parametrize("name", ["hipc_2019.bin", "dummy.bin", "data_7035.bin"])
+def test_cli_output_file_and_csv(orc, tmp_path, prog, suffix, name):
+    src = tmp_path / name
+    shutil.copy(os.path.join(DATA, name), src)
+    out = run_cli(prog, src, 0, 0, 0)
+    assert out[0] == f"Rank 0 wrote local tuples on file: {src}{suffix}"
+    want = getattr(orc, prog)(read_bin(name))
+    header, row = out[1].split(","), out[2].split(",")
+    assert header[:4] == ["# Input", "# Process", "# Iterations", "# CC" if prog == "wcc" else "# TC"]
+    assert len(row) == len(header) == (16 if prog == "wcc" else 15)
+    assert int(row[0]) == len(read_bin(name)) and int(row[1]) == 1
+    assert int(row[2]) == want["iterations"] and int(row[3]) == want["output_size"]
+    if prog == "wcc":
+        assert int(row[4]) == want["secondary"]
+    data = np.fromfile(f"{src}{suffix}", dtype=np.int32).reshape(-1, 2)
+    assert np.array_equal(data, want["rows"])
+
+
+def test_cli_job_run_1_writes_nothing(tmp_path):
+    src = tmp_path / "hipc_2019.bin"
+    shutil.copy(os.path.join(DATA, "hipc_2019.bin"), src)
+    out = run_cli("tc", src, 0, 0, 1)
+    assert len(out) == 2 and not os.path.exists(f"{src}_tc.bin")
+
+
+def test_cli_single_join():
+    out = run_cli("single_join", 100000, 0, 0, 50000)
+    header, row = out[0].split(","), out[1].split(",")
+    assert header[3] == "# Output" and len(header) == len(row) == 17
+    assert int(row[0]) == 100000 and int(row[2]) == 1
+
+
+def test_cli_missing_file_fails(tmp_path):
+    p = subprocess.run([os.path.join(dlg.BIN_DIR, "tc"), str(tmp_path / "nope.bin")], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode != 0 and "Error opening file" in p.stderr

@@ -1,0 +1,153 @@
+"""CPU-side checks of libdlg.so: it loads, exports every entry point that
+include/dlg.h declares, and its host code (file format, glibc generator,
+synthetic graph generators, argument checks) behaves.  No GPU needed."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import mnmgdatalog_amd as dlg
+from conftest import DATA, ROOT, read_bin
+
+HEADER = os.path.join(ROOT, "include", "dlg.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dlg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 40
+    out = subprocess.run(["nm", "-D", "--defined-only", dlg.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (dlg_\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    L = dlg.lib()
+    for n in names:
+        assert getattr(L, n) is not None
+
+
+def test_library_carries_gfx950_code_objects():
+    blob = open(dlg.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_kernel_names():
+    L = dlg.lib()
+    assert [L.dlg_kernel_name(i).decode() for i in range(len(dlg.KERNELS))] == dlg.KERNELS
+
+
+def test_read_rows_matches_numpy(tmp_path):
+    r = dlg.read_rows(os.path.join(DATA, "data_7035.bin"))
+    assert np.array_equal(r, read_bin("data_7035.bin"))
+    # trailing bytes are ignored: rows = filesize / 8
+    p = tmp_path / "odd.bin"
+    p.write_bytes(np.arange(5, dtype=np.int32).tobytes())
+    assert dlg.read_rows(str(p)).tolist() == [[0, 1], [2, 3]]
+
+
+def test_write_rows_truncates(tmp_path):
+    p = str(tmp_path / "out.bin")
+    dlg.write_rows(p, np.arange(20, dtype=np.int32).reshape(10, 2))
+    dlg.write_rows(p, np.array([[7, 8]], np.int32))
+    assert dlg.read_rows(p).tolist() == [[7, 8]]
+
+
+def test_read_missing_file():
+    with pytest.raises(dlg.DlgError) as e:
+        dlg.read_rows("/nonexistent/file.bin")
+    assert e.value.status == dlg.DLG_IO_ERROR
+
+
+def test_rand_rows_same_as_oracle(orc):
+    a = dlg.generate_rand_rows(100_000, 1_000_000)
+    b = orc.generate_rand_rows(100_000, 1_000_000)
+    assert np.array_equal(a, b)
+
+
+def test_powerlaw_config_graph():
+    r = dlg.gen_powerlaw(3000, 7000, 2.1, 0x4D4E4D47)
+    assert r.shape == (7000, 2)
+    assert len(set(map(tuple, r.tolist()))) == 7000
+    assert (r[:, 0] != r[:, 1]).all() and r.min() >= 0 and r.max() < 3000
+    assert np.array_equal(r, dlg.gen_powerlaw(3000, 7000, 2.1, 0x4D4E4D47))
+    assert not np.array_equal(r, dlg.gen_powerlaw(3000, 7000, 2.1, 0x4D4E4D48))
+    deg = np.bincount(r[:, 0], minlength=3000)
+    assert deg.min() >= 1 and deg.max() > 20    # every vertex has out-edges; heavy tail
+    assert (deg == 1).sum() > 1500              # most vertices have out-degree 1
+
+
+def test_grid_config_graph():
+    r = dlg.gen_grid(400, 0.55, 0.01, 20, 0x4D4E4D47)
+    s = set(map(tuple, r.tolist()))
+    assert all((b, a) in s for a, b in s)          # stored in both directions
+    assert 330_000 < len(r) < 370_000
+    d = np.abs(r[:, 0] // 400 - r[:, 1] // 400) + np.abs(r[:, 0] % 400 - r[:, 1] % 400)
+    assert d.min() >= 1 and d.max() <= 20
+    assert (d == 1).sum() > 0.97 * len(r)
+
+
+def test_tetmesh_config_graph():
+    r = dlg.gen_tetmesh(35)
+    # 3*35*36*36 lattice edges + one diagonal per face (3*35*35*36)
+    assert len(r) == 3 * 35 * 36 * 36 + 3 * 35 * 35 * 36
+    assert (r[:, 0] < r[:, 1]).all()
+    assert len(set(map(tuple, r.tolist()))) == len(r)
+    small = dlg.gen_tetmesh(1)
+    assert len(small) == 12 + 6   # cube edges + 6 face diagonals of the central tetrahedron
+
+
+def test_erdos_renyi_and_rmat():
+    e = dlg.gen_erdos_renyi(1000, 3000, 7)
+    assert e.shape == (3000, 2) and (e[:, 0] != e[:, 1]).all()
+    assert len(set(map(tuple, e.tolist()))) == 3000
+    m = dlg.gen_rmat(12, 20000, 0.57, 0.19, 0.19, 9)
+    assert m.shape == (20000, 2) and (m[:, 0] != m[:, 1]).all() and m.max() < 4096
+    assert len(set(map(tuple, m.tolist()))) == 20000
+    u = dlg.gen_rmat(10, 3000, 0.57, 0.19, 0.19, 9, undirected=True)
+    s = set(map(tuple, u.tolist()))
+    assert len(u) == 6000 and all((b, a) in s for a, b in s)
+
+
+def test_generator_argument_checks():
+    with pytest.raises(dlg.DlgError):
+        dlg.gen_powerlaw(1, 1, 2.1, 0)
+    with pytest.raises(dlg.DlgError):
+        dlg.gen_tetmesh(0)
+
+
+def test_pack_order_matches_signed_lexicographic():
+    rng = np.random.default_rng(1)
+    rows = rng.integers(-2**31, 2**31, size=(2000, 2), dtype=np.int64).astype(np.int32)
+    t = dlg.pack(rows)
+    order = np.lexsort((rows[:, 1], rows[:, 0]))
+    assert np.array_equal(np.argsort(t, kind="stable"), order) or np.array_equal(t[order], np.sort(t))
+    assert np.array_equal(dlg.unpack(t), rows)
+
+
+def test_no_gpu_context_fails_cleanly():
+    import ctypes
+    L = dlg.lib()
+    if L.dlg_ctx_create(0, ctypes.byref(ctypes.c_void_p())) == dlg.DLG_OK:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(dlg.DlgError) as e:
+        dlg.Ctx()
+    assert e.value.status in (dlg.DLG_DEVICE_ERROR, dlg.DLG_INVALID_INPUT)
+
+
+def test_cli_reports_error_without_gpu(tmp_path):
+    import ctypes
+    L = dlg.lib()
+    h = ctypes.c_void_p()
+    if L.dlg_ctx_create(0, ctypes.byref(h)) == dlg.DLG_OK:
+        L.dlg_ctx_destroy(h)
+        pytest.skip("a GPU is visible")
+    p = subprocess.run([os.path.join(dlg.BIN_DIR, "tc"), os.path.join(DATA, "hipc_2019.bin")],
+                       capture_output=True, text=True)
+    assert p.returncode != 0 and "failed" in p.stderr
