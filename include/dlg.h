@@ -198,6 +198,36 @@ dlg_status dlg_run_single_join_device(dlg_ctx *ctx, const int32_t *d_rows, int64
                                       const dlg_opts *o, dlg_result *res);
 void dlg_result_free(dlg_result *res);
 
+/* ------------------------------------------------------- batched form */
+/* No counterpart in the reference (one instance per process there).  `count`
+ * independent instances of one program advance together: their inputs are
+ * placed in disjoint id ranges and run as ONE fixpoint over the disjoint
+ * union, so every kernel launch serves every live instance.  Per-instance
+ * results are exact: rows, sizes and iteration counts equal the instance's
+ * plain run (an instance's iteration count is the first round in which it
+ * derived nothing; WCC instances stop by their own termination rule and are
+ * then frozen).  The summed id ranges of one batch must fit 2^32 ids
+ * (DLG_OVERFLOW otherwise: split the batch).  results[i].raw_join_rows is -1. */
+typedef enum dlg_prog { DLG_PROG_TC = 0, DLG_PROG_SG = 1, DLG_PROG_WCC = 2, DLG_PROG_SJ = 3 } dlg_prog;
+typedef struct dlg_batch_stats {
+    double device_seconds;   /* upload-to-resident wall time of the whole batch      */
+    int32_t rounds;          /* fixpoint rounds run (max over instances)             */
+    int64_t total_output;    /* sum of output_size                                    */
+    int64_t union_rows;      /* input rows of the union                               */
+} dlg_batch_stats;
+dlg_status dlg_batch_run(dlg_ctx *ctx, int prog, int32_t count, const int32_t *const *h_rows,
+                         const int64_t *n, const dlg_opts *o, dlg_result *results,
+                         dlg_batch_stats *stats);
+
+/* Autotuner: times the program on `h_rows` for every candidate in the grid
+ * load_factor x radix_bits x block_size (zero-terminated lists, NULL = the
+ * built-in grid) and writes the fastest setting to *best.  Results of every
+ * candidate are checked to be identical (DLG_DEVICE_ERROR otherwise). */
+dlg_status dlg_autotune(dlg_ctx *ctx, int prog, const int32_t *h_rows, int64_t n,
+                        const double *load_factors, const int32_t *radix_bits,
+                        const int32_t *block_sizes, int32_t repeats, dlg_opts *best,
+                        double *best_seconds);
+
 /* ------------------------------------------------------- input formats */
 /* Replaces parallel_read (common/parallel_io.cu:5-41) on one rank: raw
  * little-endian int32 pairs, rows = filesize / 8 (trailing bytes ignored).

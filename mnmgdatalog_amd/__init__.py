@@ -51,6 +51,13 @@ class Result(ctypes.Structure):
                 ("phase_seconds", c_double * 10)]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [("device_seconds", c_double), ("rounds", c_int32), ("total_output", c_int64),
+                ("union_rows", c_int64)]
+
+
+PROGS = {"tc": 0, "sg": 1, "wcc": 2, "single_join": 3}
+
 _LIB = None
 
 
@@ -100,6 +107,10 @@ def lib() -> ctypes.CDLL:
         "dlg_gen_erdos_renyi": (c_int, [c_int32, i64, c_uint64, POINTER(POINTER(c_int32)), POINTER(c_int64)]),
         "dlg_gen_rmat": (c_int, [c_int32, i64, c_double, c_double, c_double, c_uint64, c_int, POINTER(POINTER(c_int32)), POINTER(c_int64)]),
     }
+    sig["dlg_batch_run"] = (c_int, [vp, c_int, c_int32, POINTER(c_void_p), POINTER(c_int64), POINTER(Opts),
+                                     POINTER(Result), POINTER(BatchStats)])
+    sig["dlg_autotune"] = (c_int, [vp, c_int, vp, i64, POINTER(c_double), POINTER(c_int32), POINTER(c_int32),
+                                    c_int32, POINTER(Opts), POINTER(c_double)])
     for prog in ("tc", "sg", "wcc", "single_join"):
         sig[f"dlg_run_{prog}"] = (c_int, [vp, vp, i64, POINTER(Opts), POINTER(Result)])
         sig[f"dlg_run_{prog}_device"] = (c_int, [vp, vp, i64, POINTER(Opts), POINTER(Result)])
@@ -341,6 +352,46 @@ class Ctx:
 
     def kernel_stats_reset(self):
         self.check(self.L.dlg_kernel_stats_reset(self.h), "kernel_stats_reset")
+
+    # ---- batched form
+    def batch_run(self, prog: str, instances, keep_output: bool = True, **tune):
+        """Run many independent instances of one program as one fixpoint; returns (results, stats)."""
+        arrs = [np.ascontiguousarray(np.asarray(r, dtype=np.int32).reshape(-1, 2)) for r in instances]
+        B = len(arrs)
+        ptrs = (c_void_p * max(B, 1))(*[a.ctypes.data for a in arrs])
+        ns = (c_int64 * max(B, 1))(*[a.shape[0] for a in arrs])
+        o = Opts()
+        o.keep_output = 1 if keep_output else 0
+        for k, v in tune.items():
+            setattr(o, k, v)
+        res = (Result * max(B, 1))()
+        st = BatchStats()
+        self.check(self.L.dlg_batch_run(self.h, PROGS[prog], B, ptrs, ns, byref(o), res, byref(st)), "batch_run")
+        out = []
+        try:
+            for i in range(B):
+                r = res[i]
+                out.append({"iterations": r.iterations, "output_size": r.output_size, "secondary": r.secondary,
+                            "nodes": r.nodes, "rows": _rows_from(r.rows, r.n_rows) if r.rows else None})
+        finally:
+            for i in range(B):
+                self.L.dlg_result_free(byref(res[i]))
+        return out, {"device_seconds": st.device_seconds, "rounds": st.rounds,
+                     "total_output": st.total_output, "union_rows": st.union_rows}
+
+    def autotune(self, prog: str, rows: np.ndarray, load_factors=None, radix_bits=None, block_sizes=None,
+                 repeats: int = 1):
+        rows = np.ascontiguousarray(rows, dtype=np.int32).reshape(-1, 2)
+
+        def arr(vals, ctype):
+            return None if vals is None else (ctype * (len(vals) + 1))(*vals, 0)
+
+        best, sec = Opts(), c_double()
+        self.check(self.L.dlg_autotune(self.h, PROGS[prog], rows.ctypes.data, rows.shape[0],
+                                       arr(load_factors, c_double), arr(radix_bits, c_int32),
+                                       arr(block_sizes, c_int32), repeats, byref(best), byref(sec)), "autotune")
+        return {"load_factor": best.load_factor, "radix_bits": best.radix_bits,
+                "block_size": best.block_size, "seconds": sec.value}
 
     # ---- whole programs
     def run(self, prog: str, rows: np.ndarray, keep_output: bool = True, device_rows: int | None = None,

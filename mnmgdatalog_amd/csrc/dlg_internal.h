@@ -151,6 +151,15 @@ inline unsigned grid_for(u64 n, u64 per_block) {
     u64 g = (n + per_block - 1) / per_block;
     return (unsigned)(g ? g : 1);
 }
+// Grid-stride kernels run ctx->block_size threads per block (tunable) and at
+// most 4096 threads per CU.
+inline unsigned sgrid(const dlg_ctx *ctx, u64 n) {
+    u64 bs = (u64)ctx->block_size;
+    u64 g = (n + bs - 1) / bs;
+    u64 cap = (u64)ctx->num_cus * 4096 / bs;
+    if (g > cap) g = cap;
+    return (unsigned)(g ? g : 1);
+}
 
 #define DLG_LAUNCH(ctx, kid, bytes, kern, grid, block, shmem, ...)                        \
     do {                                                                                  \

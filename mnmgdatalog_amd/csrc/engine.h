@@ -1,0 +1,85 @@
+// engine.h -- fixpoint drivers shared by the plain (engine.hip), batched
+// (batch.hip) and served (serve.hip) forms.
+#pragma once
+
+#include <chrono>
+#include <utility>
+#include <vector>
+
+#include "dlg_internal.h"
+
+namespace dlg {
+
+struct Opts {
+    double lf;
+    int bits;
+    int max_it;
+    bool keep;
+    int strategy;
+    int block_size;  // 0 = the context's
+};
+Opts read_opts(dlg_ctx *ctx, const dlg_opts *o);
+
+// Applies a call's block size to the context for the duration of the call.
+struct BlockSizeScope {
+    dlg_ctx *ctx;
+    int saved;
+    BlockSizeScope(dlg_ctx *c, const Opts &o) : ctx(c), saved(c->block_size) {
+        if (o.block_size) ctx->block_size = o.block_size;
+    }
+    ~BlockSizeScope() { ctx->block_size = saved; }
+};
+
+// Phase timing with events on the context stream: mark(p) starts phase p.
+struct PhaseClock {
+    dlg_ctx *ctx;
+    std::vector<std::pair<int, hipEvent_t>> marks;
+    explicit PhaseClock(dlg_ctx *c) : ctx(c) {}
+    ~PhaseClock();
+    void mark(int ph);
+    void finish(double *out);  // adds seconds per phase into out[DLG_PH_COUNT]
+};
+
+typedef std::chrono::steady_clock Clock;
+inline double secs(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+}
+
+// Called once per fixpoint round with the round's new tuples N (sorted, unique,
+// disjoint from the full relation) -- the batched form counts them per instance.
+struct RoundHook {
+    virtual ~RoundHook() {}
+    virtual void on_round(dlg_ctx *ctx, const u64 *N, u64 nn, int round) = 0;
+};
+
+struct FixpointOut {
+    DBuf<u64> full;   // final relation, sorted
+    u64 nfull = 0;
+    int iterations = 0;
+    i64 raw = 0;
+    explicit FixpointOut(dlg_ctx *c) : full(c) {}
+};
+
+// Relations from int32 device rows (n rows).  `hook` may be null.
+void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
+                 RoundHook *hook, FixpointOut &out);
+void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
+                 RoundHook *hook, FixpointOut &out);
+void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
+                 FixpointOut &out);
+
+// WCC set-up shared by the plain and batched forms: symmetric deduplicated
+// edges, index over them (groups = nodes) and dense neighbour ids.
+struct WccGraph {
+    Index ix;
+    DBuf<u32> nbr;
+    u64 V = 0, ne = 0;
+    explicit WccGraph(dlg_ctx *c) : ix(c), nbr(c) {}
+};
+void wcc_graph_build(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
+                     WccGraph &g);
+
+void download_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, int32_t *h_rows);
+void check_rows_arg(int64_t n, const void *rows);
+
+}  // namespace dlg

@@ -12,62 +12,57 @@
 //   SJ  single_join.cu:100-283  unique((R⁻¹) ⋈ R) over deduplicated R, R⁻¹
 // Full relation F: sorted array; N is found by one fused compaction
 // ("first of an equal run AND not found in F"), then F' = merge(F, N).
-#include "dlg_internal.h"
+#include "engine.h"
 
 #include <algorithm>
-#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 
 namespace dlg {
 
-// Phase timing with events on the context stream: mark(p) starts phase p.
-struct PhaseClock {
-    dlg_ctx *ctx;
-    std::vector<std::pair<int, hipEvent_t>> marks;
-    explicit PhaseClock(dlg_ctx *c) : ctx(c) {}
-    ~PhaseClock() {
-        for (auto &m : marks) ctx->event_free.push_back(m.second);
-    }
-    void mark(int ph) {
-        if (ctx->event_free.empty()) {
-            hipEvent_t e;
-            HIPCHK(hipEventCreate(&e));
-            ctx->event_free.push_back(e);
-        }
-        hipEvent_t e = ctx->event_free.back();
-        ctx->event_free.pop_back();
-        HIPCHK(hipEventRecord(e, ctx->stream));
-        marks.push_back({ph, e});
-    }
-    void finish(double *out) {
-        mark(-1);
-        HIPCHK(hipEventSynchronize(marks.back().second));
-        for (size_t i = 1; i < marks.size(); i++) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, marks[i - 1].second, marks[i].second));
-            if (marks[i - 1].first >= 0) out[marks[i - 1].first] += ms * 1e-3;
-        }
-    }
-};
+PhaseClock::~PhaseClock() {
+    for (auto &m : marks) ctx->event_free.push_back(m.second);
+}
 
-struct Opts {
-    double lf;
-    int bits;
-    int max_it;
-    bool keep;
-};
+void PhaseClock::mark(int ph) {
+    if (ctx->event_free.empty()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        ctx->event_free.push_back(e);
+    }
+    hipEvent_t e = ctx->event_free.back();
+    ctx->event_free.pop_back();
+    HIPCHK(hipEventRecord(e, ctx->stream));
+    marks.push_back({ph, e});
+}
 
-static Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
-    Opts r{ctx->load_factor, ctx->radix_bits, 0, true};
+void PhaseClock::finish(double *out) {
+    mark(-1);
+    HIPCHK(hipEventSynchronize(marks.back().second));
+    for (size_t i = 1; i < marks.size(); i++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, marks[i - 1].second, marks[i].second));
+        if (marks[i - 1].first >= 0) out[marks[i - 1].first] += ms * 1e-3;
+    }
+}
+
+Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
+    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0};
     if (o) {
+        if (o->load_factor < 0 || o->load_factor > 0.95)
+            fail(DLG_INVALID_INPUT, "load_factor out of (0, 0.95]");
+        if (o->radix_bits != 0 && (o->radix_bits < 4 || o->radix_bits > 8))
+            fail(DLG_INVALID_INPUT, "radix_bits out of 4..8");
+        if (o->strategy < 0 || o->strategy > 2) fail(DLG_INVALID_INPUT, "strategy out of 0..2");
+        if (o->max_iterations < 0) fail(DLG_INVALID_INPUT, "negative max_iterations");
+        if (o->block_size != 0 && (o->block_size < 64 || o->block_size > 1024 || o->block_size % 64))
+            fail(DLG_INVALID_INPUT, "block_size must be 64..1024 and a multiple of 64");
+        r.block_size = o->block_size;
         if (o->load_factor > 0) r.lf = o->load_factor;
         if (o->radix_bits > 0) r.bits = o->radix_bits;
         r.max_it = o->max_iterations;
         r.keep = o->keep_output != 0;
-        if (o->load_factor < 0 || o->load_factor > 0.95) fail(DLG_INVALID_INPUT, "load_factor out of (0, 0.95]");
-        if (o->radix_bits != 0 && (o->radix_bits < 4 || o->radix_bits > 8))
-            fail(DLG_INVALID_INPUT, "radix_bits out of 4..8");
+        r.strategy = o->strategy;
     }
     return r;
 }
@@ -92,34 +87,54 @@ static u64 join_into(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, int
     return total;
 }
 
-static void download_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, dlg_result *res) {
-    res->n_rows = (int64_t)n;
-    res->rows = (int32_t *)std::malloc(sizeof(int32_t) * 2 * (n ? n : 1));
-    if (!res->rows) fail(DLG_OUT_OF_MEMORY, "host allocation of result rows");
+void download_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, int32_t *h_rows) {
     if (!n) return;
     DBuf<int32_t> r(ctx, 2 * n);
     unpack_rows(ctx, d, n, orient, r.p);
-    HIPCHK(hipMemcpyAsync(res->rows, r.p, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h_rows, r.p, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
 }
 
-static void check_n(int64_t n, const void *rows) {
+static void result_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, dlg_result *res) {
+    res->n_rows = (int64_t)n;
+    res->rows = (int32_t *)std::malloc(sizeof(int32_t) * 2 * (n ? n : 1));
+    if (!res->rows) fail(DLG_OUT_OF_MEMORY, "host allocation of result rows");
+    download_rows(ctx, d, n, orient, res->rows);
+}
+
+void check_rows_arg(int64_t n, const void *rows) {
     if (n < 0) fail(DLG_INVALID_INPUT, "negative row count");
     if (n > 0 && !rows) fail(DLG_INVALID_INPUT, "null rows with n > 0");
     if ((u64)n >= 0x7fffffffull) fail(DLG_OVERFLOW, "more than 2^31-2 input rows");
 }
 
-typedef std::chrono::steady_clock Clock;
-static double secs(Clock::time_point a, Clock::time_point b) {
-    return std::chrono::duration<double>(b - a).count();
+// New tuples of a round from the sorted join output: N = unique(J) \ F, then F ∪= N.
+static u64 absorb_round(dlg_ctx *ctx, DBuf<u64> &J, u64 nj, DBuf<u64> &tmp, DBuf<u64> &N,
+                        DBuf<u64> &full, DBuf<u64> &full2, u64 &nfull, const Opts &o, PhaseClock &pc) {
+    u64 nn = 0;
+    pc.mark(DLG_PH_DEDUP);
+    if (nj) {
+        radix_sort(ctx, J.p, nj, tmp, o.bits);
+        pc.mark(DLG_PH_MERGE);
+        N.ensure(nj);
+        nn = select(ctx, SEL_UNIQUE_NOT_IN, J.p, nj, full.p, nfull, N.p);
+    }
+    pc.mark(DLG_PH_MERGE);
+    if (nn) {
+        full2.ensure(nfull + nn);
+        merge(ctx, full.p, nfull, N.p, nn, full2.p);
+        full.swap(full2);
+        nfull += nn;
+    }
+    return nn;
 }
 
 // ------------------------------------------------------------------ TC
-void run_tc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
-    PhaseClock pc(ctx);
-    auto t0 = Clock::now();
+void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
+                 RoundHook *hook, FixpointOut &out) {
     pc.mark(DLG_PH_INIT);
-    DBuf<u64> edge(ctx, n), delta(ctx, n), J(ctx, n), full(ctx), full2(ctx), N(ctx), tmp(ctx);
+    DBuf<u64> edge(ctx, n), delta(ctx, n), J(ctx, n), full2(ctx), N(ctx), tmp(ctx);
+    DBuf<u64> &full = out.full;
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, edge.p);   // e(src, dst) keyed by src
     pack_rows(ctx, d_rows, n, DLG_ORIENT_VK, J.p);      // Δ0 rows (dst, src)
     pc.mark(DLG_PH_HASHTABLE);
@@ -134,48 +149,28 @@ void run_tc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_resul
     u64 nfull = nd;
     JoinScratch js(ctx);
     int it = 0;
-    i64 raw = 0;
     while (true) {
         pc.mark(DLG_PH_JOIN);
         u64 nj = join_into(ctx, ix, delta.p, nd, DLG_PROJ_HV_PV, js, J);   // (z, x)
-        raw += (i64)nj;
-        pc.mark(DLG_PH_DEDUP);
-        u64 nn = 0;
-        if (nj) {
-            radix_sort(ctx, J.p, nj, tmp, o.bits);
-            pc.mark(DLG_PH_MERGE);
-            N.ensure(nj);
-            nn = select(ctx, SEL_UNIQUE_NOT_IN, J.p, nj, full.p, nfull, N.p);
-        }
-        pc.mark(DLG_PH_MERGE);
-        if (nn) {
-            full2.ensure(nfull + nn);
-            merge(ctx, full.p, nfull, N.p, nn, full2.p);
-            full.swap(full2);
-            nfull += nn;
-        }
+        out.raw += (i64)nj;
+        u64 nn = absorb_round(ctx, J, nj, tmp, N, full, full2, nfull, o, pc);
         it++;
+        if (hook) hook->on_round(ctx, N.p, nn, it);
         if (nn == 0) break;                         // |F| unchanged (tc.cu:294-300)
         delta.swap(N);
         nd = nn;
         if (o.max_it > 0 && it >= o.max_it) break;
     }
-    pc.mark(DLG_PH_FINAL);
-    ctx->sync();
-    res->device_seconds = secs(t0, Clock::now());
-    res->iterations = it;
-    res->output_size = (int64_t)nfull;
-    res->raw_join_rows = raw;
-    if (o.keep) download_rows(ctx, full.p, nfull, DLG_ORIENT_VK, res);   // (src, dst)
-    pc.finish(res->phase_seconds);
+    out.nfull = nfull;
+    out.iterations = it;
 }
 
 // ------------------------------------------------------------------ SG
-void run_sg(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
-    PhaseClock pc(ctx);
-    auto t0 = Clock::now();
+void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
+                 RoundHook *hook, FixpointOut &out) {
     pc.mark(DLG_PH_INIT);
-    DBuf<u64> edge(ctx), J(ctx, n), T(ctx), delta(ctx), full(ctx), full2(ctx), N(ctx), tmp(ctx);
+    DBuf<u64> edge(ctx), J(ctx, n), T(ctx), delta(ctx), full2(ctx), N(ctx), tmp(ctx);
+    DBuf<u64> &full = out.full;
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, J.p);
     pc.mark(DLG_PH_DEDUP);
     u64 ne = sort_unique_into(ctx, J, n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
@@ -183,10 +178,10 @@ void run_sg(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_resul
     Index ix(ctx);
     index_build(ctx, ix, edge.p, ne, o.lf, true);
     JoinScratch js(ctx);
-    i64 raw = 0;
     pc.mark(DLG_PH_JOIN);
     u64 nb = join_into(ctx, ix, edge.p, ne, DLG_PROJ_HV_PV, js, J);   // (y, x)
-    raw += (i64)nb;
+    out.raw += (i64)nb;
+    edge.release();
     pc.mark(DLG_PH_DEDUP);
     u64 nfull = sort_unique_into(ctx, J, nb, tmp, full, SEL_UNIQUE_NODIAG, nullptr, 0, o.bits);
     pc.mark(DLG_PH_MERGE);
@@ -197,49 +192,29 @@ void run_sg(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_resul
     while (true) {
         pc.mark(DLG_PH_JOIN);
         u64 n1 = join_into(ctx, ix, delta.p, nd, DLG_PROJ_PV_HV, js, J);  // (b, x)
-        raw += (i64)n1;
+        out.raw += (i64)n1;
         pc.mark(DLG_PH_DEDUP);
         u64 nt = sort_unique_into(ctx, J, n1, tmp, T, SEL_UNIQUE, nullptr, 0, o.bits);
         pc.mark(DLG_PH_JOIN);
         u64 n2 = join_into(ctx, ix, T.p, nt, DLG_PROJ_PV_HV, js, J);      // (x, y)
-        raw += (i64)n2;
-        pc.mark(DLG_PH_DEDUP);
-        u64 nn = 0;
-        if (n2) {
-            radix_sort(ctx, J.p, n2, tmp, o.bits);
-            pc.mark(DLG_PH_MERGE);
-            N.ensure(n2);
-            nn = select(ctx, SEL_UNIQUE_NOT_IN, J.p, n2, full.p, nfull, N.p);
-        }
-        pc.mark(DLG_PH_MERGE);
-        if (nn) {
-            full2.ensure(nfull + nn);
-            merge(ctx, full.p, nfull, N.p, nn, full2.p);
-            full.swap(full2);
-            nfull += nn;
-        }
+        out.raw += (i64)n2;
+        u64 nn = absorb_round(ctx, J, n2, tmp, N, full, full2, nfull, o, pc);
         it++;
+        if (hook) hook->on_round(ctx, N.p, nn, it);
         if (nn == 0) break;                         // sg.cu:376-389
         delta.swap(N);
         nd = nn;
         if (o.max_it > 0 && it >= o.max_it) break;
     }
-    pc.mark(DLG_PH_FINAL);
-    ctx->sync();
-    res->device_seconds = secs(t0, Clock::now());
-    res->iterations = it;
-    res->output_size = (int64_t)nfull;
-    res->raw_join_rows = raw;
-    if (o.keep) download_rows(ctx, full.p, nfull, DLG_ORIENT_KV, res);   // (x, y)
-    pc.finish(res->phase_seconds);
+    out.nfull = nfull;
+    out.iterations = it;
 }
 
 // ------------------------------------------------------------------ single join
-void run_sj(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
-    PhaseClock pc(ctx);
-    auto t0 = Clock::now();
+void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
+                 FixpointOut &out) {
     pc.mark(DLG_PH_INIT);
-    DBuf<u64> a(ctx, n), R(ctx), Rr(ctx), J(ctx), U(ctx), tmp(ctx);
+    DBuf<u64> a(ctx, n), R(ctx), Rr(ctx), J(ctx), tmp(ctx);
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, a.p);
     pc.mark(DLG_PH_DEDUP);
     u64 nr = sort_unique_into(ctx, a, n, tmp, R, SEL_UNIQUE, nullptr, 0, o.bits);
@@ -251,48 +226,93 @@ void run_sj(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_resul
     pc.mark(DLG_PH_HASHTABLE);
     Index ix(ctx);
     index_build(ctx, ix, R.p, nr, o.lf, true);
+    R.release();
     pc.mark(DLG_PH_JOIN);
     JoinScratch js(ctx);
     u64 nj = join_into(ctx, ix, Rr.p, nrr, DLG_PROJ_HV_PV, js, J);   // (c, a)
+    out.raw += (i64)nj;
     pc.mark(DLG_PH_DEDUP);
-    u64 nu = sort_unique_into(ctx, J, nj, tmp, U, SEL_UNIQUE, nullptr, 0, o.bits);
+    out.nfull = sort_unique_into(ctx, J, nj, tmp, out.full, SEL_UNIQUE, nullptr, 0, o.bits);
+    out.iterations = 1;
+}
+
+// ------------------------------------------------------------------ WCC
+void wcc_graph_build(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
+                     WccGraph &g) {
+    pc.mark(DLG_PH_INIT);
+    DBuf<u64> sym(ctx, 2 * n), edge(ctx), tmp(ctx);
+    pack_sym_rows(ctx, d_rows, n, sym.p);                  // E ∪ E⁻¹ (wcc.cu:140-160)
+    pc.mark(DLG_PH_DEDUP);
+    g.ne = sort_unique_into(ctx, sym, 2 * n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
+    sym.release();
+    tmp.release();
+    pc.mark(DLG_PH_HASHTABLE);
+    index_build(ctx, g.ix, edge.p, g.ne, o.lf, true);
+    edge.release();
+    g.V = g.ix.groups;
+    g.nbr.ensure(g.ne ? g.ne : 1);
+    index_lookup_values(ctx, g.ix, g.nbr.p);               // neighbour dense ids
+}
+
+static void run_tc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+    PhaseClock pc(ctx);
+    auto t0 = Clock::now();
+    FixpointOut f(ctx);
+    tc_fixpoint(ctx, d_rows, n, o, pc, nullptr, f);
+    pc.mark(DLG_PH_FINAL);
+    ctx->sync();
+    res->device_seconds = secs(t0, Clock::now());
+    res->iterations = f.iterations;
+    res->output_size = (int64_t)f.nfull;
+    res->raw_join_rows = f.raw;
+    if (o.keep) result_rows(ctx, f.full.p, f.nfull, DLG_ORIENT_VK, res);   // (src, dst)
+    pc.finish(res->phase_seconds);
+}
+
+static void run_sg(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+    PhaseClock pc(ctx);
+    auto t0 = Clock::now();
+    FixpointOut f(ctx);
+    sg_fixpoint(ctx, d_rows, n, o, pc, nullptr, f);
+    pc.mark(DLG_PH_FINAL);
+    ctx->sync();
+    res->device_seconds = secs(t0, Clock::now());
+    res->iterations = f.iterations;
+    res->output_size = (int64_t)f.nfull;
+    res->raw_join_rows = f.raw;
+    if (o.keep) result_rows(ctx, f.full.p, f.nfull, DLG_ORIENT_KV, res);   // (x, y)
+    pc.finish(res->phase_seconds);
+}
+
+static void run_sj(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+    PhaseClock pc(ctx);
+    auto t0 = Clock::now();
+    FixpointOut f(ctx);
+    sj_fixpoint(ctx, d_rows, n, o, pc, f);
     pc.mark(DLG_PH_FINAL);
     ctx->sync();
     res->device_seconds = secs(t0, Clock::now());
     res->iterations = 1;
-    res->output_size = (int64_t)nu;
-    res->raw_join_rows = (i64)nj;
-    if (o.keep) download_rows(ctx, U.p, nu, DLG_ORIENT_VK, res);   // (a, c)
+    res->output_size = (int64_t)f.nfull;
+    res->raw_join_rows = f.raw;
+    if (o.keep) result_rows(ctx, f.full.p, f.nfull, DLG_ORIENT_VK, res);   // (a, c)
     pc.finish(res->phase_seconds);
 }
 
-// ------------------------------------------------------------------ WCC
-void run_wcc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+static void run_wcc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
     PhaseClock pc(ctx);
     auto t0 = Clock::now();
-    pc.mark(DLG_PH_INIT);
-    DBuf<u64> sym(ctx, 2 * n), edge(ctx), tmp(ctx), out(ctx);
-    pack_sym_rows(ctx, d_rows, n, sym.p);                  // E ∪ E⁻¹ (wcc.cu:140-160)
-    pc.mark(DLG_PH_DEDUP);
-    u64 ne = sort_unique_into(ctx, sym, 2 * n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
-    sym.release();
-    tmp.release();
-    pc.mark(DLG_PH_HASHTABLE);
-    Index ix(ctx);
-    index_build(ctx, ix, edge.p, ne, o.lf, true);
-    edge.release();
-    u64 V = ix.groups;
-    DBuf<u32> nbr(ctx, ne ? ne : 1);
-    index_lookup_values(ctx, ix, nbr.p);                   // neighbour dense ids
+    WccGraph g(ctx);
+    wcc_graph_build(ctx, d_rows, n, o, pc, g);
     pc.mark(DLG_PH_INIT);
     WccState s(ctx);
-    wcc_setup(ctx, s, V);
-    u64 nf = V;
-    u64 prev = V;                                          // wcc.cu:236-242
+    wcc_setup(ctx, s, g.V);
+    u64 nf = g.V;
+    u64 prev = g.V;                                        // wcc.cu:236-242
     int it = 0;
     while (true) {
         pc.mark(DLG_PH_JOIN);
-        u64 cur = nf ? wcc_round(ctx, s, ix, nbr.p, nf, (u32)(it + 1)) : 0;
+        u64 cur = nf ? wcc_round(ctx, s, g.ix, g.nbr.p, nf, (u32)(it + 1)) : 0;
         it++;
         nf = cur;
         if (cur == prev) break;                            // wcc.cu:366-383
@@ -300,15 +320,16 @@ void run_wcc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_resu
         if (o.max_it > 0 && it >= o.max_it) break;
     }
     pc.mark(DLG_PH_FINAL);
+    DBuf<u64> out(ctx);
     i64 largest = -1;
-    u64 ncomp = wcc_finalize(ctx, s, ix, out, &largest);
+    u64 ncomp = wcc_finalize(ctx, s, g.ix, out, &largest);
     ctx->sync();
     res->device_seconds = secs(t0, Clock::now());
     res->iterations = it;
     res->output_size = (int64_t)ncomp;
     res->secondary = largest;
-    res->nodes = (int64_t)V;
-    if (o.keep) download_rows(ctx, out.p, ncomp, DLG_ORIENT_KV, res);   // (label, 0)
+    res->nodes = (int64_t)g.V;
+    if (o.keep) result_rows(ctx, out.p, ncomp, DLG_ORIENT_KV, res);   // (label, 0)
     pc.finish(res->phase_seconds);
 }
 
@@ -329,8 +350,9 @@ static dlg_status run_device(dlg_ctx *ctx, runner_t fn, const int32_t *d_rows, i
     if (!ctx || !res) return DLG_INVALID_INPUT;
     std::memset(res, 0, sizeof(*res));
     API_BEGIN(ctx)
-    check_n(n, d_rows);
+    check_rows_arg(n, d_rows);
     Opts op = read_opts(ctx, o);
+    BlockSizeScope bss(ctx, op);
     res->input_rows = n;
     fn(ctx, d_rows, (u64)n, op, res);
     API_END(ctx)
@@ -341,8 +363,9 @@ static dlg_status run_host(dlg_ctx *ctx, runner_t fn, const int32_t *h_rows, int
     if (!ctx || !res) return DLG_INVALID_INPUT;
     std::memset(res, 0, sizeof(*res));
     API_BEGIN(ctx)
-    check_n(n, h_rows);
+    check_rows_arg(n, h_rows);
     Opts op = read_opts(ctx, o);
+    BlockSizeScope bss(ctx, op);
     res->input_rows = n;
     auto t0 = Clock::now();
     DBuf<int32_t> d(ctx, 2 * (size_t)(n ? n : 1));

@@ -83,12 +83,12 @@ void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double lf, bool
     ix.nslots = S;
     ix.slots.ensure(S);
     fill_u64(ctx, ix.slots.p, S, EMPTY_SLOT);
-    unsigned g = (unsigned)std::min<u64>((G + 255) / 256, (u64)ctx->num_cus * 16);
-    DLG_LAUNCH(ctx, DLG_K_HASH_BUILD, 8.0 * S + 12.0 * G + 8.0 * G, hash_insert_k, g, 256, 0,
+    DLG_LAUNCH(ctx, DLG_K_HASH_BUILD, 8.0 * S + 12.0 * G + 8.0 * G, hash_insert_k, sgrid(ctx, G),
+               ctx->block_size, 0,
                (const u64 *)ix.rows.p, (const u32 *)ix.gstart.p, G, ix.slots.p, S - 1);
 }
 
-__global__ __launch_bounds__(256) void join_count_k(const u64 *probe, u64 n, const u64 *slots,
+__global__ __launch_bounds__(1024) void join_count_k(const u64 *probe, u64 n, const u64 *slots,
                                                     u64 mask, const u32 *gstart, u32 *cnt,
                                                     u32 *pstart) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -110,10 +110,9 @@ u64 join_count(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScrat
     js.cnt.ensure(n);
     js.pstart.ensure(n);
     js.offs.ensure(n + 1);
-    unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 32);
     // probe read + one slot line per probe + gstart pair + count/start writes
-    DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 8.0 * n + 8.0 * n + 8.0 * n, join_count_k, g, 256,
-               0, probe, n, (const u64 *)ix.slots.p, ix.nslots - 1, (const u32 *)ix.gstart.p,
+    DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 8.0 * n + 8.0 * n + 8.0 * n, join_count_k,
+               sgrid(ctx, n), ctx->block_size, 0, probe, n, (const u64 *)ix.slots.p, ix.nslots - 1, (const u32 *)ix.gstart.p,
                js.cnt.p, js.pstart.p);
     return scan_u32_u64(ctx, js.cnt.p, n, js.offs.p);
 }
@@ -169,8 +168,8 @@ __global__ void lookup_values_k(const u64 *rows, u64 n, const u64 *slots, u64 ma
 
 void index_lookup_values(dlg_ctx *ctx, const Index &ix, u32 *out) {
     if (ix.n == 0) return;
-    unsigned g = (unsigned)std::min<u64>((ix.n + 255) / 256, (u64)ctx->num_cus * 32);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * ix.n, lookup_values_k, g, 256, 0, (const u64 *)ix.rows.p,
+    DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * ix.n, lookup_values_k, sgrid(ctx, ix.n), ctx->block_size, 0,
+               (const u64 *)ix.rows.p,
                ix.n, (const u64 *)ix.slots.p, ix.nslots - 1, out);
 }
 

@@ -36,24 +36,17 @@ __global__ void copy_u64_k(u64 *dst, const u64 *src, u64 n) {
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
-static unsigned stream_grid(dlg_ctx *ctx, u64 n) {
-    u64 g = (n + 255) / 256;
-    u64 cap = (u64)ctx->num_cus * 16;
-    if (g > cap) g = cap;
-    return (unsigned)(g ? g : 1);
-}
-
 void fill_u64(dlg_ctx *ctx, u64 *p, u64 n, u64 v) {
     if (!n) return;
-    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, fill_u64_k, stream_grid(ctx, n), 256, 0, p, n, v);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, fill_u64_k, sgrid(ctx, n), ctx->block_size, 0, p, n, v);
 }
 void fill_u32(dlg_ctx *ctx, u32 *p, u64 n, u32 v) {
     if (!n) return;
-    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * n, fill_u32_k, stream_grid(ctx, n), 256, 0, p, n, v);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * n, fill_u32_k, sgrid(ctx, n), ctx->block_size, 0, p, n, v);
 }
 void copy_u64(dlg_ctx *ctx, u64 *dst, const u64 *src, u64 n) {
     if (!n) return;
-    DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * n, copy_u64_k, stream_grid(ctx, n), 256, 0, dst, src, n);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * n, copy_u64_k, sgrid(ctx, n), ctx->block_size, 0, dst, src, n);
 }
 
 // =============================================================== pack / unpack
@@ -90,17 +83,17 @@ __global__ void unpack_k(const u64 *t, u64 n, int orient, int2 *rows) {
 
 void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out) {
     if (!n) return;
-    DLG_LAUNCH(ctx, DLG_K_PACK, 16.0 * n, pack_k, stream_grid(ctx, n), 256, 0,
+    DLG_LAUNCH(ctx, DLG_K_PACK, 16.0 * n, pack_k, sgrid(ctx, n), ctx->block_size, 0,
                (const int2 *)rows, n, orient, out);
 }
 void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out) {
     if (!n) return;
-    DLG_LAUNCH(ctx, DLG_K_PACK, 24.0 * n, pack_sym_k, stream_grid(ctx, n), 256, 0,
+    DLG_LAUNCH(ctx, DLG_K_PACK, 24.0 * n, pack_sym_k, sgrid(ctx, n), ctx->block_size, 0,
                (const int2 *)rows, n, out);
 }
 void unpack_rows(dlg_ctx *ctx, const u64 *t, u64 n, int orient, int32_t *rows) {
     if (!n) return;
-    DLG_LAUNCH(ctx, DLG_K_PACK, 16.0 * n, unpack_k, stream_grid(ctx, n), 256, 0, t, n, orient,
+    DLG_LAUNCH(ctx, DLG_K_PACK, 16.0 * n, unpack_k, sgrid(ctx, n), ctx->block_size, 0, t, n, orient,
                (int2 *)rows);
 }
 

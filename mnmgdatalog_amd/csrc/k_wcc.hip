@@ -122,16 +122,12 @@ __global__ void wcc_emit_k(const u32 *flags, const u64 *offs, u64 V, const u64 *
         if (flags[i]) out[offs[i]] = tmake(tkey(rows[gstart[i]]), 0x80000000u);
 }
 
-static unsigned sgrid(dlg_ctx *ctx, u64 n) {
-    return (unsigned)std::max<u64>(1, std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 16));
-}
-
 // One round.  Returns |Δ'| (the new frontier size); the frontier is replaced.
 u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf, u32 round) {
     s.cnt.ensure(nf);
     s.pstart.ensure(nf);
     s.offs.ensure(nf + 1);
-    DLG_LAUNCH(ctx, DLG_K_WCC_RELAX, 16.0 * nf, wcc_degree_k, sgrid(ctx, nf), 256, 0,
+    DLG_LAUNCH(ctx, DLG_K_WCC_RELAX, 16.0 * nf, wcc_degree_k, sgrid(ctx, nf), ctx->block_size, 0,
                (const u32 *)s.fnode.p, nf, (const u32 *)ix.gstart.p, s.cnt.p, s.pstart.p);
     u64 total = scan_u32_u64(ctx, s.cnt.p, nf, s.offs.p);
     fill_u32(ctx, s.counter.p, 1, 0u);
@@ -147,7 +143,7 @@ u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf
     ctx->sync();
     h = *(u32 *)ctx->h_scalars;
     if (h) {
-        DLG_LAUNCH(ctx, DLG_K_WCC_RELAX, 12.0 * h, wcc_gather_k, sgrid(ctx, h), 256, 0,
+        DLG_LAUNCH(ctx, DLG_K_WCC_RELAX, 12.0 * h, wcc_gather_k, sgrid(ctx, h), ctx->block_size, 0,
                    (const u32 *)s.nnode.p, h, (const u32 *)s.L.p, s.flabel.p);
     }
     s.fnode.swap(s.nnode);
@@ -163,7 +159,7 @@ void wcc_setup(dlg_ctx *ctx, WccState &s, u64 V) {
     s.nnode.ensure(V);
     s.counter.ensure(4);
     if (V)
-        DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * V, wcc_init_k, sgrid(ctx, V), 256, 0, s.L.p, s.fnode.p,
+        DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * V, wcc_init_k, sgrid(ctx, V), ctx->block_size, 0, s.L.p, s.fnode.p,
                    s.flabel.p, s.mark.p, V);
 }
 
@@ -176,16 +172,16 @@ u64 wcc_finalize(dlg_ctx *ctx, WccState &s, const Index &ix, DBuf<u64> &out, i64
     DBuf<u64> offs(ctx, V + 1);
     fill_u32(ctx, ccount.p, V, 0u);
     fill_u32(ctx, s.counter.p, 1, 0u);
-    DLG_LAUNCH(ctx, DLG_K_WCC_FINAL, 12.0 * V, wcc_comp_count_k, sgrid(ctx, V), 256, 0,
+    DLG_LAUNCH(ctx, DLG_K_WCC_FINAL, 12.0 * V, wcc_comp_count_k, sgrid(ctx, V), ctx->block_size, 0,
                (const u32 *)s.L.p, V, ccount.p);
-    DLG_LAUNCH(ctx, DLG_K_WCC_FINAL, 8.0 * V, wcc_comp_flags_k, sgrid(ctx, V), 256, 0,
+    DLG_LAUNCH(ctx, DLG_K_WCC_FINAL, 8.0 * V, wcc_comp_flags_k, sgrid(ctx, V), ctx->block_size, 0,
                (const u32 *)ccount.p, V, flags.p, s.counter.p);
     u64 ncomp = scan_u32_u64(ctx, flags.p, V, offs.p);
     HIPCHK(hipMemcpyAsync(ctx->h_scalars, s.counter.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
     *largest = (i64) * (u32 *)ctx->h_scalars;
     out.ensure(ncomp ? ncomp : 1);
-    DLG_LAUNCH(ctx, DLG_K_WCC_FINAL, 16.0 * V, wcc_emit_k, sgrid(ctx, V), 256, 0,
+    DLG_LAUNCH(ctx, DLG_K_WCC_FINAL, 16.0 * V, wcc_emit_k, sgrid(ctx, V), ctx->block_size, 0,
                (const u32 *)flags.p, (const u64 *)offs.p, V, (const u64 *)ix.rows.p,
                (const u32 *)ix.gstart.p, out.p);
     return ncomp;
