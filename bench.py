@@ -57,7 +57,7 @@ def workload_rows(name):
 
 
 # bounded CPU sample per workload: number of fixpoint iterations the oracle runs
-CPU_SAMPLE_ITERS = {"sg_mesh": 9, "tc_powerlaw": 0, "tc_usroad": 40, "wcc_grid": 0, "single_join": 0}
+CPU_SAMPLE_ITERS = {"sg_mesh": 3, "tc_powerlaw": 0, "tc_usroad": 40, "wcc_grid": 0, "single_join": 0}
 
 
 def cpu_baseline(name, prog, rows, ctx, threads):

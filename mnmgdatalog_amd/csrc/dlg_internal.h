@@ -239,6 +239,9 @@ u64 scan_u32_u64(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out);
 void scan_u32_u64_async(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *d_total);
 // Stable LSD radix sort of u64 keys; result in `data`.  `tmp` is scratch of >= n.
 void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits = 0);
+// Same on a pool buffer: the sorted keys may end in the other buffer, in which
+// case `data` and `tmp` swap ownership (no copy).
+void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits = 0);
 // Compaction modes.
 enum SelectMode {
     SEL_UNIQUE = 0,         // keep first of each run of equal elements
@@ -248,8 +251,18 @@ enum SelectMode {
     SEL_KEY_HEAD = 4,       // output INDEX of first element of each equal-key run
 };
 // Writes kept elements (or indices for SEL_KEY_HEAD, as u32) to out; returns count.
+// `out` must not alias `in`.
 u64 select(dlg_ctx *ctx, int mode, const u64 *in, u64 n, const u64 *other, u64 n_other,
            void *out);
+// A set of sorted, pairwise disjoint runs (the full relation kept in tiers).
+constexpr int MAX_RUNS = 24;
+struct RunSet {
+    const u64 *p[MAX_RUNS];
+    u64 n[MAX_RUNS];
+    int k;
+};
+// Unique elements of sorted `in` that no run of `rs` contains.
+u64 select_unique_not_in_runs(dlg_ctx *ctx, const u64 *in, u64 n, const RunSet &rs, u64 *out);
 // Merge of sorted a, b into out (a first on ties).
 void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out);
 void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out);

@@ -71,7 +71,7 @@ Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
 static u64 sort_unique_into(dlg_ctx *ctx, DBuf<u64> &buf, u64 n, DBuf<u64> &tmp, DBuf<u64> &out,
                             int mode, const u64 *other, u64 n_other, int bits) {
     if (n == 0) return 0;
-    radix_sort(ctx, buf.p, n, tmp, bits);
+    radix_sort(ctx, buf, n, tmp, bits);
     out.ensure(n);
     return select(ctx, mode, buf.p, n, other, n_other, out.p);
 }
@@ -108,33 +108,89 @@ void check_rows_arg(int64_t n, const void *rows) {
     if ((u64)n >= 0x7fffffffull) fail(DLG_OVERFLOW, "more than 2^31-2 input rows");
 }
 
-// New tuples of a round from the sorted join output: N = unique(J) \ F, then F ∪= N.
-static u64 absorb_round(dlg_ctx *ctx, DBuf<u64> &J, u64 nj, DBuf<u64> &tmp, DBuf<u64> &N,
-                        DBuf<u64> &full, DBuf<u64> &full2, u64 &nfull, const Opts &o, PhaseClock &pc) {
-    u64 nn = 0;
+// The full relation F.  Strategy 1 keeps one sorted array and merges every
+// round's new tuples into it (the reference's scheme, tc.cu:268-286): each
+// round rewrites all of F.  Otherwise F is a list of sorted, disjoint runs
+// whose sizes fall geometrically from the oldest to the newest; a round's new
+// tuples become the newest run and runs of similar size are merged (a binary
+// counter), so each tuple is rewritten O(log rounds) times.  N = unique(J) \ F
+// searches every run.  collapse() merges everything into one sorted array.
+struct FullRel {
+    dlg_ctx *ctx;
+    bool tiered;
+    std::vector<DBuf<u64>> runs;
+    std::vector<u64> sizes;
+    u64 total = 0;
+    FullRel(dlg_ctx *c, const Opts &o) : ctx(c), tiered(o.strategy != 1) {}
+
+    void merge_last_two() {
+        size_t k = runs.size();
+        DBuf<u64> m(ctx, sizes[k - 2] + sizes[k - 1]);
+        merge(ctx, runs[k - 2].p, sizes[k - 2], runs[k - 1].p, sizes[k - 1], m.p);
+        u64 s = sizes[k - 2] + sizes[k - 1];
+        runs.pop_back();
+        sizes.pop_back();
+        runs.back() = std::move(m);
+        sizes.back() = s;
+    }
+    // Takes ownership of buf[0..n) (sorted, disjoint from F).
+    void add(DBuf<u64> &buf, u64 n) {
+        if (n == 0) return;
+        total += n;
+        runs.push_back(std::move(buf));
+        sizes.push_back(n);
+        buf = DBuf<u64>(ctx);
+        if (!tiered) {
+            if (runs.size() == 2) merge_last_two();
+            return;
+        }
+        while (runs.size() >= 2 &&
+               (sizes[runs.size() - 2] <= 2 * sizes[runs.size() - 1] || runs.size() >= (size_t)MAX_RUNS))
+            merge_last_two();
+    }
+    // N = first of each run of equal tuples in sorted J, minus F.
+    u64 new_tuples(const u64 *J, u64 nj, u64 *N) {
+        if (runs.size() == 1) return select(ctx, SEL_UNIQUE_NOT_IN, J, nj, runs[0].p, sizes[0], N);
+        RunSet rs;
+        rs.k = (int)runs.size();
+        for (int j = 0; j < rs.k; j++) {
+            rs.p[j] = runs[j].p;
+            rs.n[j] = sizes[j];
+        }
+        return select_unique_not_in_runs(ctx, J, nj, rs, N);
+    }
+    void collapse(DBuf<u64> &out) {
+        while (runs.size() >= 2) merge_last_two();
+        if (runs.empty()) {
+            out.ensure(1);
+        } else {
+            out = std::move(runs[0]);
+            runs.clear();
+            sizes.clear();
+        }
+    }
+};
+
+// Sort J and keep its tuples that are new: N = unique(J) \ F.
+static u64 new_tuples(dlg_ctx *ctx, DBuf<u64> &J, u64 nj, DBuf<u64> &tmp, DBuf<u64> &N, FullRel &F,
+                      const Opts &o, PhaseClock &pc) {
     pc.mark(DLG_PH_DEDUP);
-    if (nj) {
-        radix_sort(ctx, J.p, nj, tmp, o.bits);
-        pc.mark(DLG_PH_MERGE);
-        N.ensure(nj);
-        nn = select(ctx, SEL_UNIQUE_NOT_IN, J.p, nj, full.p, nfull, N.p);
-    }
+    if (nj == 0) return 0;
+    radix_sort(ctx, J, nj, tmp, o.bits);
     pc.mark(DLG_PH_MERGE);
-    if (nn) {
-        full2.ensure(nfull + nn);
-        merge(ctx, full.p, nfull, N.p, nn, full2.p);
-        full.swap(full2);
-        nfull += nn;
-    }
-    return nn;
+    N.ensure(nj);
+    return F.new_tuples(J.p, nj, N.p);
 }
 
 // ------------------------------------------------------------------ TC
+// Round k: J = Δ ⋈ E, then Δ (the previous round's N) joins F, then
+// N = unique(J) \ F.  Adding Δ to F after its join is the reference's
+// "F = F ∪ N" of the previous round (tc.cu:268-286) done one step later,
+// so every set difference sees the same F as the reference's.
 void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
                  RoundHook *hook, FixpointOut &out) {
     pc.mark(DLG_PH_INIT);
-    DBuf<u64> edge(ctx, n), delta(ctx, n), J(ctx, n), full2(ctx), N(ctx), tmp(ctx);
-    DBuf<u64> &full = out.full;
+    DBuf<u64> edge(ctx, n), delta(ctx, n), J(ctx, n), N(ctx), tmp(ctx);
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, edge.p);   // e(src, dst) keyed by src
     pack_rows(ctx, d_rows, n, DLG_ORIENT_VK, J.p);      // Δ0 rows (dst, src)
     pc.mark(DLG_PH_HASHTABLE);
@@ -143,25 +199,29 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     edge.release();
     pc.mark(DLG_PH_DEDUP);
     u64 nd = sort_unique_into(ctx, J, n, tmp, delta, SEL_UNIQUE, nullptr, 0, o.bits);
-    pc.mark(DLG_PH_MERGE);
-    full.ensure(nd);
-    copy_u64(ctx, full.p, delta.p, nd);
-    u64 nfull = nd;
+    FullRel F(ctx, o);                                  // F = Δ0, added after the first join
     JoinScratch js(ctx);
     int it = 0;
     while (true) {
         pc.mark(DLG_PH_JOIN);
         u64 nj = join_into(ctx, ix, delta.p, nd, DLG_PROJ_HV_PV, js, J);   // (z, x)
         out.raw += (i64)nj;
-        u64 nn = absorb_round(ctx, J, nj, tmp, N, full, full2, nfull, o, pc);
+        pc.mark(DLG_PH_MERGE);
+        F.add(delta, nd);
+        u64 nn = new_tuples(ctx, J, nj, tmp, N, F, o, pc);
         it++;
         if (hook) hook->on_round(ctx, N.p, nn, it);
         if (nn == 0) break;                         // |F| unchanged (tc.cu:294-300)
         delta.swap(N);
         nd = nn;
-        if (o.max_it > 0 && it >= o.max_it) break;
+        if (o.max_it > 0 && it >= o.max_it) {
+            F.add(delta, nd);
+            break;
+        }
     }
-    out.nfull = nfull;
+    pc.mark(DLG_PH_MERGE);
+    out.nfull = F.total;
+    F.collapse(out.full);
     out.iterations = it;
 }
 
@@ -169,8 +229,7 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
 void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
                  RoundHook *hook, FixpointOut &out) {
     pc.mark(DLG_PH_INIT);
-    DBuf<u64> edge(ctx), J(ctx, n), T(ctx), delta(ctx), full2(ctx), N(ctx), tmp(ctx);
-    DBuf<u64> &full = out.full;
+    DBuf<u64> edge(ctx), J(ctx, n), T(ctx), delta(ctx), N(ctx), tmp(ctx);
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, J.p);
     pc.mark(DLG_PH_DEDUP);
     u64 ne = sort_unique_into(ctx, J, n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
@@ -183,30 +242,34 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     out.raw += (i64)nb;
     edge.release();
     pc.mark(DLG_PH_DEDUP);
-    u64 nfull = sort_unique_into(ctx, J, nb, tmp, full, SEL_UNIQUE_NODIAG, nullptr, 0, o.bits);
-    pc.mark(DLG_PH_MERGE);
-    delta.ensure(nfull);
-    copy_u64(ctx, delta.p, full.p, nfull);
-    u64 nd = nfull;
+    u64 nd = sort_unique_into(ctx, J, nb, tmp, delta, SEL_UNIQUE_NODIAG, nullptr, 0, o.bits);
+    FullRel F(ctx, o);                                  // F = base, added after the first join
     int it = 0;
     while (true) {
         pc.mark(DLG_PH_JOIN);
         u64 n1 = join_into(ctx, ix, delta.p, nd, DLG_PROJ_PV_HV, js, J);  // (b, x)
         out.raw += (i64)n1;
+        pc.mark(DLG_PH_MERGE);
+        F.add(delta, nd);
         pc.mark(DLG_PH_DEDUP);
         u64 nt = sort_unique_into(ctx, J, n1, tmp, T, SEL_UNIQUE, nullptr, 0, o.bits);
         pc.mark(DLG_PH_JOIN);
         u64 n2 = join_into(ctx, ix, T.p, nt, DLG_PROJ_PV_HV, js, J);      // (x, y)
         out.raw += (i64)n2;
-        u64 nn = absorb_round(ctx, J, n2, tmp, N, full, full2, nfull, o, pc);
+        u64 nn = new_tuples(ctx, J, n2, tmp, N, F, o, pc);
         it++;
         if (hook) hook->on_round(ctx, N.p, nn, it);
         if (nn == 0) break;                         // sg.cu:376-389
         delta.swap(N);
         nd = nn;
-        if (o.max_it > 0 && it >= o.max_it) break;
+        if (o.max_it > 0 && it >= o.max_it) {
+            F.add(delta, nd);
+            break;
+        }
     }
-    out.nfull = nfull;
+    pc.mark(DLG_PH_MERGE);
+    out.nfull = F.total;
+    F.collapse(out.full);
     out.iterations = it;
 }
 

@@ -71,7 +71,7 @@ void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double lf, bool
     if (rel != ix.rows.p) copy_u64(ctx, ix.rows.p, rel, n);
     if (!presorted) {
         DBuf<u64> tmp(ctx);
-        radix_sort(ctx, ix.rows.p, n, tmp);
+        radix_sort(ctx, ix.rows, n, tmp);
     }
     ix.gstart.ensure(n + 1);
     u64 G = select(ctx, SEL_KEY_HEAD, ix.rows.p, n, nullptr, 0, ix.gstart.p);
@@ -119,11 +119,22 @@ u64 join_count(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScrat
 
 constexpr int EX_T = 256, EX_I = 8, EX_TILE = EX_T * EX_I;
 
+// The probes whose output ranges meet this block's EX_TILE outputs are
+// [lo, hi); when at most EX_TILE + 1 of them (every probe with a non-empty
+// range in the tile adds at least one output), their offsets, row starts and
+// values are staged in LDS and each output finds its probe by a binary
+// search in LDS.  Tiles spanning more probes (runs of empty probes) search
+// global memory.
+constexpr int EX_LDS = EX_TILE + 1;
+
 template <int PROJ>
 __global__ __launch_bounds__(EX_T) void join_expand_k(const u64 *probe, u64 np, const u32 *pstart,
                                                       const u64 *offs, u64 total, const u64 *rows,
                                                       u64 *out) {
     __shared__ u64 s_lo, s_hi;
+    __shared__ u64 l_offs[EX_LDS];
+    __shared__ u32 l_start[EX_LDS];
+    __shared__ u32 l_pv[EX_LDS];
     const u64 base = (u64)blockIdx.x * EX_TILE;
     if (threadIdx.x == 0) {
         u64 last = (base + EX_TILE < total ? base + EX_TILE : total) - 1;
@@ -133,6 +144,30 @@ __global__ __launch_bounds__(EX_T) void join_expand_k(const u64 *probe, u64 np, 
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
+    const u64 span = hi - lo;
+    if (span <= (u64)EX_LDS) {
+        for (u64 j = threadIdx.x; j < span; j += EX_T) {
+            l_offs[j] = offs[lo + j];
+            l_start[j] = pstart[lo + j];
+            l_pv[j] = tval(probe[lo + j]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < EX_I; r++) {
+            u64 o = base + (u64)r * EX_T + threadIdx.x;
+            if (o < total) {
+                u32 a = 0, b = (u32)span;   // last j with l_offs[j] <= o
+                while (b - a > 1) {
+                    u32 m = (a + b) >> 1;
+                    if (l_offs[m] <= o) a = m; else b = m;
+                }
+                u32 hv = tval(rows[(u64)l_start[a] + (o - l_offs[a])]);
+                u32 pv = l_pv[a];
+                out[o] = PROJ == DLG_PROJ_HV_PV ? tmake(hv, pv) : tmake(pv, hv);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < EX_I; r++) {
         u64 o = base + (u64)r * EX_T + threadIdx.x;

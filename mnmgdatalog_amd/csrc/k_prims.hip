@@ -207,7 +207,12 @@ static void scan_u32_inplace(dlg_ctx *ctx, u32 *data, u64 n, DBuf<u64> &bsum) {
 }
 
 // =============================================================== radix sort
-constexpr int RS_T = 256, RS_I = 16, RS_TILE = RS_T * RS_I, RS_WAVES = RS_T / WAVE;
+// Tile shapes: 256 threads x 16 keys (4096) for small inputs, 512 threads x
+// 32 keys (16384, 128 KiB of LDS staging) for large ones, where longer runs
+// per digit make the scatter's writes closer to whole cache lines.
+constexpr int RS_SMALL_T = 256, RS_SMALL_I = 16;
+constexpr int RS_BIG_T = 512, RS_BIG_I = 32;
+constexpr u64 RS_BIG_MIN = 1ull << 21;
 
 struct RadixKey {
     u32 min_hi, min_lo;
@@ -244,62 +249,82 @@ __global__ __launch_bounds__(256) void minmax_k(const u64 *in, u64 n, u32 *out) 
         mnl = min(mnl, (u32)__shfl_xor((int)mnl, d, WAVE));
         mxl = max(mxl, (u32)__shfl_xor((int)mxl, d, WAVE));
     }
+    // one set of atomics per block: the per-wave form contended on 4 addresses
+    __shared__ u32 sm[4][256 / WAVE];
+    const int wave = threadIdx.x / WAVE;
     if ((threadIdx.x & (WAVE - 1)) == 0) {
+        sm[0][wave] = mnh; sm[1][wave] = mxh; sm[2][wave] = mnl; sm[3][wave] = mxl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / WAVE); w++) {
+            mnh = min(mnh, sm[0][w]); mxh = max(mxh, sm[1][w]);
+            mnl = min(mnl, sm[2][w]); mxl = max(mxl, sm[3][w]);
+        }
         atomicMin(&out[0], mnh); atomicMax(&out[1], mxh);
         atomicMin(&out[2], mnl); atomicMax(&out[3], mxl);
     }
 }
 
-template <int BITS>
-__global__ __launch_bounds__(RS_T) void rs_upsweep_k(const u64 *in, u64 n, RadixKey key,
-                                                     u32 *counts, u32 nb) {
-    constexpr int BINS = 1 << BITS;
-    __shared__ u32 h[RS_WAVES][BINS];
-    for (int i = threadIdx.x; i < RS_WAVES * BINS; i += RS_T) (&h[0][0])[i] = 0;
+template <int BITS, int T, int I>
+__global__ __launch_bounds__(T) void rs_upsweep_k(const u64 *in, u64 n, RadixKey key, u32 *counts,
+                                                  u32 nb) {
+    constexpr int BINS = 1 << BITS, W = T / WAVE, TILE = T * I;
+    __shared__ u32 h[W][BINS];
+    for (int i = threadIdx.x; i < W * BINS; i += T) (&h[0][0])[i] = 0;
     __syncthreads();
     const int wave = threadIdx.x / WAVE;
-    u64 base = (u64)blockIdx.x * RS_TILE;
+    u64 base = (u64)blockIdx.x * TILE;
 #pragma unroll 4
-    for (int r = 0; r < RS_I; r++) {
-        u64 idx = base + (u64)r * RS_T + threadIdx.x;
+    for (int r = 0; r < I; r++) {
+        u64 idx = base + (u64)r * T + threadIdx.x;
         if (idx < n) atomicAdd(&h[wave][radix_digit(in[idx], key, BINS - 1)], 1u);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < BINS; d += RS_T) {
+    for (int d = threadIdx.x; d < BINS; d += T) {
         u32 s = 0;
 #pragma unroll
-        for (int w = 0; w < RS_WAVES; w++) s += h[w][d];
+        for (int w = 0; w < W; w++) s += h[w][d];
         counts[(u64)d * nb + blockIdx.x] = s;
     }
 }
 
-template <int BITS>
-__global__ __launch_bounds__(RS_T) void rs_downsweep_k(const u64 *in, u64 *out, u64 n,
-                                                       RadixKey key, const u32 *offs, u32 nb) {
-    constexpr int BINS = 1 << BITS;
-    __shared__ u64 stage[RS_TILE];
-    __shared__ u32 wcnt[RS_WAVES][BINS];
+// Stable scatter of one tile: wave-match ranking (one ballot per digit bit),
+// per-wave running counts in LDS, keys staged in LDS in digit order, then
+// written to offs[digit][tile] + rank.  Wave w owns keys
+// [tile0 + w*64*I, tile0 + (w+1)*64*I), consumed 64 at a time in order, so
+// the (wave, round, lane) order is the input order and the sort is stable.
+template <int BITS, int T, int I>
+__global__ __launch_bounds__(T) void rs_downsweep_k(const u64 *in, u64 *out, u64 n, RadixKey key,
+                                                    const u32 *offs, u32 nb) {
+    constexpr int BINS = 1 << BITS, W = T / WAVE, TILE = T * I;
+    __shared__ u64 stage[TILE];
+    __shared__ u32 wcnt[W][BINS];
     __shared__ u32 bstart[BINS];
     __shared__ u32 gofs[BINS];
-    __shared__ u32 sc[RS_T / WAVE + 1];
+    __shared__ u32 sc[T / WAVE + 1];
 
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-    for (int i = threadIdx.x; i < RS_WAVES * BINS; i += RS_T) (&wcnt[0][0])[i] = 0;
-    for (int d = threadIdx.x; d < BINS; d += RS_T) gofs[d] = offs[(u64)d * nb + blockIdx.x];
+    for (int i = threadIdx.x; i < W * BINS; i += T) (&wcnt[0][0])[i] = 0;
+    for (int d = threadIdx.x; d < BINS; d += T) gofs[d] = offs[(u64)d * nb + blockIdx.x];
     __syncthreads();
 
-    const u64 tile0 = (u64)blockIdx.x * RS_TILE;
-    const u64 wbase = tile0 + (u64)wave * (WAVE * RS_I);
+    const u64 tile0 = (u64)blockIdx.x * TILE;
+    const u64 wbase = tile0 + (u64)wave * (WAVE * I);
     const u64 lt = lanemask_lt();
-    u64 keys[RS_I];
-    u32 pk[RS_I];  // digit | (rank within wave's digit run << 8)
+    u64 keys[I];
+    u32 pk[I];  // digit | (rank within the wave's digit run << 8)
 
 #pragma unroll
-    for (int r = 0; r < RS_I; r++) {
+    for (int r = 0; r < I; r++) {
+        u64 idx = wbase + (u64)r * WAVE + lane;
+        keys[r] = idx < n ? in[idx] : 0ull;
+    }
+#pragma unroll
+    for (int r = 0; r < I; r++) {
         u64 idx = wbase + (u64)r * WAVE + lane;
         bool valid = idx < n;
-        u64 x = valid ? in[idx] : 0ull;
-        u32 d = radix_digit(x, key, BINS - 1);
+        u32 d = radix_digit(keys[r], key, BINS - 1);
         u64 m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < BITS; b++) {
@@ -312,7 +337,6 @@ __global__ __launch_bounds__(RS_T) void rs_downsweep_k(const u64 *in, u64 *out, 
         // The lowest lane of each digit group publishes the new running count;
         // every lane of the wave has read the old value in the same LDS op.
         if (valid && before == 0) wcnt[wave][d] = cnt + (u32)__popcll(m);
-        keys[r] = x;
         pk[r] = d | ((cnt + before) << 8);
         __builtin_amdgcn_wave_barrier();
     }
@@ -323,7 +347,7 @@ __global__ __launch_bounds__(RS_T) void rs_downsweep_k(const u64 *in, u64 *out, 
     if (threadIdx.x < BINS) {
         u32 s = 0;
 #pragma unroll
-        for (int w = 0; w < RS_WAVES; w++) {
+        for (int w = 0; w < W; w++) {
             u32 c = wcnt[w][threadIdx.x];
             wcnt[w][threadIdx.x] = s;
             s += c;
@@ -335,7 +359,7 @@ __global__ __launch_bounds__(RS_T) void rs_downsweep_k(const u64 *in, u64 *out, 
     __syncthreads();
 
 #pragma unroll
-    for (int r = 0; r < RS_I; r++) {
+    for (int r = 0; r < I; r++) {
         u64 idx = wbase + (u64)r * WAVE + lane;
         if (idx < n) {
             u32 d = pk[r] & 0xffu;
@@ -345,8 +369,8 @@ __global__ __launch_bounds__(RS_T) void rs_downsweep_k(const u64 *in, u64 *out, 
     __syncthreads();
 
     u64 rem = n - tile0;
-    u32 tn = rem < (u64)RS_TILE ? (u32)rem : (u32)RS_TILE;
-    for (u32 i = threadIdx.x; i < tn; i += RS_T) {
+    u32 tn = rem < (u64)TILE ? (u32)rem : (u32)TILE;
+    for (u32 i = threadIdx.x; i < tn; i += T) {
         u64 x = stage[i];
         u32 d = radix_digit(x, key, BINS - 1);
         out[(u64)gofs[d] + (i - bstart[d])] = x;
@@ -355,24 +379,32 @@ __global__ __launch_bounds__(RS_T) void rs_downsweep_k(const u64 *in, u64 *out, 
 
 static int bit_width(u32 v) { return v ? 32 - __builtin_clz(v) : 0; }
 
-template <int BITS>
+template <int BITS, int T, int I>
 static void radix_passes(dlg_ctx *ctx, u64 *&src, u64 *&dst, u64 n, RadixKey key, int total_bits) {
-    constexpr int BINS = 1 << BITS;
-    u32 nb = (u32)((n + RS_TILE - 1) / RS_TILE);
+    constexpr int BINS = 1 << BITS, TILE = T * I;
+    u32 nb = (u32)((n + TILE - 1) / TILE);
     DBuf<u32> counts(ctx, (size_t)BINS * nb);
     DBuf<u64> bsum(ctx);
     for (int shift = 0; shift < total_bits; shift += BITS) {
         key.shift = (u32)shift;
-        DLG_LAUNCH(ctx, DLG_K_SORT_UPSWEEP, 8.0 * n + 4.0 * BINS * nb, rs_upsweep_k<BITS>, nb,
-                   RS_T, 0, (const u64 *)src, n, key, counts.p, nb);
+        DLG_LAUNCH(ctx, DLG_K_SORT_UPSWEEP, 8.0 * n + 4.0 * BINS * nb, (rs_upsweep_k<BITS, T, I>), nb,
+                   T, 0, (const u64 *)src, n, key, counts.p, nb);
         scan_u32_inplace(ctx, counts.p, (u64)BINS * nb, bsum);
-        DLG_LAUNCH(ctx, DLG_K_SORT_DOWNSWEEP, 16.0 * n + 4.0 * BINS * nb, rs_downsweep_k<BITS>,
-                   nb, RS_T, 0, (const u64 *)src, dst, n, key, (const u32 *)counts.p, nb);
+        DLG_LAUNCH(ctx, DLG_K_SORT_DOWNSWEEP, 16.0 * n + 4.0 * BINS * nb, (rs_downsweep_k<BITS, T, I>),
+                   nb, T, 0, (const u64 *)src, dst, n, key, (const u32 *)counts.p, nb);
         std::swap(src, dst);
     }
 }
 
-void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits) {
+template <int BITS>
+static void radix_passes_sized(dlg_ctx *ctx, u64 *&src, u64 *&dst, u64 n, RadixKey key, int total_bits) {
+    if (n >= RS_BIG_MIN)
+        radix_passes<BITS, RS_BIG_T, RS_BIG_I>(ctx, src, dst, n, key, total_bits);
+    else
+        radix_passes<BITS, RS_SMALL_T, RS_SMALL_I>(ctx, src, dst, n, key, total_bits);
+}
+
+void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits) {
     if (n < 2) return;
     if (n > 0xffffffffull) fail(DLG_OVERFLOW, "radix_sort: more than 2^32-1 tuples in one buffer");
     if (bits <= 0) bits = ctx->radix_bits;
@@ -381,7 +413,7 @@ void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits) {
     DBuf<u32> mm(ctx, 4);
     DLG_LAUNCH(ctx, DLG_K_MISC, 0.0, minmax_init_k, 1, 64, 0, mm.p);
     unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 8);
-    DLG_LAUNCH(ctx, DLG_K_SORT_HIST, 8.0 * n, minmax_k, g, 256, 0, (const u64 *)data, n, mm.p);
+    DLG_LAUNCH(ctx, DLG_K_SORT_HIST, 8.0 * n, minmax_k, g, 256, 0, (const u64 *)data.p, n, mm.p);
     u32 h[4];
     HIPCHK(hipMemcpyAsync(ctx->h_scalars, mm.p, 16, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -394,15 +426,28 @@ void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits) {
     int total_bits = (int)key.vb + bit_width(h[1] - h[0]);
     if (total_bits == 0) return;  // all equal
     tmp.ensure(n);
-    u64 *src = data, *dst = tmp.p;
+    u64 *src = data.p, *dst = tmp.p;
     switch (bits) {
-        case 4: radix_passes<4>(ctx, src, dst, n, key, total_bits); break;
-        case 5: radix_passes<5>(ctx, src, dst, n, key, total_bits); break;
-        case 6: radix_passes<6>(ctx, src, dst, n, key, total_bits); break;
-        case 7: radix_passes<7>(ctx, src, dst, n, key, total_bits); break;
-        default: radix_passes<8>(ctx, src, dst, n, key, total_bits); break;
+        case 4: radix_passes_sized<4>(ctx, src, dst, n, key, total_bits); break;
+        case 5: radix_passes_sized<5>(ctx, src, dst, n, key, total_bits); break;
+        case 6: radix_passes_sized<6>(ctx, src, dst, n, key, total_bits); break;
+        case 7: radix_passes_sized<7>(ctx, src, dst, n, key, total_bits); break;
+        default: radix_passes_sized<8>(ctx, src, dst, n, key, total_bits); break;
     }
-    if (src != data) copy_u64(ctx, data, src, n);
+    if (src != data.p) data.swap(tmp);   // result is in the other buffer: swap ownership
+}
+
+void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits) {
+    DBuf<u64> view(ctx);
+    view.p = data;          // borrowed: released without returning it to the pool
+    view.cap = n;
+    radix_sort(ctx, view, n, tmp, bits);
+    if (view.p != data) {   // sorted copy ended in tmp's old block; bring it back
+        copy_u64(ctx, data, view.p, n);
+        tmp.swap(view);     // tmp owns that block again
+    }
+    view.p = nullptr;
+    view.cap = 0;
 }
 
 // =============================================================== select
@@ -534,6 +579,61 @@ u64 select(dlg_ctx *ctx, int mode, const u64 *in, u64 n, const u64 *other, u64 n
     else
         DLG_LAUNCH(ctx, DLG_K_SELECT, 8.0 * ((n + 63) / 64) + 16.0 * total,
                    select_scatter_k<false>, (unsigned)nb, SEL_T, 0, in, n, flags.p, bsum.p, out);
+    return total;
+}
+
+// First of each equal run of sorted `in` and absent from every run of `rs`.
+// Each block narrows every run to the range spanned by its own tile (one
+// thread per run), so a candidate's searches stay inside small ranges.
+__global__ __launch_bounds__(SEL_T) void select_count_runs_k(const u64 *in, u64 n, RunSet rs,
+                                                             u64 *flags, u64 *bsum) {
+    __shared__ u64 s_lo[MAX_RUNS], s_hi[MAX_RUNS];
+    __shared__ u64 sc[SEL_T / WAVE + 1];
+    const u64 base = (u64)blockIdx.x * SEL_TILE;
+    if ((int)threadIdx.x < rs.k) {
+        const int j = threadIdx.x;
+        u64 last = base + SEL_TILE - 1 < n ? base + SEL_TILE - 1 : n - 1;
+        u64 lo = lower_bound_u64(rs.p[j], 0, rs.n[j], in[base]);
+        s_lo[j] = lo;
+        s_hi[j] = upper_bound_u64(rs.p[j], lo, rs.n[j], in[last]);
+    }
+    __syncthreads();
+    u64 c = 0;
+#pragma unroll
+    for (int r = 0; r < SEL_I; r++) {
+        u64 idx = base + (u64)r * SEL_T + threadIdx.x;
+        bool keep = false;
+        if (idx < n) {
+            u64 x = in[idx];
+            keep = idx == 0 || in[idx - 1] != x;
+            for (int j = 0; keep && j < rs.k; j++)
+                if (s_lo[j] < s_hi[j]) keep = !found_sorted(rs.p[j], s_lo[j], s_hi[j], x);
+        }
+        u64 m = __ballot(keep);
+        u64 w0 = base + (u64)r * SEL_T + (threadIdx.x & ~(WAVE - 1));
+        if ((threadIdx.x & (WAVE - 1)) == 0 && w0 < n) flags[w0 / WAVE] = m;
+        c += keep ? 1 : 0;
+    }
+    u64 tot;
+    block_exclusive_scan<u64>(c, sc, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+u64 select_unique_not_in_runs(dlg_ctx *ctx, const u64 *in, u64 n, const RunSet &rs, u64 *out) {
+    if (n == 0) return 0;
+    if (rs.k == 0) return select(ctx, SEL_UNIQUE, in, n, nullptr, 0, out);
+    u64 nb = (n + SEL_TILE - 1) / SEL_TILE;
+    DBuf<u64> flags(ctx, (n + WAVE - 1) / WAVE + 1);
+    DBuf<u64> bsum(ctx, nb);
+    DBuf<u64> tot(ctx, 1);
+    DLG_LAUNCH(ctx, DLG_K_SELECT, 8.0 * n + 8.0 * ((n + 63) / 64), select_count_runs_k,
+               (unsigned)nb, SEL_T, 0, in, n, rs, flags.p, bsum.p);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb, tot.p);
+    u64 total = 0;
+    ctx->read_scalars(tot.p, 1, &total);
+    if (total == 0) return 0;
+    DLG_LAUNCH(ctx, DLG_K_SELECT, 8.0 * ((n + 63) / 64) + 16.0 * total, select_scatter_k<false>,
+               (unsigned)nb, SEL_T, 0, in, n, flags.p, bsum.p, (void *)out);
     return total;
 }
 
