@@ -38,7 +38,8 @@ typedef enum dlg_status {
     DLG_OVERFLOW = 3,        /* a count exceeds what the engine represents (2^32-1 rows per buffer) */
     DLG_DEVICE_ERROR = 4,    /* HIP runtime error                                    */
     DLG_IO_ERROR = 5,        /* file open/read/write failure                         */
-    DLG_REFUSED = 6          /* serving: admission refused (predicted need > budget) */
+    DLG_REFUSED = 6,         /* serving: admission refused (predicted need > budget) */
+    DLG_PENDING = 7          /* serving: the request has not finished yet            */
 } dlg_status;
 
 typedef struct dlg_ctx dlg_ctx;
@@ -149,8 +150,10 @@ typedef struct dlg_opts {
     double load_factor;      /* hash table load factor, (0, 0.95]                      */
     int32_t block_size;      /* threads per block for the streaming kernels (64..1024) */
     int32_t radix_bits;      /* radix digit width 4..8                                 */
-    int32_t strategy;        /* 0 auto, 1 sorted full relation (sort/diff/merge),
-                                2 hash-set full relation (insert-dedup)                */
+    int32_t strategy;        /* 0 auto; 1 the reference's scheme (one sorted full
+                                relation merged every round; WCC rounds launched one
+                                by one); 2 tiered sorted runs for the full relation and
+                                hipGraph-captured WCC rounds                          */
     int32_t max_iterations;  /* 0 = run to the fixpoint; >0 stops early (sampling)     */
     int32_t keep_output;     /* 1: copy the output rows to host (dlg_result.rows)      */
     int32_t reserved[6];
@@ -227,6 +230,57 @@ dlg_status dlg_autotune(dlg_ctx *ctx, int prog, const int32_t *h_rows, int64_t n
                         const double *load_factors, const int32_t *radix_bits,
                         const int32_t *block_sizes, int32_t repeats, dlg_opts *best,
                         double *best_seconds);
+
+/* ------------------------------------------------------- served form */
+/* No counterpart in the reference.  A request loop on one device:
+ *  - `workers` slots, each with its own HIP stream and memory pool; kernels
+ *    of requests in different slots overlap on the device;
+ *  - admission in arrival order against an HBM budget: a request reserves
+ *    its predicted peak footprint (from its row count and id range) and is
+ *    admitted when a slot is free and the reservation fits; a request that
+ *    can never fit is refused at submission (DLG_REFUSED);
+ *  - continuous batching: a slot that frees up takes the oldest admissible
+ *    request and, if it is small (<= small_rows), every other waiting small
+ *    request of the same program that fits, and runs them as one batch
+ *    (dlg_batch_run semantics, exact per request);
+ *  - a request that outgrows its reservation is re-run alone with the whole
+ *    budget; resident graphs stay on the device across requests.
+ * Every result equals the request's plain run (same rows and counts). */
+typedef struct dlg_server dlg_server;
+typedef struct dlg_server_opts {
+    int32_t device;
+    int32_t workers;         /* concurrent slots (streams), default 4                   */
+    int64_t hbm_budget;      /* bytes for requests; 0 = 85 % of free device memory       */
+    int32_t max_batch;       /* most requests in one batch, default 64                   */
+    int64_t small_rows;      /* requests up to this many rows are batched, default 20000 */
+    int64_t retain_bytes;    /* pool cache kept per slot between requests, default 256 MiB */
+    int64_t max_request_bytes; /* refuse requests predicted above this; 0 = the budget   */
+    dlg_opts tune;           /* tuning applied to every request                         */
+} dlg_server_opts;
+typedef struct dlg_request_info {
+    int32_t status;          /* dlg_status of the request                               */
+    int32_t batch_size;      /* requests in the batch it ran in                          */
+    int32_t attempts;        /* 2 when it was re-run alone                               */
+    int32_t slot;
+    double submit_s, start_s, end_s;   /* seconds since the server was created           */
+    int64_t reserved_bytes;
+} dlg_request_info;
+dlg_status dlg_server_create(const dlg_server_opts *o, dlg_server **out);
+/* Keep a graph resident for WCC queries (graph ids from 0). */
+dlg_status dlg_server_add_graph(dlg_server *s, const int32_t *h_rows, int64_t n, int32_t *graph_id);
+/* Queue a request: prog on the given rows (copied), or, with graph_id >= 0,
+ * WCC on a resident graph (rows ignored). */
+dlg_status dlg_server_submit(dlg_server *s, int prog, const int32_t *h_rows, int64_t n,
+                             int32_t graph_id, uint64_t *ticket);
+/* Wait up to timeout_s (< 0: forever) for a request.  DLG_PENDING on timeout;
+ * otherwise DLG_OK and the request's own status in info->status.  The result
+ * is handed over once (free with dlg_result_free). */
+dlg_status dlg_server_wait(dlg_server *s, uint64_t ticket, double timeout_s, dlg_result *res,
+                           dlg_request_info *info);
+/* Finishes queued requests, joins the slots and frees device memory. */
+void dlg_server_destroy(dlg_server *s);
+/* Last server-level error text. */
+const char *dlg_server_error(const dlg_server *s);
 
 /* ------------------------------------------------------- input formats */
 /* Replaces parallel_read (common/parallel_io.cu:5-41) on one rank: raw

@@ -57,6 +57,19 @@ class BatchStats(ctypes.Structure):
 
 
 PROGS = {"tc": 0, "sg": 1, "wcc": 2, "single_join": 3}
+DLG_PENDING = 7
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device", c_int32), ("workers", c_int32), ("hbm_budget", c_int64), ("max_batch", c_int32),
+                ("small_rows", c_int64), ("retain_bytes", c_int64), ("max_request_bytes", c_int64),
+                ("tune", Opts)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", c_int32), ("batch_size", c_int32), ("attempts", c_int32), ("slot", c_int32),
+                ("submit_s", c_double), ("start_s", c_double), ("end_s", c_double),
+                ("reserved_bytes", c_int64)]
 
 _LIB = None
 
@@ -111,6 +124,12 @@ def lib() -> ctypes.CDLL:
                                      POINTER(Result), POINTER(BatchStats)])
     sig["dlg_autotune"] = (c_int, [vp, c_int, vp, i64, POINTER(c_double), POINTER(c_int32), POINTER(c_int32),
                                     c_int32, POINTER(Opts), POINTER(c_double)])
+    sig["dlg_server_create"] = (c_int, [POINTER(ServerOpts), POINTER(c_void_p)])
+    sig["dlg_server_add_graph"] = (c_int, [vp, vp, i64, POINTER(c_int32)])
+    sig["dlg_server_submit"] = (c_int, [vp, c_int, vp, i64, c_int32, POINTER(c_uint64)])
+    sig["dlg_server_wait"] = (c_int, [vp, c_uint64, c_double, POINTER(Result), POINTER(RequestInfo)])
+    sig["dlg_server_destroy"] = (None, [vp])
+    sig["dlg_server_error"] = (c_char_p, [vp])
     for prog in ("tc", "sg", "wcc", "single_join"):
         sig[f"dlg_run_{prog}"] = (c_int, [vp, vp, i64, POINTER(Opts), POINTER(Result)])
         sig[f"dlg_run_{prog}_device"] = (c_int, [vp, vp, i64, POINTER(Opts), POINTER(Result)])
@@ -418,3 +437,69 @@ class Ctx:
             return out
         finally:
             self.L.dlg_result_free(byref(r))
+
+
+class Server:
+    """Served form: request loop with HBM-budget admission and continuous batching."""
+
+    def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 0, max_batch: int = 64,
+                 small_rows: int = 20000, retain_bytes: int = 0, max_request_bytes: int = 0, **tune):
+        self.L = lib()
+        o = ServerOpts()
+        o.device, o.workers, o.hbm_budget, o.max_batch = device, workers, hbm_budget, max_batch
+        o.small_rows, o.retain_bytes, o.max_request_bytes = small_rows, retain_bytes, max_request_bytes
+        for k, v in tune.items():
+            setattr(o.tune, k, v)
+        self.h = c_void_p()
+        s = self.L.dlg_server_create(byref(o), byref(self.h))
+        if s != DLG_OK:
+            raise DlgError(s, "dlg_server_create")
+
+    def _err(self, s, what):
+        if s != DLG_OK:
+            raise DlgError(s, f"{what}: {self.L.dlg_server_error(self.h).decode(errors='replace')}")
+
+    def add_graph(self, rows: np.ndarray) -> int:
+        rows = np.ascontiguousarray(rows, dtype=np.int32).reshape(-1, 2)
+        gid = c_int32()
+        self._err(self.L.dlg_server_add_graph(self.h, rows.ctypes.data, rows.shape[0], byref(gid)), "add_graph")
+        return gid.value
+
+    def submit(self, prog: str, rows=None, graph: int = -1) -> int:
+        """Queue a request; returns its ticket.  Raises DlgError(DLG_REFUSED) when it can never fit."""
+        t = c_uint64()
+        if rows is None:
+            s = self.L.dlg_server_submit(self.h, PROGS[prog], None, 0, graph, byref(t))
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.int32).reshape(-1, 2)
+            s = self.L.dlg_server_submit(self.h, PROGS[prog], rows.ctypes.data, rows.shape[0], graph, byref(t))
+        self._err(s, "submit")
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0):
+        r, info = Result(), RequestInfo()
+        s = self.L.dlg_server_wait(self.h, ticket, timeout, byref(r), byref(info))
+        if s == DLG_PENDING:
+            return None
+        self._err(s, "wait")
+        try:
+            out = {"status": info.status, "iterations": r.iterations, "output_size": r.output_size,
+                   "secondary": r.secondary, "nodes": r.nodes,
+                   "rows": _rows_from(r.rows, r.n_rows) if r.rows else None,
+                   "submit_s": info.submit_s, "start_s": info.start_s, "end_s": info.end_s,
+                   "latency_s": info.end_s - info.submit_s, "batch_size": info.batch_size,
+                   "attempts": info.attempts, "slot": info.slot, "reserved_bytes": info.reserved_bytes}
+        finally:
+            self.L.dlg_result_free(byref(r))
+        return out
+
+    def close(self):
+        if self.h:
+            self.L.dlg_server_destroy(self.h)
+            self.h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

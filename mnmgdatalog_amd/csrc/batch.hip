@@ -212,7 +212,7 @@ static void split_rows(const std::vector<int32_t> &all, const std::vector<u64> &
     }
 }
 
-static void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const int64_t *n,
+void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const int64_t *n,
                       const Opts &o, dlg_result *res, dlg_batch_stats *st) {
     Layout L = make_layout(B, rows, n);
     if (L.total_rows >= 0x7fffffffLL) fail(DLG_OVERFLOW, "batch has more than 2^31-2 rows");

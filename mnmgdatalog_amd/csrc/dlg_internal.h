@@ -45,6 +45,7 @@ public:
     void *get(size_t bytes);
     void put(void *p);
     void trim();                      // release cached blocks to the runtime
+    void trim_to(size_t keep);        // release cached blocks until at most `keep` bytes remain
     size_t live_bytes() const { return live_bytes_; }
     size_t cached_bytes() const { return cached_bytes_; }
     size_t peak_bytes() const { return peak_bytes_; }
@@ -308,6 +309,10 @@ void wcc_setup(dlg_ctx *ctx, WccState &s, u64 V);
 u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf, u32 round);
 // #components, largest component size and the (label, 0) tuples in `out`.
 u64 wcc_finalize(dlg_ctx *ctx, WccState &s, const Index &ix, DBuf<u64> &out, i64 *largest);
+// All rounds through replays of a captured graph of `kr` rounds; returns the
+// iteration count (labels in s.L).  `ne` = edges of the index.
+int wcc_rounds_graph(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 ne, int max_it,
+                     int kr);
 
 }  // namespace dlg
 

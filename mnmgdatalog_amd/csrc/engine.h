@@ -79,6 +79,15 @@ struct WccGraph {
 void wcc_graph_build(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
                      WccGraph &g);
 
+void wcc_solve(dlg_ctx *ctx, const WccGraph &g, const Opts &o, PhaseClock &pc, dlg_result *res);
+
+// Plain run of one program on device rows (engine.hip).
+void run_program(dlg_ctx *ctx, int prog, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res);
+
+// Batched form (batch.hip): B instances of one program as one fixpoint.
+void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const int64_t *n,
+               const Opts &o, dlg_result *res, dlg_batch_stats *st);
+
 void download_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, int32_t *h_rows);
 void check_rows_arg(int64_t n, const void *rows);
 

@@ -362,37 +362,50 @@ static void run_sj(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dl
     pc.finish(res->phase_seconds);
 }
 
-static void run_wcc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
-    PhaseClock pc(ctx);
-    auto t0 = Clock::now();
-    WccGraph g(ctx);
-    wcc_graph_build(ctx, d_rows, n, o, pc, g);
+// Min-label propagation over a built graph; fills iterations, sizes and
+// (with o.keep) the (label, 0) rows.  The graph is only read, so one
+// resident graph can serve queries from several contexts at once.
+void wcc_solve(dlg_ctx *ctx, const WccGraph &g, const Opts &o, PhaseClock &pc, dlg_result *res) {
     pc.mark(DLG_PH_INIT);
     WccState s(ctx);
     wcc_setup(ctx, s, g.V);
-    u64 nf = g.V;
-    u64 prev = g.V;                                        // wcc.cu:236-242
     int it = 0;
-    while (true) {
-        pc.mark(DLG_PH_JOIN);
-        u64 cur = nf ? wcc_round(ctx, s, g.ix, g.nbr.p, nf, (u32)(it + 1)) : 0;
-        it++;
-        nf = cur;
-        if (cur == prev) break;                            // wcc.cu:366-383
-        prev = cur;
-        if (o.max_it > 0 && it >= o.max_it) break;
+    const bool graph = o.strategy == 2 || (o.strategy == 0 && g.ne <= (1ull << 25));
+    pc.mark(DLG_PH_JOIN);
+    if (graph) {
+        it = wcc_rounds_graph(ctx, s, g.ix, g.nbr.p, g.ne, o.max_it, 16);
+    } else {
+        u64 nf = g.V;
+        u64 prev = g.V;                                    // wcc.cu:236-242
+        while (true) {
+            pc.mark(DLG_PH_JOIN);
+            u64 cur = nf ? wcc_round(ctx, s, g.ix, g.nbr.p, nf, (u32)(it + 1)) : 0;
+            it++;
+            nf = cur;
+            if (cur == prev) break;                        // wcc.cu:366-383
+            prev = cur;
+            if (o.max_it > 0 && it >= o.max_it) break;
+        }
     }
     pc.mark(DLG_PH_FINAL);
     DBuf<u64> out(ctx);
     i64 largest = -1;
     u64 ncomp = wcc_finalize(ctx, s, g.ix, out, &largest);
     ctx->sync();
-    res->device_seconds = secs(t0, Clock::now());
     res->iterations = it;
     res->output_size = (int64_t)ncomp;
     res->secondary = largest;
     res->nodes = (int64_t)g.V;
     if (o.keep) result_rows(ctx, out.p, ncomp, DLG_ORIENT_KV, res);   // (label, 0)
+}
+
+static void run_wcc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+    PhaseClock pc(ctx);
+    auto t0 = Clock::now();
+    WccGraph g(ctx);
+    wcc_graph_build(ctx, d_rows, n, o, pc, g);
+    wcc_solve(ctx, g, o, pc, res);
+    res->device_seconds = secs(t0, Clock::now());
     pc.finish(res->phase_seconds);
 }
 
@@ -609,3 +622,16 @@ dlg_status dlg_exclusive_scan(dlg_ctx *c, const uint32_t *in, int64_t n, uint64_
 }
 
 }  // extern "C"
+
+namespace dlg {
+// Plain run of one program on device rows (used by the served form).
+void run_program(dlg_ctx *ctx, int prog, const int32_t *d_rows, u64 n, const Opts &o, dlg_result *res) {
+    switch (prog) {
+        case DLG_PROG_TC: run_tc(ctx, d_rows, n, o, res); break;
+        case DLG_PROG_SG: run_sg(ctx, d_rows, n, o, res); break;
+        case DLG_PROG_WCC: run_wcc(ctx, d_rows, n, o, res); break;
+        case DLG_PROG_SJ: run_sj(ctx, d_rows, n, o, res); break;
+        default: fail(DLG_INVALID_INPUT, "unknown program");
+    }
+}
+}  // namespace dlg

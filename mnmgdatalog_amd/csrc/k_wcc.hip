@@ -188,3 +188,233 @@ u64 wcc_finalize(dlg_ctx *ctx, WccState &s, const Index &ix, DBuf<u64> &out, i64
 }
 
 }  // namespace dlg
+
+// =====================================================================
+// Graph-captured rounds.  For long runs with small frontiers the eager
+// round costs more in launches and host synchronisation than in work.
+// Here every size lives in device memory (WccDev), every kernel has a
+// fixed grid sized for the whole graph and exits early past the live size,
+// the stop rule runs on the device, and KR rounds (frontier buffers in
+// ping-pong) are captured once into a hipGraph that the host replays,
+// reading back one flag per replay.  No memset nodes: counters are cleared
+// by kernels.
+namespace dlg {
+
+struct WccDev {
+    u64 nf;      // live frontier size
+    u64 total;   // edges leaving the frontier
+    u32 count;   // next frontier size (append counter)
+    u32 done;    // the stop rule fired
+    u32 it;      // rounds run
+    u32 max_it;  // 0 = none
+    u64 prev;    // |Δ| of the previous round (wcc.cu:236-242, 366-383)
+};
+
+constexpr int GW_T = 256, GW_I = 8, GW_TILE = GW_T * GW_I;
+
+__global__ void gw_begin_k(WccDev *st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !st->done) st->count = 0;
+}
+
+__global__ __launch_bounds__(GW_T) void gw_degree_scan_k(const u32 *fnode, const u32 *gstart, u32 *cnt,
+                                                         u32 *pstart, u64 *bsum, const WccDev *st) {
+    __shared__ u64 sc[GW_T / WAVE + 1];
+    const u64 nf = st->done ? 0 : st->nf;
+    const u64 base = (u64)blockIdx.x * GW_TILE + (u64)threadIdx.x * GW_I;
+    u64 s = 0;
+#pragma unroll
+    for (int i = 0; i < GW_I; i++) {
+        u64 j = base + i;
+        if (j < nf) {
+            u32 v = fnode[j];
+            u32 a = gstart[v];
+            u32 c = gstart[v + 1] - a;
+            cnt[j] = c;
+            pstart[j] = a;
+            s += c;
+        }
+    }
+    u64 tot;
+    block_exclusive_scan<u64>(s, sc, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void gw_scan_blocks_k(u64 *bsum, u64 nb, WccDev *st) {
+    __shared__ u64 sc[1024 / WAVE + 1];
+    u64 carry = 0;
+    for (u64 base = 0; base < nb; base += 1024) {
+        u64 i = base + threadIdx.x;
+        u64 v = i < nb ? bsum[i] : 0;
+        u64 tot;
+        u64 ex = block_exclusive_scan<u64>(v, sc, &tot);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) st->total = st->done ? 0 : carry;
+}
+
+__global__ __launch_bounds__(GW_T) void gw_scan_apply_k(const u32 *cnt, const u64 *bpre, u64 *offs,
+                                                        const WccDev *st) {
+    __shared__ u64 sc[GW_T / WAVE + 1];
+    const u64 nf = st->done ? 0 : st->nf;
+    const u64 base = (u64)blockIdx.x * GW_TILE + (u64)threadIdx.x * GW_I;
+    u32 v[GW_I];
+    u64 s = 0;
+#pragma unroll
+    for (int i = 0; i < GW_I; i++) {
+        v[i] = base + i < nf ? cnt[base + i] : 0;
+        s += v[i];
+    }
+    u64 ex = block_exclusive_scan<u64>(s, sc, nullptr) + bpre[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < GW_I; i++) {
+        if (base + i < nf) offs[base + i] = ex;
+        ex += v[i];
+    }
+}
+
+__global__ __launch_bounds__(WR_T) void gw_relax_k(const u32 *flabel, const u32 *pstart, const u64 *offs,
+                                                   const u32 *nbr, u32 *L, u32 *mark, u32 *next_node,
+                                                   WccDev *st) {
+    __shared__ u64 s_lo, s_hi;
+    const u64 total = st->total, nf = st->nf;
+    const u64 base = (u64)blockIdx.x * WR_TILE;
+    if (st->done || base >= total) return;
+    const u32 round = st->it + 1;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (threadIdx.x == 0) {
+        u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
+        u64 lo = upper_bound_u64(offs, 0, nf, base) - 1;
+        s_lo = lo;
+        s_hi = upper_bound_u64(offs, lo, nf, last);
+    }
+    __syncthreads();
+    const u64 lo = s_lo, hi = s_hi;
+    const u64 lt = lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < WR_I; r++) {
+        u64 o = base + (u64)r * WR_T + threadIdx.x;
+        bool app = false;
+        u32 m = 0;
+        if (o < total) {
+            u64 i = upper_bound_u64(offs, lo, hi, o) - 1;
+            m = nbr[(u64)pstart[i] + (o - offs[i])];
+            u32 l = flabel[i];
+            if (L[m] > l) {
+                u32 old = atomicMin(&L[m], l);
+                if (old > l) app = atomicExch(&mark[m], round) != round;
+            }
+        }
+        u64 bm = __ballot(app);
+        if (bm) {
+            int leader = __ffsll((unsigned long long)bm) - 1;
+            u32 b = 0;
+            if (lane == leader) b = atomicAdd(&st->count, (u32)__popcll(bm));
+            b = __shfl(b, leader, WAVE);
+            if (app) next_node[b + (u32)__popcll(bm & lt)] = m;
+        }
+    }
+}
+
+__global__ void gw_gather_k(const u32 *node, const u32 *L, u32 *label, const WccDev *st) {
+    if (st->done) return;
+    const u64 n = st->count;
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) label[i] = L[node[i]];
+}
+
+// The reference's stop rule, on the device: count the round, stop when two
+// consecutive delta sizes are equal.
+__global__ void gw_end_k(WccDev *st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || st->done) return;
+    u64 cur = st->count;
+    st->it += 1;
+    st->nf = cur;
+    if (cur == st->prev || (st->max_it && st->it >= st->max_it)) st->done = 1;
+    else st->prev = cur;
+}
+
+static void gw_capture_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 V, u64 E,
+                             WccDev *st, u32 *fnode, u32 *flabel, u32 *nnode, u32 *nlabel, u64 *bsum) {
+    const u64 nbv = (V + GW_TILE - 1) / GW_TILE;
+    const unsigned gb = grid_for(V, GW_TILE);
+    hipLaunchKernelGGL(gw_begin_k, dim3(1), dim3(64), 0, ctx->stream, st);
+    hipLaunchKernelGGL(gw_degree_scan_k, dim3(gb), dim3(GW_T), 0, ctx->stream, (const u32 *)fnode,
+                       (const u32 *)ix.gstart.p, s.cnt.p, s.pstart.p, bsum, (const WccDev *)st);
+    hipLaunchKernelGGL(gw_scan_blocks_k, dim3(1), dim3(1024), 0, ctx->stream, bsum, nbv, st);
+    hipLaunchKernelGGL(gw_scan_apply_k, dim3(gb), dim3(GW_T), 0, ctx->stream, (const u32 *)s.cnt.p,
+                       (const u64 *)bsum, s.offs.p, (const WccDev *)st);
+    hipLaunchKernelGGL(gw_relax_k, dim3(grid_for(E, WR_TILE)), dim3(WR_T), 0, ctx->stream,
+                       (const u32 *)flabel, (const u32 *)s.pstart.p, (const u64 *)s.offs.p, nbr, s.L.p,
+                       s.mark.p, nnode, st);
+    hipLaunchKernelGGL(gw_gather_k, dim3(sgrid(ctx, V)), dim3(256), 0, ctx->stream, (const u32 *)nnode,
+                       (const u32 *)s.L.p, nlabel, (const WccDev *)st);
+    hipLaunchKernelGGL(gw_end_k, dim3(1), dim3(64), 0, ctx->stream, st);
+}
+
+// Runs the rounds of wcc_solve with captured graphs of `kr` rounds (even).
+// Returns the iteration count; the labels are left in s.L.
+int wcc_rounds_graph(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 ne, int max_it,
+                     int kr) {
+    const u64 V = s.V;
+    if (V == 0) return 1;   // empty graph: the first round already sees 0 == 0
+    if (kr < 2) kr = 2;
+    kr &= ~1;
+    s.cnt.ensure(V);
+    s.pstart.ensure(V);
+    s.offs.ensure(V + 1);
+    DBuf<u32> nlabel(ctx, V);
+    DBuf<u64> bsum(ctx, (V + GW_TILE - 1) / GW_TILE + 1);
+    DBuf<u64> stbuf(ctx, (sizeof(WccDev) + 7) / 8);
+    WccDev *st = (WccDev *)stbuf.p;
+    WccDev h;
+    std::memset(&h, 0, sizeof(h));
+    h.nf = V;
+    h.prev = V;
+    h.max_it = (u32)(max_it > 0 ? max_it : 0);
+    HIPCHK(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+    ctx->sync();
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool prof = ctx->prof;
+    ctx->prof = false;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < kr; r++) {
+        if (r % 2 == 0)
+            gw_capture_round(ctx, s, ix, nbr, V, ne, st, s.fnode.p, s.flabel.p, s.nnode.p, nlabel.p, bsum.p);
+        else
+            gw_capture_round(ctx, s, ix, nbr, V, ne, st, s.nnode.p, nlabel.p, s.fnode.p, s.flabel.p, bsum.p);
+    }
+    hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    ctx->prof = prof;
+    HIPCHK(ce);
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        HIPCHK(ie);
+    }
+    int it = 0;
+    try {
+        while (true) {
+            if (ctx->prof) ctx->prof_begin();
+            HIPCHK(hipGraphLaunch(exec, ctx->stream));
+            if (ctx->prof) ctx->prof_end(DLG_K_WCC_RELAX, 0.0);
+            HIPCHK(hipMemcpyAsync(ctx->h_scalars, st, sizeof(WccDev), hipMemcpyDeviceToHost, ctx->stream));
+            ctx->sync();
+            WccDev cur;
+            std::memcpy(&cur, ctx->h_scalars, sizeof(cur));
+            it = (int)cur.it;
+            if (cur.done) break;
+        }
+    } catch (...) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        throw;
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    return it;
+}
+
+}  // namespace dlg

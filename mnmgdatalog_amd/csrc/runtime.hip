@@ -44,11 +44,15 @@ void *Pool::get(size_t bytes) {
         peak_bytes_ = std::max(peak_bytes_, live_bytes_);
         return p;
     }
-    if (budget_ && live_bytes_ + sz > budget_) {
-        char buf[160];
-        std::snprintf(buf, sizeof(buf), "device memory budget exceeded: need %zu more bytes, budget %zu",
-                      sz, budget_);
-        fail(DLG_OUT_OF_MEMORY, buf);
+    if (budget_ && live_bytes_ + cached_bytes_ + sz > budget_) {
+        // the budget covers cached blocks too: release cache before giving up
+        trim_to(budget_ > live_bytes_ + sz ? budget_ - live_bytes_ - sz : 0);
+        if (live_bytes_ + sz > budget_) {
+            char buf[160];
+            std::snprintf(buf, sizeof(buf),
+                          "device memory budget exceeded: need %zu more bytes, budget %zu", sz, budget_);
+            fail(DLG_OUT_OF_MEMORY, buf);
+        }
     }
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, sz);
@@ -84,6 +88,21 @@ void Pool::trim() {
         for (void *p : kv.second) (void)hipFree(p);
     free_.clear();
     cached_bytes_ = 0;
+}
+
+void Pool::trim_to(size_t keep) {
+    // free the largest cached blocks first
+    while (cached_bytes_ > keep && !free_.empty()) {
+        auto it = std::prev(free_.end());
+        if (it->second.empty()) {
+            free_.erase(it);
+            continue;
+        }
+        (void)hipFree(it->second.back());
+        it->second.pop_back();
+        cached_bytes_ -= it->first;
+        if (it->second.empty()) free_.erase(it);
+    }
 }
 
 }  // namespace dlg

@@ -248,3 +248,38 @@ def test_cli_missing_file_fails(tmp_path):
     p = subprocess.run([os.path.join(dlg.BIN_DIR, "tc"), str(tmp_path / "nope.bin")], capture_output=True,
                        text=True, timeout=120)
     assert p.returncode != 0 and "Error opening file" in p.stderr
+
+
+# ------------------------------------------------------------------ strategies
+@pytest.mark.parametrize("name", ["data_223001.bin", "data_214078.bin", "paper.bin", "dummy.bin"])
+def test_wcc_graph_captured_and_eager_agree(ctx, orc, name):
+    rows = read_bin(name)
+    want = orc.wcc(rows)
+    for strategy in (1, 2, 0):                       # eager rounds, captured rounds, auto
+        assert_same(ctx.run("wcc", rows, strategy=strategy), want, secondary=True)
+
+
+def test_wcc_graph_captured_max_iterations(ctx):
+    rows = read_bin("data_223001.bin")
+    a = ctx.run("wcc", rows, strategy=1, max_iterations=37)
+    b = ctx.run("wcc", rows, strategy=2, max_iterations=37)
+    assert a["iterations"] == b["iterations"] == 37
+    assert np.array_equal(a["rows"], b["rows"])
+
+
+@pytest.mark.parametrize("prog,name", [("tc", "data_23874.bin"), ("sg", "data_7035.bin"),
+                                       ("tc", "data_22.bin")])
+def test_full_relation_strategies_agree(ctx, orc, prog, name):
+    rows = read_bin(name)
+    want = getattr(orc, prog)(rows)
+    for strategy in (1, 2):                          # one merged array, tiered runs
+        assert_same(ctx.run(prog, rows, strategy=strategy), want)
+
+
+@pytest.mark.parametrize("prog", ["tc", "sg"])
+def test_max_iterations_prefix_matches_oracle(ctx, orc, prog):
+    rows = read_bin("data_7035.bin")
+    for k in (1, 2, 5):
+        want = getattr(orc, prog)(rows, max_iterations=k)
+        for strategy in (1, 2):
+            assert_same(ctx.run(prog, rows, strategy=strategy, max_iterations=k), want)

@@ -1,0 +1,86 @@
+"""Served form: requests of several programs and sizes in flight together
+(continuous batching, several streams, a resident graph) must each give the
+same result as the request run alone."""
+import numpy as np
+import pytest
+
+import mnmgdatalog_amd as dlg
+from conftest import read_bin
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dlg.Ctx(0)
+    yield c
+    c.close()
+
+
+def same(got, want, secondary=False):
+    assert got["status"] == dlg.DLG_OK
+    assert got["iterations"] == want["iterations"]
+    assert got["output_size"] == want["output_size"]
+    if secondary:
+        assert got["secondary"] == want["secondary"]
+    assert np.array_equal(got["rows"], want["rows"])
+
+
+def mixed_requests(seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(60):
+        m = int(np.exp(rng.uniform(np.log(50), np.log(3000))))
+        if i % 2 == 0:
+            rows = dlg.gen_erdos_renyi(max(8, int(m / rng.uniform(2, 4))), m, 1000 + i)
+        else:
+            rows = dlg.gen_rmat(max(4, int(np.ceil(np.log2(max(m / 4, 2))))), m, 0.57, 0.19, 0.19, 1000 + i)
+        prog = ["tc", "wcc", "sg", "tc", "wcc", "single_join"][i % 6]
+        reqs.append((prog, rows))
+    reqs.append(("tc", read_bin("data_7035.bin")))
+    reqs.append(("wcc", read_bin("data_214078.bin")))
+    reqs.append(("sg", read_bin("data_7035.bin")))
+    return reqs
+
+
+def test_served_results_equal_alone(ctx):
+    reqs = mixed_requests(5)
+    resident = dlg.gen_rmat(14, 60000, 0.57, 0.19, 0.19, 77, undirected=True)
+    with dlg.Server(0, workers=4, small_rows=5000) as srv:
+        gid = srv.add_graph(resident)
+        tickets = [srv.submit(p, r) for p, r in reqs]
+        gt = [srv.submit("wcc", graph=gid) for _ in range(3)]
+        got = [srv.wait(t) for t in tickets]
+        ggot = [srv.wait(t) for t in gt]
+    for (prog, rows), g in zip(reqs, got):
+        same(g, ctx.run(prog, rows), secondary=prog == "wcc")
+    alone = ctx.run("wcc", resident)
+    for g in ggot:
+        same(g, alone, secondary=True)
+    assert max(g["batch_size"] for g in got) > 1            # small requests were batched
+    assert len({g["slot"] for g in got}) > 1                 # several slots ran concurrently
+    for g in got + ggot:
+        assert g["submit_s"] <= g["start_s"] <= g["end_s"]
+
+
+def test_served_refuses_what_cannot_fit():
+    rng = np.random.default_rng(1)
+    big = rng.integers(0, 2**31 - 1, size=(400_000, 2)).astype(np.int32)   # predicted closure far too large
+    with dlg.Server(0, workers=2) as srv:
+        with pytest.raises(dlg.DlgError) as e:
+            srv.submit("tc", big)
+        assert e.value.status == dlg.DLG_REFUSED
+        t = srv.submit("wcc", big)                                   # the same rows fit for WCC
+        r = srv.wait(t)
+        assert r["status"] == dlg.DLG_OK and r["nodes"] > 0
+
+
+def test_served_wait_timeout_and_unknown_ticket():
+    with dlg.Server(0, workers=1) as srv:
+        t = srv.submit("tc", read_bin("data_23874.bin"))
+        first = srv.wait(t, timeout=0.0)
+        if first is None:                                           # still running
+            first = srv.wait(t)
+        assert first["status"] == dlg.DLG_OK and first["output_size"] == 481121
+        with pytest.raises(dlg.DlgError):
+            srv.wait(t)                                             # handed over once
