@@ -90,6 +90,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def rmat_scale_for(m):
+    """2^ceil(log2(m / 4)) vertices, raised until m distinct edges fit."""
+    import math
+    s = max(3, math.ceil(math.log2(max(m / 4, 2))))
+    while (2 ** s) * (2 ** s - 1) // 4 < m:
+        s += 1
+    return s
+
+
 def c4_instances(count, max_edges, seed_base=SEED):
     """Batched-TC instances with edge counts log-uniform in [1k, max_edges]:
     even i Erdos-Renyi (n = m / d, d ~ U[2, 4]), odd i R-MAT(.57, .19, .19, .05) on
@@ -106,7 +115,7 @@ def c4_instances(count, max_edges, seed_base=SEED):
             n = max(8, int(round(m / d)))
             out.append(dlg.gen_erdos_renyi(n, min(m, n * (n - 1) // 2), seed_base + i))
         else:
-            scale = max(3, int(np.ceil(np.log2(m / 4))))
+            scale = rmat_scale_for(m)
             out.append(dlg.gen_rmat(scale, m, 0.57, 0.19, 0.19, seed_base + i))
     return out
 
@@ -164,7 +173,7 @@ def bench_served(args):
             n = max(8, int(round(m / rng.uniform(2.0, 4.0))))
             rows = dlg.gen_erdos_renyi(n, min(m, n * (n - 1) // 2), SEED + 1000 + i)
         else:
-            scale = max(3, int(np.ceil(np.log2(max(m / 4, 2)))))
+            scale = rmat_scale_for(m)
             rows = dlg.gen_rmat(scale, m, 0.57, 0.19, 0.19, SEED + 1000 + i)
         reqs.append((prog, rows))
     log(f"served: {len(reqs)} requests generated")

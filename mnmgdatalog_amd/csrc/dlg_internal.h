@@ -241,8 +241,13 @@ void scan_u32_u64_async(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *d_tot
 // Stable LSD radix sort of u64 keys; result in `data`.  `tmp` is scratch of >= n.
 void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits = 0);
 // Same on a pool buffer: the sorted keys may end in the other buffer, in which
-// case `data` and `tmp` swap ownership (no copy).
-void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits = 0);
+// case `data` and `tmp` swap ownership (no copy).  `bounds` (min key, max key,
+// min value, max value of the u64 halves) may be given when known to cover
+// every element; otherwise a pass over the data computes them.
+void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits = 0,
+                const u32 *bounds = nullptr);
+// Ranges of the two u64 halves over data[0..n): out = {min_hi, max_hi, min_lo, max_lo}.
+void tuple_bounds(dlg_ctx *ctx, const u64 *data, u64 n, u32 *out);
 // Compaction modes.
 enum SelectMode {
     SEL_UNIQUE = 0,         // keep first of each run of equal elements

@@ -69,9 +69,10 @@ Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
 
 // sort + unique of buf[0..n) into out; returns distinct count.  buf is clobbered.
 static u64 sort_unique_into(dlg_ctx *ctx, DBuf<u64> &buf, u64 n, DBuf<u64> &tmp, DBuf<u64> &out,
-                            int mode, const u64 *other, u64 n_other, int bits) {
+                            int mode, const u64 *other, u64 n_other, int bits,
+                            const u32 *bounds = nullptr) {
     if (n == 0) return 0;
-    radix_sort(ctx, buf, n, tmp, bits);
+    radix_sort(ctx, buf, n, tmp, bits, bounds);
     out.ensure(n);
     return select(ctx, mode, buf.p, n, other, n_other, out.p);
 }
@@ -171,12 +172,23 @@ struct FullRel {
     }
 };
 
+// Every tuple of a TC / SG fixpoint pairs two node ids of the input, so the
+// id range of the edge relation bounds both halves of every tuple; the radix
+// sorts of the loop use it instead of a min/max pass over each join output.
+static void id_bounds(dlg_ctx *ctx, const u64 *edge, u64 n, u32 *b) {
+    u32 h[4];
+    tuple_bounds(ctx, edge, n, h);
+    u32 lo = std::min(h[0], h[2]), hi = std::max(h[1], h[3]);
+    if (n == 0) lo = hi = 0;
+    b[0] = lo; b[1] = hi; b[2] = lo; b[3] = hi;
+}
+
 // Sort J and keep its tuples that are new: N = unique(J) \ F.
 static u64 new_tuples(dlg_ctx *ctx, DBuf<u64> &J, u64 nj, DBuf<u64> &tmp, DBuf<u64> &N, FullRel &F,
-                      const Opts &o, PhaseClock &pc) {
+                      const Opts &o, PhaseClock &pc, const u32 *bounds) {
     pc.mark(DLG_PH_DEDUP);
     if (nj == 0) return 0;
-    radix_sort(ctx, J, nj, tmp, o.bits);
+    radix_sort(ctx, J, nj, tmp, o.bits, bounds);
     pc.mark(DLG_PH_MERGE);
     N.ensure(nj);
     return F.new_tuples(J.p, nj, N.p);
@@ -193,12 +205,14 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     DBuf<u64> edge(ctx, n), delta(ctx, n), J(ctx, n), N(ctx), tmp(ctx);
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, edge.p);   // e(src, dst) keyed by src
     pack_rows(ctx, d_rows, n, DLG_ORIENT_VK, J.p);      // Δ0 rows (dst, src)
+    u32 ib[4];
+    id_bounds(ctx, edge.p, n, ib);
     pc.mark(DLG_PH_HASHTABLE);
     Index ix(ctx);
     index_build(ctx, ix, edge.p, n, o.lf, false);      // duplicates kept, as the reference
     edge.release();
     pc.mark(DLG_PH_DEDUP);
-    u64 nd = sort_unique_into(ctx, J, n, tmp, delta, SEL_UNIQUE, nullptr, 0, o.bits);
+    u64 nd = sort_unique_into(ctx, J, n, tmp, delta, SEL_UNIQUE, nullptr, 0, o.bits, ib);
     FullRel F(ctx, o);                                  // F = Δ0, added after the first join
     JoinScratch js(ctx);
     int it = 0;
@@ -208,7 +222,7 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         out.raw += (i64)nj;
         pc.mark(DLG_PH_MERGE);
         F.add(delta, nd);
-        u64 nn = new_tuples(ctx, J, nj, tmp, N, F, o, pc);
+        u64 nn = new_tuples(ctx, J, nj, tmp, N, F, o, pc, ib);
         it++;
         if (hook) hook->on_round(ctx, N.p, nn, it);
         if (nn == 0) break;                         // |F| unchanged (tc.cu:294-300)
@@ -231,8 +245,10 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     pc.mark(DLG_PH_INIT);
     DBuf<u64> edge(ctx), J(ctx, n), T(ctx), delta(ctx), N(ctx), tmp(ctx);
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, J.p);
+    u32 ib[4];
+    id_bounds(ctx, J.p, n, ib);
     pc.mark(DLG_PH_DEDUP);
-    u64 ne = sort_unique_into(ctx, J, n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
+    u64 ne = sort_unique_into(ctx, J, n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits, ib);
     pc.mark(DLG_PH_HASHTABLE);
     Index ix(ctx);
     index_build(ctx, ix, edge.p, ne, o.lf, true);
@@ -242,7 +258,7 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     out.raw += (i64)nb;
     edge.release();
     pc.mark(DLG_PH_DEDUP);
-    u64 nd = sort_unique_into(ctx, J, nb, tmp, delta, SEL_UNIQUE_NODIAG, nullptr, 0, o.bits);
+    u64 nd = sort_unique_into(ctx, J, nb, tmp, delta, SEL_UNIQUE_NODIAG, nullptr, 0, o.bits, ib);
     FullRel F(ctx, o);                                  // F = base, added after the first join
     int it = 0;
     while (true) {
@@ -252,11 +268,11 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         pc.mark(DLG_PH_MERGE);
         F.add(delta, nd);
         pc.mark(DLG_PH_DEDUP);
-        u64 nt = sort_unique_into(ctx, J, n1, tmp, T, SEL_UNIQUE, nullptr, 0, o.bits);
+        u64 nt = sort_unique_into(ctx, J, n1, tmp, T, SEL_UNIQUE, nullptr, 0, o.bits, ib);
         pc.mark(DLG_PH_JOIN);
         u64 n2 = join_into(ctx, ix, T.p, nt, DLG_PROJ_PV_HV, js, J);      // (x, y)
         out.raw += (i64)n2;
-        u64 nn = new_tuples(ctx, J, n2, tmp, N, F, o, pc);
+        u64 nn = new_tuples(ctx, J, n2, tmp, N, F, o, pc, ib);
         it++;
         if (hook) hook->on_round(ctx, N.p, nn, it);
         if (nn == 0) break;                         // sg.cu:376-389

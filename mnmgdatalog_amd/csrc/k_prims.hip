@@ -404,20 +404,27 @@ static void radix_passes_sized(dlg_ctx *ctx, u64 *&src, u64 *&dst, u64 n, RadixK
         radix_passes<BITS, RS_SMALL_T, RS_SMALL_I>(ctx, src, dst, n, key, total_bits);
 }
 
-void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits) {
+void tuple_bounds(dlg_ctx *ctx, const u64 *data, u64 n, u32 *h) {
+    DBuf<u32> mm(ctx, 4);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 0.0, minmax_init_k, 1, 64, 0, mm.p);
+    if (n) {
+        unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 8);
+        DLG_LAUNCH(ctx, DLG_K_SORT_HIST, 8.0 * n, minmax_k, g, 256, 0, data, n, mm.p);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->h_scalars, mm.p, 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(h, ctx->h_scalars, 16);
+}
+
+void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits, const u32 *bounds) {
     if (n < 2) return;
     if (n > 0xffffffffull) fail(DLG_OVERFLOW, "radix_sort: more than 2^32-1 tuples in one buffer");
     if (bits <= 0) bits = ctx->radix_bits;
     if (bits < 4) bits = 4;
     if (bits > 8) bits = 8;
-    DBuf<u32> mm(ctx, 4);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 0.0, minmax_init_k, 1, 64, 0, mm.p);
-    unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 8);
-    DLG_LAUNCH(ctx, DLG_K_SORT_HIST, 8.0 * n, minmax_k, g, 256, 0, (const u64 *)data.p, n, mm.p);
     u32 h[4];
-    HIPCHK(hipMemcpyAsync(ctx->h_scalars, mm.p, 16, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    std::memcpy(h, ctx->h_scalars, 16);
+    if (bounds) std::memcpy(h, bounds, sizeof(h));
+    else tuple_bounds(ctx, data.p, n, h);
     RadixKey key;
     key.min_hi = h[0];
     key.min_lo = h[2];
@@ -441,7 +448,7 @@ void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits) {
     DBuf<u64> view(ctx);
     view.p = data;          // borrowed: released without returning it to the pool
     view.cap = n;
-    radix_sort(ctx, view, n, tmp, bits);
+    radix_sort(ctx, view, n, tmp, bits, nullptr);
     if (view.p != data) {   // sorted copy ended in tmp's old block; bring it back
         copy_u64(ctx, data, view.p, n);
         tmp.swap(view);     // tmp owns that block again

@@ -34,7 +34,10 @@ def mixed_requests(seed):
         if i % 2 == 0:
             rows = dlg.gen_erdos_renyi(max(8, int(m / rng.uniform(2, 4))), m, 1000 + i)
         else:
-            rows = dlg.gen_rmat(max(4, int(np.ceil(np.log2(max(m / 4, 2))))), m, 0.57, 0.19, 0.19, 1000 + i)
+            scale = max(4, int(np.ceil(np.log2(max(m / 4, 2)))))
+            while (2 ** scale) * (2 ** scale - 1) // 4 < m:
+                scale += 1
+            rows = dlg.gen_rmat(scale, m, 0.57, 0.19, 0.19, 1000 + i)
         prog = ["tc", "wcc", "sg", "tc", "wcc", "single_join"][i % 6]
         reqs.append((prog, rows))
     reqs.append(("tc", read_bin("data_7035.bin")))
