@@ -60,7 +60,7 @@ def workload_rows(name):
 CPU_SAMPLE_ITERS = {"sg_mesh": 3, "tc_powerlaw": 0, "tc_usroad": 40, "wcc_grid": 0, "single_join": 0}
 
 
-def cpu_baseline(name, prog, rows, ctx, threads):
+def cpu_baseline(name, prog, rows, ctx, threads, strategy=0):
     import numpy as np
 
     import oracle
@@ -73,7 +73,7 @@ def cpu_baseline(name, prog, rows, ctx, threads):
         want = fn(rows, threads=threads)
     dt = time.perf_counter() - t0
     extra = {"max_iterations": k} if k and prog in ("tc", "sg") else {}
-    got = ctx.run(prog, rows, keep_output=True, **extra)
+    got = ctx.run(prog, rows, keep_output=True, strategy=strategy, **extra)
     same = (got["iterations"] == want["iterations"] and got["output_size"] == want["output_size"]
             and np.array_equal(got["rows"], want["rows"]))
     sample = (f"{prog} on the same input, first {want['iterations']} fixpoint iterations"
@@ -271,6 +271,7 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="served: slots")
     ap.add_argument("--rmat-scale", type=int, default=23, help="served: resident R-MAT scale")
     ap.add_argument("--max-request-gb", type=int, default=8, help="served: per-request limit")
+    ap.add_argument("--strategy", type=int, default=0, help="full-relation layout (0 auto, 1..3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
@@ -293,14 +294,14 @@ def main():
         d_rows = ctx.upload(rows)
         res = None
         for _ in range(args.warmup):
-            res = ctx.run(prog, None, keep_output=False, device_rows=d_rows, n=n)
+            res = ctx.run(prog, None, keep_output=False, device_rows=d_rows, n=n, strategy=args.strategy)
         ctx.kernel_stats_reset()
         ctx.profiling(True)
         ctx.check(ctx.L.dlg_ctx_sync(ctx.h), "sync")
         t0 = time.perf_counter()
         dev_s = 0.0
         for _ in range(args.steps):
-            res = ctx.run(prog, None, keep_output=False, device_rows=d_rows, n=n)
+            res = ctx.run(prog, None, keep_output=False, device_rows=d_rows, n=n, strategy=args.strategy)
             dev_s += res["device_seconds"]
         ctx.check(ctx.L.dlg_ctx_sync(ctx.h), "sync")
         t1 = time.perf_counter()
@@ -320,7 +321,7 @@ def main():
                    for k, v in ks.items() if v["launches"]}
         cpu = None
         if not args.no_cpu:
-            cpu = cpu_baseline(args.workload, prog, rows, ctx, args.cpu_threads)
+            cpu = cpu_baseline(args.workload, prog, rows, ctx, args.cpu_threads, args.strategy)
         ctx.free(d_rows)
     line = {
         "metric": "derived tuples/s, 1 MI355X, bit-exact; batched/served vs plain speedup + p99",
@@ -328,7 +329,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": dict(cfg, iterations=res["iterations"], output_tuples=int(out_tuples),
-                       raw_join_rows=int(res["raw_join_rows"])),
+                       raw_join_rows=int(res["raw_join_rows"]), strategy=args.strategy),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": alg_per_launch, "avg_launch_ms": avg_ms,

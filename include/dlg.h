@@ -63,7 +63,7 @@ dlg_status dlg_profiling(dlg_ctx *ctx, int enable);
 enum {
     DLG_K_PACK = 0, DLG_K_HASH_BUILD, DLG_K_JOIN_COUNT, DLG_K_SCAN, DLG_K_JOIN_EXPAND,
     DLG_K_SORT_HIST, DLG_K_SORT_UPSWEEP, DLG_K_SORT_DOWNSWEEP, DLG_K_SELECT, DLG_K_MERGE,
-    DLG_K_WCC_RELAX, DLG_K_WCC_FINAL, DLG_K_HSET_INSERT, DLG_K_MISC, DLG_K_COUNT
+    DLG_K_WCC_RELAX, DLG_K_WCC_FINAL, DLG_K_HSET_INSERT, DLG_K_BITMAP_INSERT, DLG_K_MISC, DLG_K_COUNT
 };
 const char *dlg_kernel_name(int kernel_id);
 /* Accumulated since the last reset: launches, summed duration (ms), summed
@@ -153,7 +153,9 @@ typedef struct dlg_opts {
     int32_t strategy;        /* 0 auto; 1 the reference's scheme (one sorted full
                                 relation merged every round; WCC rounds launched one
                                 by one); 2 tiered sorted runs for the full relation and
-                                hipGraph-captured WCC rounds                          */
+                                hipGraph-captured WCC rounds; 3 append-only full
+                                relation with a device hash set for the set
+                                difference, sorted once at the end                    */
     int32_t max_iterations;  /* 0 = run to the fixpoint; >0 stops early (sampling)     */
     int32_t keep_output;     /* 1: copy the output rows to host (dlg_result.rows)      */
     int32_t reserved[6];

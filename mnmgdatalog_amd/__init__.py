@@ -25,7 +25,8 @@ STATUS_NAMES = ["OK", "INVALID_INPUT", "OUT_OF_MEMORY", "OVERFLOW", "DEVICE_ERRO
 PHASES = ["join", "buffer_preparation", "communication", "deduplication", "merge", "initialization",
           "hashtable", "finalization", "clear", "file_io"]
 KERNELS = ["pack", "hash_build", "join_count", "scan", "join_expand", "sort_minmax", "sort_upsweep",
-           "sort_downsweep", "select", "merge", "wcc_relax", "wcc_final", "hashset_insert", "misc"]
+           "sort_downsweep", "select", "merge", "wcc_relax", "wcc_final", "hashset_insert",
+           "bitmap_insert", "misc"]
 
 PROJ_HV_PV, PROJ_PV_HV = 0, 1
 ORIENT_KV, ORIENT_VK = 0, 1

@@ -181,19 +181,54 @@ static std::vector<u64> boundaries(dlg_ctx *ctx, const u64 *a, u64 n, const DBuf
     return h;
 }
 
+// Per-instance count of unordered tuples: instance = last i with ub[i] <= key.
+__global__ void key_hist_k(const u64 *a, u64 n, const u32 *ub, u32 B, u32 *cnt) {
+    __shared__ u32 s_ub[MAX_LDS_INST + 1];
+    const bool lds = B <= MAX_LDS_INST;
+    if (lds)
+        for (u32 i = threadIdx.x; i <= B; i += blockDim.x) s_ub[i] = ub[i];
+    __syncthreads();
+    const u32 *tab = lds ? s_ub : ub;
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        u32 k = tkey(a[i]);
+        u32 lo = 0, hi = B;   // tab[0] <= every key; the last range is open-ended
+        while (hi - lo > 1) {
+            u32 mid = (lo + hi) >> 1;
+            if (tab[mid] <= k) lo = mid; else hi = mid;
+        }
+        atomicAdd(&cnt[lo], 1u);
+    }
+}
+
 struct BatchHook : RoundHook {
     const DBuf<u32> &ub;
     int B;
     DBuf<u64> pos;
+    DBuf<u32> cnt;
     std::vector<int> iters;
     std::vector<char> done;
-    BatchHook(dlg_ctx *ctx, const DBuf<u32> &u, int b) : ub(u), B(b), pos(ctx, b + 1), iters(b, 0), done(b, 0) {}
-    void on_round(dlg_ctx *ctx, const u64 *N, u64 nn, int round) override {
-        std::vector<u64> p = boundaries(ctx, N, nn, ub, B, pos);
+    BatchHook(dlg_ctx *ctx, const DBuf<u32> &u, int b)
+        : ub(u), B(b), pos(ctx, b + 1), cnt(ctx, b), iters(b, 0), done(b, 0) {}
+    void on_round(dlg_ctx *ctx, const u64 *N, u64 nn, int round, bool sorted) override {
+        std::vector<u64> c(B, 0);
+        if (sorted) {
+            std::vector<u64> p = boundaries(ctx, N, nn, ub, B, pos);
+            for (int i = 0; i < B; i++) c[i] = p[i + 1] - p[i];
+        } else if (nn) {
+            fill_u32(ctx, cnt.p, B, 0u);
+            DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nn, key_hist_k, sgrid(ctx, nn), ctx->block_size, 0, N, nn,
+                       (const u32 *)ub.p, (u32)B, cnt.p);
+            std::vector<u32> h(B);
+            HIPCHK(hipMemcpyAsync(h.data(), cnt.p, 4 * B, hipMemcpyDeviceToHost, ctx->stream));
+            ctx->sync();
+            for (int i = 0; i < B; i++) c[i] = h[i];
+        }
         for (int i = 0; i < B; i++) {
             if (done[i]) continue;
             iters[i] = round;
-            if (p[i + 1] == p[i]) done[i] = 1;
+            if (c[i] == 0) done[i] = 1;
         }
     }
 };

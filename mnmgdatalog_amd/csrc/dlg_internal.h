@@ -269,6 +269,56 @@ struct RunSet {
 };
 // Unique elements of sorted `in` that no run of `rs` contains.
 u64 select_unique_not_in_runs(dlg_ctx *ctx, const u64 *in, u64 n, const RunSet &rs, u64 *out);
+// Second half of a compaction whose count kernel (tiles of 2048, flags as
+// ballot words, per-tile counts in bsum) already ran: scan, read the total
+// back, scatter.  Returns the kept count.
+u64 select_scatter(dlg_ctx *ctx, const u64 *in, u64 n, const u64 *flags, DBuf<u64> &bsum, DBuf<u64> &tot,
+                   u64 *out);
+
+// Exclusive scan of bsum[0..nb) in place by one block; the total to *d_total.
+void scan_blocks(dlg_ctx *ctx, u64 *bsum, u64 nb, u64 *d_total);
+
+// ------------------------------------------------------------- bitmap (k_bitmap.hip)
+struct Index;
+struct JoinScratch;
+// The full relation as a W*W-bit map over the id range [lo, lo + W); bit
+// (k - lo) * W + (v - lo) for tuple (k, v), or (v - lo) * W + (k - lo) when
+// transposed.
+struct Bitmap {
+    DBuf<u32> words;
+    DBuf<u64> count;   // append counter of the insert kernels
+    u32 lo = 0;
+    u64 W = 0, nbits = 0, nwords = 0;
+    bool tr = false;
+    explicit Bitmap(dlg_ctx *c) : words(c), count(c) {}
+};
+void bitmap_init(dlg_ctx *ctx, Bitmap &bm, u32 lo, u32 hi, bool transposed);
+// probe ⋈ index, each row inserted; rows that set a clear bit go to `out`
+// (unordered).  Returns their number; *raw += |J|.
+u64 bitmap_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, int proj,
+                       bool nodiag, Bitmap &bm, DBuf<u64> &out, u64 *raw);
+// Insert a list; the tuples that were absent go to `out`.
+u64 bitmap_insert_list(dlg_ctx *ctx, const u64 *in, u64 n, bool nodiag, Bitmap &bm, u64 *out);
+// Clear the bits of a list whose bits are the only ones set in their words.
+void bitmap_clear_list(dlg_ctx *ctx, const u64 *in, u64 n, Bitmap &bm);
+// All set bits as sorted tuples; `expect` = the number of set bits.  A
+// transposed map is emitted in (value, key) order and then radix sorted with
+// `bits` bits per pass.
+u64 bitmap_emit(dlg_ctx *ctx, const Bitmap &bm, DBuf<u64> &out, u64 expect, int bits);
+
+// ------------------------------------------------------------- hash set (k_hashset.hip)
+struct HashSet {
+    DBuf<u64> slots;   // open addressing, EMPTY_SLOT = free
+    DBuf<u32> flag;    // holds the one tuple equal to EMPTY_SLOT
+    u64 cap = 0, count = 0;
+    explicit HashSet(dlg_ctx *c) : slots(c), flag(c) {}
+};
+// Make room for `need` elements at load <= lf; a new table is refilled from all[0..n_all).
+void hashset_reserve(dlg_ctx *ctx, HashSet &hs, u64 need, double lf, const u64 *all, u64 n_all);
+// Insert tuples that are absent and pairwise distinct.
+void hashset_insert(dlg_ctx *ctx, HashSet &hs, const u64 *x, u64 n);
+// First of each equal run of sorted `in` that the set does not contain.
+u64 select_unique_not_in_hash(dlg_ctx *ctx, const u64 *in, u64 n, const HashSet &hs, u64 *out);
 // Merge of sorted a, b into out (a first on ties).
 void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out);
 void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out);

@@ -45,12 +45,17 @@ inline double secs(Clock::time_point a, Clock::time_point b) {
     return std::chrono::duration<double>(b - a).count();
 }
 
-// Called once per fixpoint round with the round's new tuples N (sorted, unique,
-// disjoint from the full relation) -- the batched form counts them per instance.
+// Called once per fixpoint round with the round's new tuples N (unique,
+// disjoint from the full relation; `sorted` tells whether N is in tuple
+// order) -- the batched form counts them per instance.
 struct RoundHook {
     virtual ~RoundHook() {}
-    virtual void on_round(dlg_ctx *ctx, const u64 *N, u64 nn, int round) = 0;
+    virtual void on_round(dlg_ctx *ctx, const u64 *N, u64 nn, int round, bool sorted) = 0;
 };
+
+// Id-range width up to which TC / SG keep the full relation as a bitmap
+// (W*W bits <= 2^36 bits = 8 GiB) when the strategy is automatic.
+constexpr u64 BITMAP_MAX_W = 1ull << 18;
 
 struct FixpointOut {
     DBuf<u64> full;   // final relation, sorted

@@ -53,7 +53,7 @@ Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
             fail(DLG_INVALID_INPUT, "load_factor out of (0, 0.95]");
         if (o->radix_bits != 0 && (o->radix_bits < 4 || o->radix_bits > 8))
             fail(DLG_INVALID_INPUT, "radix_bits out of 4..8");
-        if (o->strategy < 0 || o->strategy > 2) fail(DLG_INVALID_INPUT, "strategy out of 0..2");
+        if (o->strategy < 0 || o->strategy > 4) fail(DLG_INVALID_INPUT, "strategy out of 0..4");
         if (o->max_iterations < 0) fail(DLG_INVALID_INPUT, "negative max_iterations");
         if (o->block_size != 0 && (o->block_size < 64 || o->block_size > 1024 || o->block_size % 64))
             fail(DLG_INVALID_INPUT, "block_size must be 64..1024 and a multiple of 64");
@@ -109,20 +109,31 @@ void check_rows_arg(int64_t n, const void *rows) {
     if ((u64)n >= 0x7fffffffull) fail(DLG_OVERFLOW, "more than 2^31-2 input rows");
 }
 
-// The full relation F.  Strategy 1 keeps one sorted array and merges every
-// round's new tuples into it (the reference's scheme, tc.cu:268-286): each
-// round rewrites all of F.  Otherwise F is a list of sorted, disjoint runs
-// whose sizes fall geometrically from the oldest to the newest; a round's new
-// tuples become the newest run and runs of similar size are merged (a binary
-// counter), so each tuple is rewritten O(log rounds) times.  N = unique(J) \ F
-// searches every run.  collapse() merges everything into one sorted array.
+// The full relation F, three layouts with identical results:
+//  1  one sorted array; every round's new tuples are merged into it (the
+//     reference's scheme, tc.cu:268-286): each round rewrites all of F.
+//  2  sorted, disjoint runs whose sizes fall geometrically from the oldest to
+//     the newest; a round's new tuples become the newest run and runs of
+//     similar size are merged (a binary counter), so each tuple is rewritten
+//     O(log rounds) times; N = unique(J) \ F searches every run.
+//  3  an append-only array plus a device hash set of F: membership costs one
+//     or two slot reads however large F is, and F is sorted once at the end.
+// collapse() leaves the whole relation sorted in one array.
 struct FullRel {
     dlg_ctx *ctx;
-    bool tiered;
+    int mode;                       // 1 one array, 2 tiered runs, 3 append + hash set
     std::vector<DBuf<u64>> runs;
     std::vector<u64> sizes;
     u64 total = 0;
-    FullRel(dlg_ctx *c, const Opts &o) : ctx(c), tiered(o.strategy != 1) {}
+    // mode 3: every tuple of F in arrival order, and a hash set of them
+    DBuf<u64> all;
+    HashSet hs;
+    double lf;
+    int bits;
+    const u32 *bounds;
+    FullRel(dlg_ctx *c, const Opts &o, const u32 *b)
+        : ctx(c), mode(o.strategy == 1 ? 1 : (o.strategy == 2 ? 2 : 3)), all(c), hs(c), lf(o.lf),
+          bits(o.bits), bounds(b) {}
 
     void merge_last_two() {
         size_t k = runs.size();
@@ -134,14 +145,27 @@ struct FullRel {
         runs.back() = std::move(m);
         sizes.back() = s;
     }
-    // Takes ownership of buf[0..n) (sorted, disjoint from F).
+    // Takes buf[0..n) (sorted, disjoint from F); buf is left empty.
     void add(DBuf<u64> &buf, u64 n) {
         if (n == 0) return;
+        if (mode == 3) {
+            if (all.cap < total + n) {                       // grow x2, keep the contents
+                DBuf<u64> g(ctx, std::max<u64>(2 * (total + n), 1024));
+                copy_u64(ctx, g.p, all.p, total);
+                all = std::move(g);
+            }
+            copy_u64(ctx, all.p + total, buf.p, n);
+            hashset_reserve(ctx, hs, total + n, lf, all.p, total);
+            hashset_insert(ctx, hs, buf.p, n);
+            total += n;
+            buf.release();
+            return;
+        }
         total += n;
         runs.push_back(std::move(buf));
         sizes.push_back(n);
         buf = DBuf<u64>(ctx);
-        if (!tiered) {
+        if (mode == 1) {
             if (runs.size() == 2) merge_last_two();
             return;
         }
@@ -151,6 +175,7 @@ struct FullRel {
     }
     // N = first of each run of equal tuples in sorted J, minus F.
     u64 new_tuples(const u64 *J, u64 nj, u64 *N) {
+        if (mode == 3) return select_unique_not_in_hash(ctx, J, nj, hs, N);
         if (runs.size() == 1) return select(ctx, SEL_UNIQUE_NOT_IN, J, nj, runs[0].p, sizes[0], N);
         RunSet rs;
         rs.k = (int)runs.size();
@@ -160,7 +185,16 @@ struct FullRel {
         }
         return select_unique_not_in_runs(ctx, J, nj, rs, N);
     }
+    // The whole relation, sorted, moved into `out`.
     void collapse(DBuf<u64> &out) {
+        if (mode == 3) {
+            hs.slots.release();
+            DBuf<u64> tmp(ctx);
+            if (total) radix_sort(ctx, all, total, tmp, bits, bounds);
+            if (!all.p) all.ensure(1);
+            out = std::move(all);
+            return;
+        }
         while (runs.size() >= 2) merge_last_two();
         if (runs.empty()) {
             out.ensure(1);
@@ -181,6 +215,18 @@ static void id_bounds(dlg_ctx *ctx, const u64 *edge, u64 n, u32 *b) {
     u32 lo = std::min(h[0], h[2]), hi = std::max(h[1], h[3]);
     if (n == 0) lo = hi = 0;
     b[0] = lo; b[1] = hi; b[2] = lo; b[3] = hi;
+}
+
+// The bitmap layout of F (strategy 4, or automatic when the id range is
+// narrow enough for a W*W-bit map).
+static bool use_bitmap(const Opts &o, u64 n, const u32 *ib) {
+    if (n == 0) return false;
+    u64 W = (u64)(ib[1] - ib[0]) + 1;
+    if (o.strategy == 4) {
+        if (W > BITMAP_MAX_W * 2) fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
+        return true;
+    }
+    return o.strategy == 0 && W <= BITMAP_MAX_W;
 }
 
 // Sort J and keep its tuples that are new: N = unique(J) \ F.
@@ -211,11 +257,37 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     Index ix(ctx);
     index_build(ctx, ix, edge.p, n, o.lf, false);      // duplicates kept, as the reference
     edge.release();
-    pc.mark(DLG_PH_DEDUP);
-    u64 nd = sort_unique_into(ctx, J, n, tmp, delta, SEL_UNIQUE, nullptr, 0, o.bits, ib);
-    FullRel F(ctx, o);                                  // F = Δ0, added after the first join
     JoinScratch js(ctx);
     int it = 0;
+    if (use_bitmap(o, n, ib)) {
+        // F as a bitmap: each round's join rows go straight into it (k_bitmap.hip).
+        // Rows (z, x) of one probe share x, so x picks the bitmap row (transposed).
+        Bitmap F(ctx);
+        bitmap_init(ctx, F, ib[0], ib[1], true);
+        pc.mark(DLG_PH_DEDUP);
+        u64 nd = bitmap_insert_list(ctx, J.p, n, false, F, delta.p);    // Δ0 = F = unique(E reversed)
+        u64 total = nd, raw = 0;
+        while (true) {
+            pc.mark(DLG_PH_JOIN);
+            u64 nn = bitmap_join_insert(ctx, ix, delta.p, nd, js, DLG_PROJ_HV_PV, false, F, N, &raw);
+            total += nn;
+            it++;
+            if (hook) hook->on_round(ctx, N.p, nn, it, false);
+            if (nn == 0) break;                     // |F| unchanged (tc.cu:294-300)
+            delta.swap(N);
+            nd = nn;
+            if (o.max_it > 0 && it >= o.max_it) break;
+        }
+        pc.mark(DLG_PH_FINAL);
+        out.raw += (i64)raw;
+        out.nfull = total;
+        bitmap_emit(ctx, F, out.full, total, o.bits);
+        out.iterations = it;
+        return;
+    }
+    pc.mark(DLG_PH_DEDUP);
+    u64 nd = sort_unique_into(ctx, J, n, tmp, delta, SEL_UNIQUE, nullptr, 0, o.bits, ib);
+    FullRel F(ctx, o, ib);                              // F = Δ0, added after the first join
     while (true) {
         pc.mark(DLG_PH_JOIN);
         u64 nj = join_into(ctx, ix, delta.p, nd, DLG_PROJ_HV_PV, js, J);   // (z, x)
@@ -224,7 +296,7 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         F.add(delta, nd);
         u64 nn = new_tuples(ctx, J, nj, tmp, N, F, o, pc, ib);
         it++;
-        if (hook) hook->on_round(ctx, N.p, nn, it);
+        if (hook) hook->on_round(ctx, N.p, nn, it, true);
         if (nn == 0) break;                         // |F| unchanged (tc.cu:294-300)
         delta.swap(N);
         nd = nn;
@@ -253,14 +325,47 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     Index ix(ctx);
     index_build(ctx, ix, edge.p, ne, o.lf, true);
     JoinScratch js(ctx);
+    int it = 0;
+    if (use_bitmap(o, ne, ib)) {
+        // F and the round's T as bitmaps: join rows go straight into them (k_bitmap.hip).
+        // The loop's rows (b, x) and (x, y) share their key per probe: row-major.
+        Bitmap F(ctx), R(ctx);
+        bitmap_init(ctx, F, ib[0], ib[1], false);
+        bitmap_init(ctx, R, ib[0], ib[1], false);
+        u64 raw = 0;
+        pc.mark(DLG_PH_JOIN);
+        delta.ensure(1);
+        u64 nd = bitmap_join_insert(ctx, ix, edge.p, ne, js, DLG_PROJ_HV_PV, true, F, delta, &raw);  // base
+        edge.release();
+        u64 total = nd;
+        while (true) {
+            pc.mark(DLG_PH_JOIN);
+            u64 nt = bitmap_join_insert(ctx, ix, delta.p, nd, js, DLG_PROJ_PV_HV, false, R, T, &raw);
+            u64 nn = bitmap_join_insert(ctx, ix, T.p, nt, js, DLG_PROJ_PV_HV, false, F, N, &raw);
+            pc.mark(DLG_PH_CLEAR);
+            bitmap_clear_list(ctx, T.p, nt, R);
+            total += nn;
+            it++;
+            if (hook) hook->on_round(ctx, N.p, nn, it, false);
+            if (nn == 0) break;                     // sg.cu:376-389
+            delta.swap(N);
+            nd = nn;
+            if (o.max_it > 0 && it >= o.max_it) break;
+        }
+        pc.mark(DLG_PH_FINAL);
+        out.raw += (i64)raw;
+        out.nfull = total;
+        bitmap_emit(ctx, F, out.full, total, o.bits);
+        out.iterations = it;
+        return;
+    }
     pc.mark(DLG_PH_JOIN);
     u64 nb = join_into(ctx, ix, edge.p, ne, DLG_PROJ_HV_PV, js, J);   // (y, x)
     out.raw += (i64)nb;
     edge.release();
     pc.mark(DLG_PH_DEDUP);
     u64 nd = sort_unique_into(ctx, J, nb, tmp, delta, SEL_UNIQUE_NODIAG, nullptr, 0, o.bits, ib);
-    FullRel F(ctx, o);                                  // F = base, added after the first join
-    int it = 0;
+    FullRel F(ctx, o, ib);                              // F = base, added after the first join
     while (true) {
         pc.mark(DLG_PH_JOIN);
         u64 n1 = join_into(ctx, ix, delta.p, nd, DLG_PROJ_PV_HV, js, J);  // (b, x)
@@ -274,7 +379,7 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         out.raw += (i64)n2;
         u64 nn = new_tuples(ctx, J, n2, tmp, N, F, o, pc, ib);
         it++;
-        if (hook) hook->on_round(ctx, N.p, nn, it);
+        if (hook) hook->on_round(ctx, N.p, nn, it, true);
         if (nn == 0) break;                         // sg.cu:376-389
         delta.swap(N);
         nd = nn;

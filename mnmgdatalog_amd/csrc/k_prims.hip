@@ -626,6 +626,22 @@ __global__ __launch_bounds__(SEL_T) void select_count_runs_k(const u64 *in, u64 
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
+void scan_blocks(dlg_ctx *ctx, u64 *bsum, u64 nb, u64 *d_total) {
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum, nb, d_total);
+}
+
+u64 select_scatter(dlg_ctx *ctx, const u64 *in, u64 n, const u64 *flags, DBuf<u64> &bsum, DBuf<u64> &tot,
+                   u64 *out) {
+    u64 nb = (n + SEL_TILE - 1) / SEL_TILE;
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb, tot.p);
+    u64 total = 0;
+    ctx->read_scalars(tot.p, 1, &total);
+    if (total == 0) return 0;
+    DLG_LAUNCH(ctx, DLG_K_SELECT, 8.0 * ((n + 63) / 64) + 16.0 * total, select_scatter_k<false>,
+               (unsigned)nb, SEL_T, 0, in, n, flags, (const u64 *)bsum.p, (void *)out);
+    return total;
+}
+
 u64 select_unique_not_in_runs(dlg_ctx *ctx, const u64 *in, u64 n, const RunSet &rs, u64 *out) {
     if (n == 0) return 0;
     if (rs.k == 0) return select(ctx, SEL_UNIQUE, in, n, nullptr, 0, out);

@@ -236,7 +236,7 @@ const char *dlg_kernel_name(int k) {
     static const char *names[DLG_K_COUNT] = {
         "pack", "hash_build", "join_count", "scan", "join_expand", "sort_minmax",
         "sort_upsweep", "sort_downsweep", "select", "merge", "wcc_relax", "wcc_final",
-        "hashset_insert", "misc"};
+        "hashset_insert", "bitmap_insert", "misc"};
     return (k >= 0 && k < DLG_K_COUNT) ? names[k] : "?";
 }
 

@@ -103,7 +103,8 @@ def test_extreme_ids(ctx, orc, prog):
     rng = np.random.default_rng(9)
     rows = ids[rng.integers(0, len(ids), size=(40, 2))]
     want = getattr(orc, prog)(rows)
-    assert_same(ctx.run(prog, rows), want, secondary=prog == "wcc")
+    for strategy in (0, 1, 2, 3):
+        assert_same(ctx.run(prog, rows, strategy=strategy), want, secondary=prog == "wcc")
 
 
 @pytest.mark.parametrize("prog,its,size,second", [("tc", 1, 0, 0), ("sg", 1, 0, 0), ("wcc", 1, 0, -1),
@@ -272,7 +273,7 @@ def test_wcc_graph_captured_max_iterations(ctx):
 def test_full_relation_strategies_agree(ctx, orc, prog, name):
     rows = read_bin(name)
     want = getattr(orc, prog)(rows)
-    for strategy in (1, 2):                          # one merged array, tiered runs
+    for strategy in (1, 2, 3, 4):                    # merged array, tiered runs, hash set, bitmap
         assert_same(ctx.run(prog, rows, strategy=strategy), want)
 
 
@@ -281,5 +282,21 @@ def test_max_iterations_prefix_matches_oracle(ctx, orc, prog):
     rows = read_bin("data_7035.bin")
     for k in (1, 2, 5):
         want = getattr(orc, prog)(rows, max_iterations=k)
-        for strategy in (1, 2):
+        for strategy in (1, 2, 3, 4):
             assert_same(ctx.run(prog, rows, strategy=strategy, max_iterations=k), want)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_hash_strategy_random(ctx, orc, seed):
+    rows = rand_rows(500 + seed, 300, 500, neg=seed % 2 == 0)
+    assert_same(ctx.run("tc", rows, strategy=3, load_factor=0.9), orc.tc(rows))
+    assert_same(ctx.run("sg", rows, strategy=3, load_factor=0.3), orc.sg(rows))
+    assert_same(ctx.run("tc", rows, strategy=4), orc.tc(rows))
+    assert_same(ctx.run("sg", rows, strategy=4), orc.sg(rows))
+
+
+def test_bitmap_strategy_rejects_wide_id_range(ctx):
+    rows = np.array([[0, 1], [1, 2**30]], np.int32)
+    with pytest.raises(dlg.DlgError) as e:
+        ctx.run("tc", rows, strategy=4)
+    assert e.value.status == dlg.DLG_INVALID_INPUT
