@@ -20,6 +20,7 @@ namespace dlg {
 
 typedef uint64_t u64;
 typedef uint32_t u32;
+typedef uint16_t u16;
 typedef int64_t i64;
 
 constexpr int WAVE = 64;          // CDNA wavefront width
@@ -297,14 +298,22 @@ void bitmap_init(dlg_ctx *ctx, Bitmap &bm, u32 lo, u32 hi, bool transposed);
 // (unordered).  Returns their number; *raw += |J|.
 u64 bitmap_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, int proj,
                        bool nodiag, Bitmap &bm, DBuf<u64> &out, u64 *raw);
+// probe ⋈ index, each row inserted; the bits that were clear are also set in
+// `mark` (same geometry), or recorded nowhere when `mark` is null.
+void bitmap_join_mark(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, int proj,
+                      bool nodiag, Bitmap &bm, Bitmap *mark, u64 *raw);
+// All set bits in bit order (as (column, row) tuples when `swap`), then the
+// map cleared.  Returns their number.
+u64 bitmap_drain(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, bool swap);
 // Insert a list; the tuples that were absent go to `out`.
 u64 bitmap_insert_list(dlg_ctx *ctx, const u64 *in, u64 n, bool nodiag, Bitmap &bm, u64 *out);
 // Clear the bits of a list whose bits are the only ones set in their words.
 void bitmap_clear_list(dlg_ctx *ctx, const u64 *in, u64 n, Bitmap &bm);
 // All set bits as sorted tuples; `expect` = the number of set bits.  A
 // transposed map is emitted in (value, key) order and then radix sorted with
-// `bits` bits per pass.
-u64 bitmap_emit(dlg_ctx *ctx, const Bitmap &bm, DBuf<u64> &out, u64 expect, int bits);
+// `bits` bits per pass -- unless it holds a symmetric relation, which equals
+// its transpose and is emitted as it lies.
+u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, int bits, bool symmetric);
 
 // ------------------------------------------------------------- hash set (k_hashset.hip)
 struct HashSet {

@@ -223,7 +223,7 @@ static bool use_bitmap(const Opts &o, u64 n, const u32 *ib) {
     if (n == 0) return false;
     u64 W = (u64)(ib[1] - ib[0]) + 1;
     if (o.strategy == 4) {
-        if (W > BITMAP_MAX_W * 2) fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
+        if (W > BITMAP_FORCED_MAX_W) fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
         return true;
     }
     return o.strategy == 0 && W <= BITMAP_MAX_W;
@@ -281,7 +281,7 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         pc.mark(DLG_PH_FINAL);
         out.raw += (i64)raw;
         out.nfull = total;
-        bitmap_emit(ctx, F, out.full, total, o.bits);
+        bitmap_emit(ctx, F, out.full, total, o.bits, false);
         out.iterations = it;
         return;
     }
@@ -327,23 +327,30 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     JoinScratch js(ctx);
     int it = 0;
     if (use_bitmap(o, ne, ib)) {
-        // F and the round's T as bitmaps: join rows go straight into them (k_bitmap.hip).
-        // The loop's rows (b, x) and (x, y) share their key per probe: row-major.
-        Bitmap F(ctx), R(ctx);
-        bitmap_init(ctx, F, ib[0], ib[1], false);
+        // F, the round's T and the round's N as bitmaps: join rows go straight
+        // into them (k_bitmap.hip), and T and N come out of their maps in bit
+        // order, which keeps each join's writes local:
+        //   Δ (a, b) in b order   -> T rows (b, x) of one b share R's row b;
+        //   T (b, x) in (b, x) order -> N rows (x, y) of one b share F's rows y
+        //   (F and D transposed), at neighbouring columns x.
+        // SG is symmetric (both rules are), so F equals its transpose and is
+        // emitted without a sort.
+        Bitmap F(ctx), R(ctx), D(ctx);
+        bitmap_init(ctx, F, ib[0], ib[1], true);
         bitmap_init(ctx, R, ib[0], ib[1], false);
+        bitmap_init(ctx, D, ib[0], ib[1], true);
         u64 raw = 0;
         pc.mark(DLG_PH_JOIN);
-        delta.ensure(1);
-        u64 nd = bitmap_join_insert(ctx, ix, edge.p, ne, js, DLG_PROJ_HV_PV, true, F, delta, &raw);  // base
+        bitmap_join_mark(ctx, ix, edge.p, ne, js, DLG_PROJ_HV_PV, true, F, &D, &raw);   // base (y, x)
         edge.release();
+        u64 nd = bitmap_drain(ctx, D, delta, true);
         u64 total = nd;
         while (true) {
             pc.mark(DLG_PH_JOIN);
-            u64 nt = bitmap_join_insert(ctx, ix, delta.p, nd, js, DLG_PROJ_PV_HV, false, R, T, &raw);
-            u64 nn = bitmap_join_insert(ctx, ix, T.p, nt, js, DLG_PROJ_PV_HV, false, F, N, &raw);
-            pc.mark(DLG_PH_CLEAR);
-            bitmap_clear_list(ctx, T.p, nt, R);
+            bitmap_join_mark(ctx, ix, delta.p, nd, js, DLG_PROJ_PV_HV, false, R, nullptr, &raw);  // (b, x)
+            u64 nt = bitmap_drain(ctx, R, T, false);
+            bitmap_join_mark(ctx, ix, T.p, nt, js, DLG_PROJ_PV_HV, false, F, &D, &raw);     // (x, y)
+            u64 nn = bitmap_drain(ctx, D, N, true);
             total += nn;
             it++;
             if (hook) hook->on_round(ctx, N.p, nn, it, false);
@@ -355,7 +362,7 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         pc.mark(DLG_PH_FINAL);
         out.raw += (i64)raw;
         out.nfull = total;
-        bitmap_emit(ctx, F, out.full, total, o.bits);
+        bitmap_emit(ctx, F, out.full, total, o.bits, true);
         out.iterations = it;
         return;
     }

@@ -49,56 +49,57 @@ __device__ __forceinline__ void wave_append(bool app, u64 t, u64 *out, unsigned 
     if (app) out[b + (u64)__popcll(bm & lanemask_lt())] = t;
 }
 
-// Whole-wave insert: lane l offers bit `mask` (0 = nothing) of word `wi`.
-// Runs of lanes with the same word combine their bits (segmented inclusive
-// OR scan), the run's last lane issues the atomicOr, and a lane's tuple is
-// new when its bit was clear before that atomic and no earlier lane of the
-// run offered the same bit.  Every lane of the wave must call it.
-__device__ __forceinline__ bool wave_insert(u32 *words, u64 wi, u32 mask) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    u32 wlo = (u32)wi, whi = (u32)(wi >> 32);
-    u32 plo = __shfl_up(wlo, 1, WAVE), phi = __shfl_up(whi, 1, WAVE);
-    bool head = lane == 0 || plo != wlo || phi != whi;
-    u32 v = mask;
-    int f = head ? 1 : 0;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        u32 vv = __shfl_up(v, d, WAVE);
-        int ff = __shfl_up(f, d, WAVE);
-        if (lane >= d && !f) {
-            v |= vv;
-            f = ff;
-        }
-    }
-    u32 excl = __shfl_up(v, 1, WAVE);
-    if (head) excl = 0;
-    u64 heads = __ballot(head);
-    u64 tails = (heads >> 1) | (1ull << (WAVE - 1));
-    int tail = lane + __ffsll((unsigned long long)(tails >> lane)) - 1;
-    u32 old = 0;
-    if (lane == tail && v) old = atomicOr(words + wi, v);
-    old = __shfl(old, tail, WAVE);
-    return mask && !(old & mask) && !(excl & mask);
-}
-
 constexpr int BJ_T = 256, BJ_I = 8, BJ_TILE = BJ_T * BJ_I, BJ_LDS = BJ_TILE + 1;
-constexpr int BJ_BUF = BJ_TILE + 1024;   // new tuples staged per block; flushed above 1024
+constexpr int BH = 2 * BJ_TILE;         // LDS hash slots per tile (load <= 1/2)
+constexpr u32 BH_EMPTY = 0xffffffffu;   // word indexes stay below it (nwords < 2^32 - 1)
+
+// What the insert does with the rows that set a clear bit.
+enum BmMode {
+    BM_APPEND = 0,   // append them to `out` (unordered)
+    BM_MARK = 1,     // set the same bits in `mark` (a map of the same geometry)
+    BM_SET = 2,      // nothing: only the bits are wanted
+};
+
+// LDS layout of the join-insert kernel: the probe staging of a tile
+// (offsets, starts, probe values, row owners) and, once the tile's rows are
+// located, the tile's word hash table (keys, bits) in the same space.
+constexpr int LB_OFFS = 0, LB_START = 2052, LB_PV = 4104, LB_OWNER = 6156, LB_WORDS = LB_OWNER + BJ_TILE;
+constexpr int LB_HKEY = 0, LB_HBITS = BH;
+static_assert(LB_START >= BJ_LDS && LB_PV - LB_START >= BJ_LDS && LB_OWNER - LB_PV >= BJ_LDS, "staging overlap");
+static_assert(LB_OWNER % 4 == 0 && LB_HBITS + BH <= LB_WORDS, "hash table must fit the staging space");
+
+__device__ __forceinline__ u32 bh_hash(u32 w) { return (w * 0x9e3779b1u) >> (32 - 12); }   // BH = 2^12
+static_assert(BH == 1 << 12, "bh_hash assumes 4096 slots");
 
 // Join materialize fused with the bitmap insert.  Persistent blocks walk
-// tiles of BJ_TILE join rows (probe location as in join_expand_k: the tile's
-// probes staged in LDS when they fit, a binary search per row).
-template <int PROJ, bool NODIAG, bool TR>
+// tiles of BJ_TILE join rows.  Per tile:
+//   1 locate each row's probe: the tile's probes staged in LDS with a
+//     row -> probe owner map (a binary search in global memory when the
+//     tile spans more probes than fit);
+//   2 gather the matched values, form the tuples and their bitmap words;
+//   3 OR the tile's bits together per word in an LDS hash table, so a word
+//     that many rows of the tile hit costs one global read and at most one
+//     atomicOr (device-scope atomics execute at the memory side);
+//   4 per distinct word: a plain read; the bits not yet set go in with one
+//     atomicOr, whose result tells which of them this tile set;
+//   5 MARK: those bits are ORed into `mark`; APPEND: each such bit is
+//     claimed by the first of the tile's rows that carries it, which appends
+//     its tuple through an LDS buffer: one counter atomic per tile.
+template <int PROJ, bool NODIAG, bool TR, int MODE>
 __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 np, const u32 *pstart,
                                                          const u64 *offs, u64 total, const u64 *rows,
                                                          u32 *words, u32 lo, u64 W, u64 *out,
-                                                         unsigned long long *counter) {
+                                                         unsigned long long *counter, u32 *mark) {
     __shared__ u64 s_lo, s_hi;
     __shared__ unsigned long long s_base;
-    __shared__ u32 s_cnt;
-    __shared__ u32 l_offs[BJ_LDS];    // probe offsets relative to the tile base (the first: 0)
-    __shared__ u32 l_start[BJ_LDS];   // row of the probe's match at l_offs (the first: adjusted)
-    __shared__ u32 l_pv[BJ_LDS];
-    __shared__ u64 buf[BJ_BUF];
+    __shared__ u32 s_cnt, s_nused;
+    __shared__ __align__(16) u32 lds[LB_WORDS];
+    __shared__ u16 h_used[BJ_TILE];   // the tile's occupied slots
+    __shared__ u32 s_wmax[BJ_T / WAVE];
+    __shared__ u64 buf[MODE == BM_APPEND ? BJ_TILE : 1];   // the tile's new tuples
+    u32 *l_offs = lds + LB_OFFS, *l_start = lds + LB_START, *l_pv = lds + LB_PV, *owner = lds + LB_OWNER;
+    u32 *h_key = lds + LB_HKEY, *h_bits = lds + LB_HBITS;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const u64 ntiles = (total + BJ_TILE - 1) / BJ_TILE;
     if (threadIdx.x == 0) s_cnt = 0;
     for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -109,11 +110,12 @@ __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 n
             u64 l = upper_bound_u64(offs, 0, np, base) - 1;
             s_lo = l;
             s_hi = upper_bound_u64(offs, l, np, last);
+            s_nused = 0;
         }
         __syncthreads();
         const u64 plo = s_lo, phi = s_hi, span = phi - plo;
-        const bool lds = span <= (u64)BJ_LDS;
-        if (lds) {
+        const bool staged = span <= (u64)BJ_LDS;
+        if (staged) {
             for (u64 j = threadIdx.x; j < span; j += BJ_T) {
                 u64 of = offs[plo + j];
                 u32 st = pstart[plo + j];
@@ -125,50 +127,150 @@ __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 n
                 l_start[j] = st;
                 l_pv[j] = tval(probe[plo + j]);
             }
+            __syncthreads();
+            // owner[rel] = the last probe starting at or before rel: mark each
+            // probe's first row, then a block-wide running max fills forward.
+            // Probes without rows share their start with the next probe and
+            // are not marked.
+            for (int i = threadIdx.x; i < BJ_TILE; i += BJ_T) owner[i] = 0;
+            __syncthreads();
+            for (u64 j = threadIdx.x; j < span; j += BJ_T)
+                if (j + 1 == span || l_offs[j + 1] != l_offs[j]) owner[l_offs[j]] = (u32)j;
+            __syncthreads();
+            uint4 o0 = *(const uint4 *)&owner[threadIdx.x * BJ_I];
+            uint4 o1 = *(const uint4 *)&owner[threadIdx.x * BJ_I + 4];
+            u32 loc[BJ_I] = {o0.x, o0.y, o0.z, o0.w, o1.x, o1.y, o1.z, o1.w};
+#pragma unroll
+            for (int i = 1; i < BJ_I; i++) loc[i] = max(loc[i], loc[i - 1]);
+            u32 inc = loc[BJ_I - 1];
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                u32 x = __shfl_up(inc, d, WAVE);
+                if (lane >= d) inc = max(inc, x);
+            }
+            if (lane == WAVE - 1) s_wmax[wv] = inc;
+            u32 pre = __shfl_up(inc, 1, WAVE);
+            if (lane == 0) pre = 0;
+            __syncthreads();
+            for (int w = 0; w < wv; w++) pre = max(pre, s_wmax[w]);
+#pragma unroll
+            for (int i = 0; i < BJ_I; i++) loc[i] = max(loc[i], pre);
+            *(uint4 *)&owner[threadIdx.x * BJ_I] = make_uint4(loc[0], loc[1], loc[2], loc[3]);
+            *(uint4 *)&owner[threadIdx.x * BJ_I + 4] = make_uint4(loc[4], loc[5], loc[6], loc[7]);
+            __syncthreads();
         }
-        __syncthreads();
-        u64 t[BJ_I], wi[BJ_I];
-        u32 mask[BJ_I];
+        // 1-2: locate, gather, form tuples and words
+        u32 ridx[BJ_I], pvs[BJ_I];
 #pragma unroll
         for (int r = 0; r < BJ_I; r++) {
             u64 o = base + (u64)r * BJ_T + threadIdx.x;
-            mask[r] = 0;
-            wi[r] = ~0ull;
-            t[r] = 0;
+            ridx[r] = 0;
+            pvs[r] = 0;
             if (o < total) {
-                u32 hv, pv;
-                if (lds) {
-                    u32 rel = (u32)(o - base), a = 0, b = (u32)span;
-                    while (b - a > 1) {
-                        u32 m = (a + b) >> 1;
-                        if (l_offs[m] <= rel) a = m; else b = m;
-                    }
-                    hv = tval(rows[(u64)l_start[a] + (rel - l_offs[a])]);
-                    pv = l_pv[a];
+                if (staged) {
+                    u32 rel = (u32)(o - base), a = owner[rel];
+                    ridx[r] = l_start[a] + (rel - l_offs[a]);
+                    pvs[r] = l_pv[a];
                 } else {
                     u64 i = upper_bound_u64(offs, plo, phi, o) - 1;
-                    hv = tval(rows[(u64)pstart[i] + (o - offs[i])]);
-                    pv = tval(probe[i]);
+                    ridx[r] = pstart[i] + (u32)(o - offs[i]);
+                    pvs[r] = tval(probe[i]);
                 }
-                t[r] = PROJ == DLG_PROJ_HV_PV ? tmake(hv, pv) : tmake(pv, hv);
-                if (!NODIAG || hv != pv) {
-                    u64 b = bm_bit(t[r], lo, W, TR);
-                    wi[r] = b >> 5;
-                    mask[r] = 1u << (b & 31);
+            }
+        }
+        u32 hvs[BJ_I];
+#pragma unroll
+        for (int r = 0; r < BJ_I; r++) {
+            u64 o = base + (u64)r * BJ_T + threadIdx.x;
+            hvs[r] = o < total ? tval(rows[ridx[r]]) : 0u;
+        }
+        u64 t[BJ_I];
+        u32 wi[BJ_I], mask[BJ_I];
+#pragma unroll
+        for (int r = 0; r < BJ_I; r++) {
+            u64 o = base + (u64)r * BJ_T + threadIdx.x;
+            u32 hv = hvs[r], pv = pvs[r];
+            t[r] = PROJ == DLG_PROJ_HV_PV ? tmake(hv, pv) : tmake(pv, hv);
+            mask[r] = 0;
+            wi[r] = 0;
+            if (o < total && (!NODIAG || hv != pv)) {
+                u64 b = bm_bit(t[r], lo, W, TR);
+                wi[r] = (u32)(b >> 5);
+                mask[r] = 1u << (b & 31);
+            }
+        }
+        __syncthreads();   // the staging space becomes the hash table
+        for (int i = threadIdx.x * 4; i < BH; i += BJ_T * 4) {
+            *(uint4 *)&h_key[i] = make_uint4(BH_EMPTY, BH_EMPTY, BH_EMPTY, BH_EMPTY);
+            *(uint4 *)&h_bits[i] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        __syncthreads();
+        // 3: per-word OR in LDS
+        u32 slot[BJ_I];
+#pragma unroll
+        for (int r = 0; r < BJ_I; r++) {
+            slot[r] = 0;
+            if (!mask[r]) continue;
+            u32 h = bh_hash(wi[r]);
+            while (true) {
+                u32 k = atomicCAS(&h_key[h], BH_EMPTY, wi[r]);
+                if (k == BH_EMPTY) {
+                    h_used[atomicAdd(&s_nused, 1u)] = (u16)h;
+                    break;
                 }
+                if (k == wi[r]) break;
+                h = (h + 1) & (BH - 1);
+            }
+            atomicOr(&h_bits[h], mask[r]);
+            slot[r] = h;
+        }
+        __syncthreads();
+        // 4: one read and at most one atomic per distinct word
+        const u32 nused = s_nused;
+        u32 sh[BJ_I], sk[BJ_I], sb[BJ_I];
+#pragma unroll
+        for (int i = 0; i < BJ_I; i++) {
+            u32 j = threadIdx.x + i * BJ_T;
+            sb[i] = 0;
+            sk[i] = 0;
+            sh[i] = 0;
+            if (j < nused) {
+                sh[i] = h_used[j];
+                sk[i] = h_key[sh[i]];
+                sb[i] = h_bits[sh[i]];
             }
         }
         u32 cur[BJ_I];
 #pragma unroll
-        for (int r = 0; r < BJ_I; r++) cur[r] = mask[r] ? words[wi[r]] : 0u;   // all reads in flight
+        for (int i = 0; i < BJ_I; i++) cur[i] = sb[i] ? words[sk[i]] : 0u;
+        u32 nb[BJ_I];
+#pragma unroll
+        for (int i = 0; i < BJ_I; i++) {
+            nb[i] = sb[i] & ~cur[i];
+            if (nb[i]) {
+                if (MODE == BM_SET) atomicOr(words + sk[i], nb[i]);
+                else cur[i] = atomicOr(words + sk[i], nb[i]);
+            }
+        }
+        if (MODE == BM_SET) continue;
+#pragma unroll
+        for (int i = 0; i < BJ_I; i++) {
+            if (nb[i]) nb[i] &= ~cur[i];   // the bits this tile set
+            if (MODE == BM_MARK) {
+                if (nb[i]) atomicOr(mark + sk[i], nb[i]);
+            } else if (sb[i]) {
+                h_bits[sh[i]] = nb[i];
+            }
+        }
+        if (MODE == BM_MARK) continue;
+        __syncthreads();
+        // 5: each set bit goes to the first row of the tile that claims it
 #pragma unroll
         for (int r = 0; r < BJ_I; r++) {
-            u32 m = (cur[r] & mask[r]) ? 0u : mask[r];   // bits already in F drop out
             bool app = false;
-            if (__ballot(m != 0)) app = wave_insert(words, wi[r], m);
+            if (mask[r] && (h_bits[slot[r]] & mask[r])) app = (atomicAnd(&h_bits[slot[r]], ~mask[r]) & mask[r]) != 0;
             u64 am = __ballot(app);
             if (am) {
-                const int lane = threadIdx.x & (WAVE - 1);
                 int leader = __ffsll((unsigned long long)am) - 1;
                 u32 p = 0;
                 if (lane == leader) p = atomicAdd(&s_cnt, (u32)__popcll(am));
@@ -178,20 +280,13 @@ __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 n
         }
         __syncthreads();
         const u32 c = s_cnt;
-        if (c > (u32)(BJ_BUF - BJ_TILE)) {
+        if (c) {
             if (threadIdx.x == 0) s_base = atomicAdd(counter, (unsigned long long)c);
             __syncthreads();
             for (u32 j = threadIdx.x; j < c; j += BJ_T) out[s_base + j] = buf[j];
             __syncthreads();
             if (threadIdx.x == 0) s_cnt = 0;
         }
-    }
-    __syncthreads();
-    const u32 c = s_cnt;
-    if (c) {
-        if (threadIdx.x == 0) s_base = atomicAdd(counter, (unsigned long long)c);
-        __syncthreads();
-        for (u32 j = threadIdx.x; j < c; j += BJ_T) out[s_base + j] = buf[j];
     }
 }
 
@@ -220,33 +315,44 @@ __global__ void bm_clear_list_k(const u64 *in, u64 n, u32 *words, u32 lo, u64 W)
 }
 
 // Emission: per block, 256 threads x 8 words; popcounts -> scan -> write.
+// The word buffer is padded to whole blocks (zeros), so the passes read
+// without bounds checks, two 16-byte loads per thread.
 constexpr int BE_T = 256, BE_W = 8, BE_TILE = BE_T * BE_W;
 
-__global__ __launch_bounds__(BE_T) void bm_count_k(const u32 *words, u64 nw, u64 *bsum) {
+__device__ __forceinline__ void load8(const u32 *words, u64 base, u32 *w) {
+    uint4 a = *(const uint4 *)(words + base), b = *(const uint4 *)(words + base + 4);
+    w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+    w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+}
+
+__global__ __launch_bounds__(BE_T) void bm_count_k(const u32 *words, u64 *bsum) {
     __shared__ u64 sc[BE_T / WAVE + 1];
     u64 base = (u64)blockIdx.x * BE_TILE + (u64)threadIdx.x * BE_W;
+    u32 w[BE_W];
+    load8(words, base, w);
     u64 c = 0;
 #pragma unroll
-    for (int i = 0; i < BE_W; i++)
-        if (base + i < nw) c += __popc(words[base + i]);
+    for (int i = 0; i < BE_W; i++) c += __popc(w[i]);
     u64 tot;
     block_exclusive_scan<u64>(c, sc, &tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-template <bool TR>
-__global__ __launch_bounds__(BE_T) void bm_emit_k(const u32 *words, u64 nw, const u64 *bpre, u32 lo, u64 W,
-                                                  u64 *out) {
+// SWAP: write (column, row) instead of (row, column).  CLEAR: zero the words read.
+template <bool SWAP, bool CLEAR>
+__global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u32 lo, u64 W, u64 *out) {
     __shared__ u64 sc[BE_T / WAVE + 1];
     u64 base = (u64)blockIdx.x * BE_TILE + (u64)threadIdx.x * BE_W;
     u32 w[BE_W];
+    load8(words, base, w);
     u64 c = 0;
 #pragma unroll
-    for (int i = 0; i < BE_W; i++) {
-        w[i] = base + i < nw ? words[base + i] : 0u;
-        c += __popc(w[i]);
-    }
+    for (int i = 0; i < BE_W; i++) c += __popc(w[i]);
     u64 pos = block_exclusive_scan<u64>(c, sc, nullptr) + bpre[blockIdx.x];
+    if (CLEAR && c) {
+        *(uint4 *)(words + base) = make_uint4(0u, 0u, 0u, 0u);
+        *(uint4 *)(words + base + 4) = make_uint4(0u, 0u, 0u, 0u);
+    }
 #pragma unroll
     for (int i = 0; i < BE_W; i++) {
         u32 x = w[i];
@@ -263,7 +369,7 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(const u32 *words, u64 nw, cons
                 v -= W;
                 k++;
             }
-            out[pos++] = TR ? tmake(lo + (u32)v, lo + (u32)k) : tmake(lo + (u32)k, lo + (u32)v);
+            out[pos++] = SWAP ? tmake(lo + (u32)v, lo + (u32)k) : tmake(lo + (u32)k, lo + (u32)v);
         }
     }
 }
@@ -273,9 +379,11 @@ void bitmap_init(dlg_ctx *ctx, Bitmap &bm, u32 lo, u32 hi, bool transposed) {
     bm.W = (u64)(hi - lo) + 1;
     bm.nbits = bm.W * bm.W;
     bm.nwords = (bm.nbits + 31) / 32;
+    if (bm.nwords >= (u64)BH_EMPTY) fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
     bm.tr = transposed;
-    bm.words.ensure(bm.nwords);
-    fill_u32(ctx, bm.words.p, bm.nwords, 0u);
+    u64 padded = (bm.nwords + BE_TILE - 1) / BE_TILE * BE_TILE;
+    bm.words.ensure(padded);
+    fill_u32(ctx, bm.words.p, padded, 0u);
     bm.count.ensure(1);
 }
 
@@ -285,22 +393,34 @@ static u64 read_counter(dlg_ctx *ctx, const Bitmap &bm) {
     return h;
 }
 
-template <int PROJ, bool NODIAG>
+template <int PROJ, bool NODIAG, int MODE>
 static void launch_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, u64 total,
-                               Bitmap &bm, DBuf<u64> &out) {
-    // per row: the matched row's value + a bitmap word; per new row an atomic and a write
+                               Bitmap &bm, u64 *out, u32 *mark) {
+    // per row: the matched row's value + a bitmap word; per new row an atomic (+ a write)
     double bytes = 12.0 * total;
     u64 ntiles = (total + BJ_TILE - 1) / BJ_TILE;
     unsigned g = (unsigned)std::min<u64>(ntiles, (u64)ctx->num_cus * 4);
     unsigned long long *cnt = (unsigned long long *)bm.count.p;
     if (bm.tr)
-        DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, bytes, (bm_join_insert_k<PROJ, NODIAG, true>), g, BJ_T, 0, probe, n,
-                   (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total, (const u64 *)ix.rows.p, bm.words.p,
-                   bm.lo, bm.W, out.p, cnt);
+        DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, bytes, (bm_join_insert_k<PROJ, NODIAG, true, MODE>), g, BJ_T, 0, probe,
+                   n, (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total, (const u64 *)ix.rows.p, bm.words.p,
+                   bm.lo, bm.W, out, cnt, mark);
     else
-        DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, bytes, (bm_join_insert_k<PROJ, NODIAG, false>), g, BJ_T, 0, probe, n,
-                   (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total, (const u64 *)ix.rows.p, bm.words.p,
-                   bm.lo, bm.W, out.p, cnt);
+        DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, bytes, (bm_join_insert_k<PROJ, NODIAG, false, MODE>), g, BJ_T, 0,
+                   probe, n, (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total, (const u64 *)ix.rows.p,
+                   bm.words.p, bm.lo, bm.W, out, cnt, mark);
+}
+
+template <int MODE>
+static void join_insert_mode(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, u64 total,
+                             int proj, bool nodiag, Bitmap &bm, u64 *out, u32 *mark) {
+    if (proj == DLG_PROJ_PV_HV) {
+        if (nodiag) launch_join_insert<DLG_PROJ_PV_HV, true, MODE>(ctx, ix, probe, n, js, total, bm, out, mark);
+        else launch_join_insert<DLG_PROJ_PV_HV, false, MODE>(ctx, ix, probe, n, js, total, bm, out, mark);
+    } else {
+        if (nodiag) launch_join_insert<DLG_PROJ_HV_PV, true, MODE>(ctx, ix, probe, n, js, total, bm, out, mark);
+        else launch_join_insert<DLG_PROJ_HV_PV, false, MODE>(ctx, ix, probe, n, js, total, bm, out, mark);
+    }
 }
 
 u64 bitmap_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, int proj,
@@ -310,14 +430,22 @@ u64 bitmap_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, J
     if (total == 0) return 0;
     out.ensure(std::min<u64>(total, bm.nbits));
     fill_u64(ctx, bm.count.p, 1, 0ull);
-    if (proj == DLG_PROJ_PV_HV) {
-        if (nodiag) launch_join_insert<DLG_PROJ_PV_HV, true>(ctx, ix, probe, n, js, total, bm, out);
-        else launch_join_insert<DLG_PROJ_PV_HV, false>(ctx, ix, probe, n, js, total, bm, out);
-    } else {
-        if (nodiag) launch_join_insert<DLG_PROJ_HV_PV, true>(ctx, ix, probe, n, js, total, bm, out);
-        else launch_join_insert<DLG_PROJ_HV_PV, false>(ctx, ix, probe, n, js, total, bm, out);
-    }
+    join_insert_mode<BM_APPEND>(ctx, ix, probe, n, js, total, proj, nodiag, bm, out.p, nullptr);
     return read_counter(ctx, bm);
+}
+
+void bitmap_join_mark(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, int proj,
+                      bool nodiag, Bitmap &bm, Bitmap *mark, u64 *raw) {
+    u64 total = join_count(ctx, ix, probe, n, js);
+    if (raw) *raw += total;
+    if (total == 0) return;
+    if (mark) {
+        if (mark->W != bm.W || mark->lo != bm.lo || mark->tr != bm.tr)
+            fail(DLG_INVALID_INPUT, "bitmap_join_mark: maps of different geometry");
+        join_insert_mode<BM_MARK>(ctx, ix, probe, n, js, total, proj, nodiag, bm, nullptr, mark->words.p);
+    } else {
+        join_insert_mode<BM_SET>(ctx, ix, probe, n, js, total, proj, nodiag, bm, nullptr, nullptr);
+    }
 }
 
 u64 bitmap_insert_list(dlg_ctx *ctx, const u64 *in, u64 n, bool nodiag, Bitmap &bm, u64 *out) {
@@ -354,24 +482,47 @@ void bitmap_clear_list(dlg_ctx *ctx, const u64 *in, u64 n, Bitmap &bm) {
                    bm.words.p, bm.lo, bm.W);
 }
 
-u64 bitmap_emit(dlg_ctx *ctx, const Bitmap &bm, DBuf<u64> &out, u64 expect, int bits) {
-    out.ensure(expect ? expect : 1);
-    if (expect == 0) return 0;
-    u64 nb = (bm.nwords + BE_TILE - 1) / BE_TILE;
-    DBuf<u64> bsum(ctx, nb), tot(ctx, 1);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * bm.nwords, bm_count_k, (unsigned)nb, BE_T, 0, (const u32 *)bm.words.p,
-               bm.nwords, bsum.p);
+// Count pass + block scan; returns the number of set bits (synchronises).
+static u64 bitmap_count(dlg_ctx *ctx, const Bitmap &bm, DBuf<u64> &bsum, u64 &nb) {
+    nb = (bm.nwords + BE_TILE - 1) / BE_TILE;
+    bsum.ensure(nb);
+    DBuf<u64> tot(ctx, 1);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * nb * BE_TILE, bm_count_k, (unsigned)nb, BE_T, 0, (const u32 *)bm.words.p,
+               bsum.p);
     scan_blocks(ctx, bsum.p, nb, tot.p);
-    if (bm.tr)
-        DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * bm.nwords + 8.0 * expect, bm_emit_k<true>, (unsigned)nb, BE_T, 0,
-                   (const u32 *)bm.words.p, bm.nwords, (const u64 *)bsum.p, bm.lo, bm.W, out.p);
-    else
-        DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * bm.nwords + 8.0 * expect, bm_emit_k<false>, (unsigned)nb, BE_T, 0,
-                   (const u32 *)bm.words.p, bm.nwords, (const u64 *)bsum.p, bm.lo, bm.W, out.p);
     u64 h = 0;
     ctx->read_scalars(tot.p, 1, &h);
+    return h;
+}
+
+template <bool SWAP, bool CLEAR>
+static void emit_pass(dlg_ctx *ctx, Bitmap &bm, const DBuf<u64> &bsum, u64 nb, u64 h, u64 *out) {
+    DLG_LAUNCH(ctx, DLG_K_MISC, (CLEAR ? 8.0 : 4.0) * nb * BE_TILE + 8.0 * h, (bm_emit_k<SWAP, CLEAR>),
+               (unsigned)nb, BE_T, 0, bm.words.p, (const u64 *)bsum.p, bm.lo, bm.W, out);
+}
+
+u64 bitmap_drain(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, bool swap) {
+    DBuf<u64> bsum(ctx);
+    u64 nb = 0;
+    u64 h = bitmap_count(ctx, bm, bsum, nb);
+    out.ensure(h ? h : 1);
+    if (h == 0) return 0;
+    if (swap) emit_pass<true, true>(ctx, bm, bsum, nb, h, out.p);
+    else emit_pass<false, true>(ctx, bm, bsum, nb, h, out.p);
+    return h;
+}
+
+u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, int bits, bool symmetric) {
+    out.ensure(expect ? expect : 1);
+    if (expect == 0) return 0;
+    DBuf<u64> bsum(ctx);
+    u64 nb = 0;
+    u64 h = bitmap_count(ctx, bm, bsum, nb);
     if (h != expect) fail(DLG_DEVICE_ERROR, "bitmap emission count mismatch");
-    if (bm.tr) {   // emitted in (value, key) order
+    bool swap = bm.tr && !symmetric;
+    if (swap) emit_pass<true, false>(ctx, bm, bsum, nb, h, out.p);
+    else emit_pass<false, false>(ctx, bm, bsum, nb, h, out.p);
+    if (swap) {   // emitted in (value, key) order
         u32 hi = bm.lo + (u32)(bm.W - 1);
         u32 bounds[4] = {bm.lo, hi, bm.lo, hi};
         DBuf<u64> tmp(ctx);
