@@ -1,15 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark: derived tuples/s of the semi-naive fixpoint engine on one MI355X.
 
-Workload (default ``sg_mesh``): same generation on the synthetic 3D finite-
-element mesh -- 35x35x35 hex cells, each split into 5 tetrahedra with
-parity-alternating orientation, tetrahedron edges directed low -> high id
-(268,380 int32 edges).  It is the largest of the benchmark configurations that
-fits one GPU: the 400x400 grid's transitive closure holds >= 2.0e10 tuples
-(giant component of ~141k vertices), beyond one device and beyond the
-reference's int32 counters.  Other workloads: ``tc_powerlaw`` (3,000 vertices,
+Workload (default ``tc_grid``): transitive closure of the road-network-shaped
+grid -- 400x400 vertices, each undirected lattice edge kept with p = 0.55 and
+stored in both directions, plus shortcut edges (1% of the kept lattice edges)
+between vertices within Manhattan distance 20 (354,870 int32 rows).  Its
+closure holds 20,691,180,264 tuples (501 iterations): the full relation is
+kept as a 160,000 x 160,000-bit map (3.2 GB) and the sorted result (166 GB)
+stays resident in HBM.  Other workloads: ``sg_mesh`` (same generation on the
+35x35x35 tetrahedral mesh, 268,380 edges), ``tc_powerlaw`` (3,000 vertices,
 7,000 edges), ``tc_usroad`` (the reference's usroad graph, 606 iterations),
-``wcc_grid`` (WCC on the 400x400 grid), ``single_join`` (10M rows, range 1M).
+``wcc_grid`` (WCC on the 400x400 grid), ``single_join`` (10M rows, range 1M),
+``batched`` and ``served``.
 
 One step = one whole fixpoint (input rows already resident in HBM, output
 relation resident in HBM at the end).  value = |output relation| * steps / time.
@@ -39,6 +41,9 @@ MEASURED_TRAFFIC = {}
 
 def workload_rows(name):
     import mnmgdatalog_amd as dlg
+    if name == "tc_grid":
+        return "tc", dlg.gen_grid(400, 0.55, 0.01, 20, SEED), {
+            "workload": "tc_grid_400x400_p0.55_shortcuts1pct_d20", "program": "tc"}
     if name == "sg_mesh":
         return "sg", dlg.gen_tetmesh(35), {"workload": "sg_tetmesh_35", "cells": 35, "edges": None,
                                            "program": "sg"}
@@ -57,7 +62,7 @@ def workload_rows(name):
 
 
 # bounded CPU sample per workload: number of fixpoint iterations the oracle runs
-CPU_SAMPLE_ITERS = {"sg_mesh": 3, "tc_powerlaw": 0, "tc_usroad": 40, "wcc_grid": 0, "single_join": 0}
+CPU_SAMPLE_ITERS = {"tc_grid": 20, "sg_mesh": 3, "tc_powerlaw": 0, "tc_usroad": 40, "wcc_grid": 0, "single_join": 0}
 
 
 def cpu_baseline(name, prog, rows, ctx, threads, strategy=0):
@@ -257,12 +262,41 @@ def bench_served(args):
     }
 
 
+def bench_copy(args):
+    """HBM copy reference: the unpack kernel over 2^27 tuples (1 GiB read + 1 GiB
+    written per launch), timed with HIP events; also the calibration run for
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (known bytes per dispatch)."""
+    import ctypes
+
+    import mnmgdatalog_amd as dlg
+    n = 1 << 27
+    with dlg.Ctx(0) as c:
+        src, dst = c.alloc(8 * n), c.alloc(8 * n)
+        for _ in range(args.warmup):
+            c.check(c.L.dlg_unpack_rows(c.h, ctypes.c_void_p(src), n, 0, ctypes.c_void_p(dst)), "unpack")
+        c.kernel_stats_reset()
+        c.profiling(True)
+        for _ in range(args.steps):
+            c.check(c.L.dlg_unpack_rows(c.h, ctypes.c_void_p(src), n, 0, ctypes.c_void_p(dst)), "unpack")
+        c.check(c.L.dlg_ctx_sync(c.h), "sync")
+        c.profiling(False)
+        st = c.kernel_stats()["pack"]
+        c.free(src)
+        c.free(dst)
+    ms = st["ms"] / max(st["launches"], 1)
+    return {"metric": "HBM copy GB/s (unpack kernel, 1 GiB in + 1 GiB out)", "value": 16.0 * n / (ms * 1e-3) / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic", "config": {"workload": "copy_2^27_tuples", "bytes_read": 8 * n,
+                                            "bytes_written": 8 * n}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sg_mesh")
+    ap.add_argument("--workload", default="tc_grid")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--instances", type=int, default=512, help="batched: instance count")
@@ -271,7 +305,7 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="served: slots")
     ap.add_argument("--rmat-scale", type=int, default=23, help="served: resident R-MAT scale")
     ap.add_argument("--max-request-gb", type=int, default=8, help="served: per-request limit")
-    ap.add_argument("--strategy", type=int, default=0, help="full-relation layout (0 auto, 1..3)")
+    ap.add_argument("--strategy", type=int, default=0, help="full-relation layout (0 auto, 1..4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
@@ -282,6 +316,9 @@ def main():
         return
     if args.workload == "served":
         print(json.dumps(bench_served(args)), flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
         return
 
     import mnmgdatalog_amd as dlg

@@ -158,7 +158,9 @@ typedef struct dlg_opts {
                                 difference, sorted once at the end                    */
     int32_t max_iterations;  /* 0 = run to the fixpoint; >0 stops early (sampling)     */
     int32_t keep_output;     /* 1: copy the output rows to host (dlg_result.rows)      */
-    int32_t reserved[6];
+    int32_t check_output;    /* 1: fill dlg_result.output_sum / output_order_breaks
+                                (one pass over the device output, after the timing) */
+    int32_t reserved[5];
 } dlg_opts;
 
 /* Phase timers in the reference's CSV order (tc.cu:413-420). */
@@ -178,6 +180,14 @@ typedef struct dlg_result {
     int64_t n_rows;
     double device_seconds;   /* upload-to-resident-result wall time on the device    */
     double phase_seconds[DLG_PH_COUNT];
+    /* With check_output: a digest of the output rows (r0, r1) in file order,
+     * sum over rows of r0*0x9E3779B97F4A7C15 + r1*0xC2B2AE3D27D4EB4F + r0*r1
+     * + 0x632BE59BD9B4E019 (r0, r1 as uint32, arithmetic mod 2^64;
+     * order-independent), and the
+     * number of adjacent row pairs not strictly increasing in the output's
+     * sort order (0 for a sorted set). */
+    uint64_t output_sum;
+    int64_t output_order_breaks;
 } dlg_result;
 
 /* Whole-program entry points, the benchmark() role of tc.cu:42-423,

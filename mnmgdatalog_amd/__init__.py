@@ -42,14 +42,15 @@ class DlgError(RuntimeError):
 class Opts(ctypes.Structure):
     _fields_ = [("load_factor", c_double), ("block_size", c_int32), ("radix_bits", c_int32),
                 ("strategy", c_int32), ("max_iterations", c_int32), ("keep_output", c_int32),
-                ("reserved", c_int32 * 6)]
+                ("check_output", c_int32), ("reserved", c_int32 * 5)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("iterations", c_int32), ("output_size", c_int64), ("secondary", c_int64),
                 ("nodes", c_int64), ("raw_join_rows", c_int64), ("input_rows", c_int64),
                 ("rows", POINTER(c_int32)), ("n_rows", c_int64), ("device_seconds", c_double),
-                ("phase_seconds", c_double * 10)]
+                ("phase_seconds", c_double * 10), ("output_sum", ctypes.c_uint64),
+                ("output_order_breaks", c_int64)]
 
 
 class BatchStats(ctypes.Structure):
@@ -433,7 +434,8 @@ class Ctx:
         try:
             out = {"iterations": r.iterations, "output_size": r.output_size, "secondary": r.secondary,
                    "nodes": r.nodes, "raw_join_rows": r.raw_join_rows, "device_seconds": r.device_seconds,
-                   "phases": {PHASES[i]: r.phase_seconds[i] for i in range(10)}}
+                   "phases": {PHASES[i]: r.phase_seconds[i] for i in range(10)},
+                   "output_sum": r.output_sum, "output_order_breaks": r.output_order_breaks}
             out["rows"] = _rows_from(r.rows, r.n_rows) if r.rows else None
             return out
         finally:

@@ -239,6 +239,8 @@ __device__ __forceinline__ u64 upper_bound_u64(const u64 *a, u64 lo, u64 hi, u64
 u64 scan_u32_u64(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out);
 // Same without reading the total back; total left at d_total (device).
 void scan_u32_u64_async(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *d_total);
+// Exclusive scan of u64 values (out may alias in); total left at d_total.
+void scan_u64_async(dlg_ctx *ctx, const u64 *in, u64 n, u64 *out, u64 *d_total);
 // Stable LSD radix sort of u64 keys; result in `data`.  `tmp` is scratch of >= n.
 void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits = 0);
 // Same on a pool buffer: the sorted keys may end in the other buffer, in which
@@ -247,6 +249,9 @@ void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits = 0);
 // every element; otherwise a pass over the data computes them.
 void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits = 0,
                 const u32 *bounds = nullptr);
+// Digest of tuples as output rows in `orient` (dlg_result.output_sum) and the
+// number of i with d[i] >= d[i + 1] (synchronises).
+void rows_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u64 *sum, u64 *breaks);
 // Ranges of the two u64 halves over data[0..n): out = {min_hi, max_hi, min_lo, max_lo}.
 void tuple_bounds(dlg_ctx *ctx, const u64 *data, u64 n, u32 *out);
 // Compaction modes.
@@ -282,14 +287,14 @@ void scan_blocks(dlg_ctx *ctx, u64 *bsum, u64 nb, u64 *d_total);
 // ------------------------------------------------------------- bitmap (k_bitmap.hip)
 struct Index;
 struct JoinScratch;
-// The full relation as a W*W-bit map over the id range [lo, lo + W); bit
-// (k - lo) * W + (v - lo) for tuple (k, v), or (v - lo) * W + (k - lo) when
-// transposed.
+// The full relation as a bitmap over the id range [lo, lo + W): W rows of
+// wpr = ceil(W / 32) words (S = 32 * wpr bits); bit (k - lo) * S + (v - lo)
+// for tuple (k, v), or (v - lo) * S + (k - lo) when transposed.
 struct Bitmap {
     DBuf<u32> words;
     DBuf<u64> count;   // append counter of the insert kernels
     u32 lo = 0;
-    u64 W = 0, nbits = 0, nwords = 0;
+    u64 W = 0, wpr = 0, S = 0, nbits = 0, nwords = 0;
     bool tr = false;
     explicit Bitmap(dlg_ctx *c) : words(c), count(c) {}
 };
@@ -310,10 +315,32 @@ u64 bitmap_insert_list(dlg_ctx *ctx, const u64 *in, u64 n, bool nodiag, Bitmap &
 // Clear the bits of a list whose bits are the only ones set in their words.
 void bitmap_clear_list(dlg_ctx *ctx, const u64 *in, u64 n, Bitmap &bm);
 // All set bits as sorted tuples; `expect` = the number of set bits.  A
-// transposed map is emitted in (value, key) order and then radix sorted with
-// `bits` bits per pass -- unless it holds a symmetric relation, which equals
-// its transpose and is emitted as it lies.
-u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, int bits, bool symmetric);
+// transposed map is transposed back into a scratch map first -- unless it
+// holds a symmetric relation, which equals its transpose and is emitted as it
+// lies.
+u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symmetric);
+
+// ------------------------------------------------------------- TC rows (k_tcrows.hip)
+// TC over a transposed bitmap F with Δ grouped by source: segment s holds
+// the ends dy[seg_beg[s] ..][0, seg_len[s]) of source seg_x[s] (one segment
+// per source, fixed for the whole fixpoint; empty ones stay), plus the dense
+// CSR of E over F's id range.  seg_out = exclusive scan of the segments'
+// join rows = where their new ends go; rows = the round's join rows.
+struct TcRows {
+    DBuf<u64> csr_off, seg_beg, seg_out, next_rows, counters;
+    DBuf<u32> csr_dst, dy, ny, seg_x, seg_len, nseg_len;
+    u64 nd = 0, nseg = 0, rows = 0;
+    explicit TcRows(dlg_ctx *c)
+        : csr_off(c), seg_beg(c), seg_out(c), next_rows(c), counters(c), csr_dst(c), dy(c), ny(c), seg_x(c),
+          seg_len(c), nseg_len(c) {}
+};
+// CSR of the index's rows; Δ0 = unique(E reversed) grouped by source, its
+// bits set in F (F must be transposed and empty).
+void tc_rows_setup(dlg_ctx *ctx, const Index &ix, Bitmap &F, TcRows &tr);
+// One round: N = (Δ ⋈ E) \ F into F, Δ = N (grouped).  Returns |N|; *raw += |Δ ⋈ E|.
+u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw);
+// The current Δ as (z, x) tuples.
+void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out);
 
 // ------------------------------------------------------------- hash set (k_hashset.hip)
 struct HashSet {

@@ -17,6 +17,7 @@ struct Opts {
     bool keep;
     int strategy;
     int block_size;  // 0 = the context's
+    bool check;      // output digest
 };
 Opts read_opts(dlg_ctx *ctx, const dlg_opts *o);
 
@@ -56,9 +57,9 @@ struct RoundHook {
 // Id-range width up to which TC / SG keep the full relation as a bitmap
 // (W*W bits <= 2^36 bits = 8 GiB) when the strategy is automatic, and the
 // widest range the bitmap layout takes when asked for (32-bit word indexes:
-// W*W/32 < 2^32 - 1; 16 GiB).
+// W * ceil(W / 32) < 2^32 - 1; 16 GiB).
 constexpr u64 BITMAP_MAX_W = 1ull << 18;
-constexpr u64 BITMAP_FORCED_MAX_W = 370727;
+constexpr u64 BITMAP_FORCED_MAX_W = 370720;
 
 struct FixpointOut {
     DBuf<u64> full;   // final relation, sorted

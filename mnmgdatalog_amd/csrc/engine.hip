@@ -47,7 +47,7 @@ void PhaseClock::finish(double *out) {
 }
 
 Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
-    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0};
+    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0, false};
     if (o) {
         if (o->load_factor < 0 || o->load_factor > 0.95)
             fail(DLG_INVALID_INPUT, "load_factor out of (0, 0.95]");
@@ -62,6 +62,7 @@ Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
         if (o->radix_bits > 0) r.bits = o->radix_bits;
         r.max_it = o->max_iterations;
         r.keep = o->keep_output != 0;
+        r.check = o->check_output != 0;
         r.strategy = o->strategy;
     }
     return r;
@@ -101,6 +102,13 @@ static void result_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, dlg_resul
     res->rows = (int32_t *)std::malloc(sizeof(int32_t) * 2 * (n ? n : 1));
     if (!res->rows) fail(DLG_OUT_OF_MEMORY, "host allocation of result rows");
     download_rows(ctx, d, n, orient, res->rows);
+}
+
+static void result_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, dlg_result *res) {
+    u64 sum = 0, breaks = 0;
+    rows_digest(ctx, d, n, orient, &sum, &breaks);
+    res->output_sum = sum;
+    res->output_order_breaks = (int64_t)breaks;
 }
 
 void check_rows_arg(int64_t n, const void *rows) {
@@ -260,28 +268,31 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     JoinScratch js(ctx);
     int it = 0;
     if (use_bitmap(o, n, ib)) {
-        // F as a bitmap: each round's join rows go straight into it (k_bitmap.hip).
-        // Rows (z, x) of one probe share x, so x picks the bitmap row (transposed).
+        // F as a transposed bitmap (row = source) and Δ grouped by source: one
+        // workgroup per source row and round (k_tcrows.hip).
         Bitmap F(ctx);
         bitmap_init(ctx, F, ib[0], ib[1], true);
+        J.release();
+        TcRows tr(ctx);
         pc.mark(DLG_PH_DEDUP);
-        u64 nd = bitmap_insert_list(ctx, J.p, n, false, F, delta.p);    // Δ0 = F = unique(E reversed)
-        u64 total = nd, raw = 0;
+        tc_rows_setup(ctx, ix, F, tr);                  // Δ0 = F = unique(E reversed)
+        u64 total = tr.nd, raw = 0;
         while (true) {
             pc.mark(DLG_PH_JOIN);
-            u64 nn = bitmap_join_insert(ctx, ix, delta.p, nd, js, DLG_PROJ_HV_PV, false, F, N, &raw);
+            u64 nn = tc_rows_round(ctx, F, tr, &raw);
             total += nn;
             it++;
-            if (hook) hook->on_round(ctx, N.p, nn, it, false);
+            if (hook) {
+                tc_rows_tuples(ctx, tr, N);
+                hook->on_round(ctx, N.p, nn, it, false);
+            }
             if (nn == 0) break;                     // |F| unchanged (tc.cu:294-300)
-            delta.swap(N);
-            nd = nn;
             if (o.max_it > 0 && it >= o.max_it) break;
         }
         pc.mark(DLG_PH_FINAL);
         out.raw += (i64)raw;
         out.nfull = total;
-        bitmap_emit(ctx, F, out.full, total, o.bits, false);
+        bitmap_emit(ctx, F, out.full, total, false);
         out.iterations = it;
         return;
     }
@@ -362,7 +373,7 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         pc.mark(DLG_PH_FINAL);
         out.raw += (i64)raw;
         out.nfull = total;
-        bitmap_emit(ctx, F, out.full, total, o.bits, true);
+        bitmap_emit(ctx, F, out.full, total, true);
         out.iterations = it;
         return;
     }
@@ -456,6 +467,7 @@ static void run_tc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dl
     res->iterations = f.iterations;
     res->output_size = (int64_t)f.nfull;
     res->raw_join_rows = f.raw;
+    if (o.check) result_digest(ctx, f.full.p, f.nfull, DLG_ORIENT_VK, res);
     if (o.keep) result_rows(ctx, f.full.p, f.nfull, DLG_ORIENT_VK, res);   // (src, dst)
     pc.finish(res->phase_seconds);
 }
@@ -471,6 +483,7 @@ static void run_sg(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dl
     res->iterations = f.iterations;
     res->output_size = (int64_t)f.nfull;
     res->raw_join_rows = f.raw;
+    if (o.check) result_digest(ctx, f.full.p, f.nfull, DLG_ORIENT_KV, res);
     if (o.keep) result_rows(ctx, f.full.p, f.nfull, DLG_ORIENT_KV, res);   // (x, y)
     pc.finish(res->phase_seconds);
 }
@@ -486,6 +499,7 @@ static void run_sj(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, dl
     res->iterations = 1;
     res->output_size = (int64_t)f.nfull;
     res->raw_join_rows = f.raw;
+    if (o.check) result_digest(ctx, f.full.p, f.nfull, DLG_ORIENT_VK, res);
     if (o.keep) result_rows(ctx, f.full.p, f.nfull, DLG_ORIENT_VK, res);   // (a, c)
     pc.finish(res->phase_seconds);
 }
@@ -524,6 +538,7 @@ void wcc_solve(dlg_ctx *ctx, const WccGraph &g, const Opts &o, PhaseClock &pc, d
     res->output_size = (int64_t)ncomp;
     res->secondary = largest;
     res->nodes = (int64_t)g.V;
+    if (o.check) result_digest(ctx, out.p, ncomp, DLG_ORIENT_KV, res);
     if (o.keep) result_rows(ctx, out.p, ncomp, DLG_ORIENT_KV, res);   // (label, 0)
 }
 

@@ -1,34 +1,35 @@
 // k_bitmap.hip -- the full relation as a bitmap over the id range.
 //
 // Every tuple of a TC / SG fixpoint is a pair of node ids of the input, so
-// with W = the width of the id range, tuple (k, v) has the bit
-//     (k - lo) * W + (v - lo)            (row-major), or
-//     (v - lo) * W + (k - lo)            (transposed)
-// of a W*W-bit map (W = 46,656 ids -> 272 MB for the SG mesh).  The
-// relational steps of a round collapse into the join's materialize pass:
-//   for every join row t:  if bit(t) is clear, atomicOr it; the thread whose
-//   atomicOr flipped the bit appends t to N
+// with W = the width of the id range and rows padded to whole 32-bit words
+// (S = 32 * ceil(W / 32) bits per row), tuple (k, v) has the bit
+//     (k - lo) * S + (v - lo)            (row-major), or
+//     (v - lo) * S + (k - lo)            (transposed)
+// of a W x S-bit map (W = 46,656 ids -> 272 MB for the SG mesh; 160,000 ids
+// -> 3.2 GB for the 400 x 400 grid).  The relational steps of a round
+// collapse into the join's materialize pass:
+//   for every join row t:  if bit(t) is clear, set it; the row that set it
+//   is a new tuple
 // which is unique(J) \ F and F = F u N at once (tc.cu:250-283): J is never
-// written, sorted or merged.  The bit is read plainly first, so rows already
-// in F (most of them once a fixpoint is under way) cost one read, no atomic.
+// written, sorted or merged.
 //
 // A probe's join rows share the probe's value and differ in the matched
 // row's value; the layout puts the probe's half in the bitmap row, so one
-// probe's rows land in one bitmap row, usually in a few words.  Lanes of a
-// wave that hit the same word OR their bits together and issue one atomic
-// (a segmented OR scan over the wave), and the new tuples gather in LDS and
-// go out with one counter atomic per ~1K tuples.  N comes out unordered;
-// reading the bitmap in bit order emits the final relation (row-major:
-// already sorted; transposed: sorted afterwards).
+// probe's rows land in one bitmap row.  The bits of a tile of join rows are
+// ORed together per word in LDS first, so each distinct word costs one plain
+// read and at most one atomicOr.  New tuples are appended unordered, or
+// marked in a second map that is drained in bit order.  The final relation
+// is read out of the map in bit order: a row-major map is sorted as it lies,
+// a transposed one is transposed back (32 x 32-bit blocks) first.
 #include "dlg_internal.h"
 
 #include <algorithm>
 
 namespace dlg {
 
-__device__ __forceinline__ u64 bm_bit(u64 t, u32 lo, u64 W, bool tr) {
+__device__ __forceinline__ u64 bm_bit(u64 t, u32 lo, u64 S, bool tr) {
     u64 a = (u64)(tkey(t) - lo), b = (u64)(tval(t) - lo);
-    return tr ? b * W + a : a * W + b;
+    return tr ? b * S + a : a * S + b;
 }
 
 __device__ __forceinline__ bool bm_set(u32 *words, u64 b) {
@@ -88,7 +89,7 @@ static_assert(BH == 1 << 12, "bh_hash assumes 4096 slots");
 template <int PROJ, bool NODIAG, bool TR, int MODE>
 __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 np, const u32 *pstart,
                                                          const u64 *offs, u64 total, const u64 *rows,
-                                                         u32 *words, u32 lo, u64 W, u64 *out,
+                                                         u32 *words, u32 lo, u64 S, u64 *out,
                                                          unsigned long long *counter, u32 *mark) {
     __shared__ u64 s_lo, s_hi;
     __shared__ unsigned long long s_base;
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 n
             mask[r] = 0;
             wi[r] = 0;
             if (o < total && (!NODIAG || hv != pv)) {
-                u64 b = bm_bit(t[r], lo, W, TR);
+                u64 b = bm_bit(t[r], lo, S, TR);
                 wi[r] = (u32)(b >> 5);
                 mask[r] = 1u << (b & 31);
             }
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 n
 }
 
 template <bool NODIAG, bool TR>
-__global__ void bm_insert_list_k(const u64 *in, u64 n, u32 *words, u32 lo, u64 W, u64 *out,
+__global__ void bm_insert_list_k(const u64 *in, u64 n, u32 *words, u32 lo, u64 S, u64 *out,
                                  unsigned long long *counter) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     u64 stride = (u64)gridDim.x * blockDim.x;
@@ -301,17 +302,17 @@ __global__ void bm_insert_list_k(const u64 *in, u64 n, u32 *words, u32 lo, u64 W
         u64 t = 0;
         if (i < n) {
             t = in[i];
-            if (!NODIAG || tkey(t) != tval(t)) app = bm_set(words, bm_bit(t, lo, W, TR));
+            if (!NODIAG || tkey(t) != tval(t)) app = bm_set(words, bm_bit(t, lo, S, TR));
         }
         wave_append(app, t, out, counter);
     }
 }
 
 template <bool TR>
-__global__ void bm_clear_list_k(const u64 *in, u64 n, u32 *words, u32 lo, u64 W) {
+__global__ void bm_clear_list_k(const u64 *in, u64 n, u32 *words, u32 lo, u64 S) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     u64 stride = (u64)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) words[bm_bit(in[i], lo, W, TR) >> 5] = 0u;   // every set bit is in the list
+    for (; i < n; i += stride) words[bm_bit(in[i], lo, S, TR) >> 5] = 0u;   // every set bit is in the list
 }
 
 // Emission: per block, 256 threads x 8 words; popcounts -> scan -> write.
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(BE_T) void bm_count_k(const u32 *words, u64 *bsum) 
 
 // SWAP: write (column, row) instead of (row, column).  CLEAR: zero the words read.
 template <bool SWAP, bool CLEAR>
-__global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u32 lo, u64 W, u64 *out) {
+__global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u32 lo, u64 wpr, u64 *out) {
     __shared__ u64 sc[BE_T / WAVE + 1];
     u64 base = (u64)blockIdx.x * BE_TILE + (u64)threadIdx.x * BE_W;
     u32 w[BE_W];
@@ -357,19 +358,12 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
     for (int i = 0; i < BE_W; i++) {
         u32 x = w[i];
         if (!x) continue;
-        u64 b0 = (base + i) * 32;
-        u64 k = b0 / W, v = b0 - k * W;   // bit b0 -> (row, column); walk the word's bits in order
-        u32 prev = 0;
+        u64 k = (base + i) / wpr;               // row
+        u32 c0 = (u32)((base + i - k * wpr) * 32);   // first column of the word
         while (x) {
-            u32 j = __ffs(x) - 1;
+            u32 v = c0 + (u32)(__ffs(x) - 1);
             x &= x - 1;
-            v += j - prev;
-            prev = j;
-            while (v >= W) {
-                v -= W;
-                k++;
-            }
-            out[pos++] = SWAP ? tmake(lo + (u32)v, lo + (u32)k) : tmake(lo + (u32)k, lo + (u32)v);
+            out[pos++] = SWAP ? tmake(lo + v, lo + (u32)k) : tmake(lo + (u32)k, lo + v);
         }
     }
 }
@@ -377,8 +371,10 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
 void bitmap_init(dlg_ctx *ctx, Bitmap &bm, u32 lo, u32 hi, bool transposed) {
     bm.lo = lo;
     bm.W = (u64)(hi - lo) + 1;
-    bm.nbits = bm.W * bm.W;
-    bm.nwords = (bm.nbits + 31) / 32;
+    bm.wpr = (bm.W + 31) / 32;
+    bm.S = bm.wpr * 32;
+    bm.nwords = bm.W * bm.wpr;
+    bm.nbits = bm.nwords * 32;
     if (bm.nwords >= (u64)BH_EMPTY) fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
     bm.tr = transposed;
     u64 padded = (bm.nwords + BE_TILE - 1) / BE_TILE * BE_TILE;
@@ -404,11 +400,11 @@ static void launch_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, 
     if (bm.tr)
         DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, bytes, (bm_join_insert_k<PROJ, NODIAG, true, MODE>), g, BJ_T, 0, probe,
                    n, (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total, (const u64 *)ix.rows.p, bm.words.p,
-                   bm.lo, bm.W, out, cnt, mark);
+                   bm.lo, bm.S, out, cnt, mark);
     else
         DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, bytes, (bm_join_insert_k<PROJ, NODIAG, false, MODE>), g, BJ_T, 0,
                    probe, n, (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total, (const u64 *)ix.rows.p,
-                   bm.words.p, bm.lo, bm.W, out, cnt, mark);
+                   bm.words.p, bm.lo, bm.S, out, cnt, mark);
 }
 
 template <int MODE>
@@ -457,17 +453,17 @@ u64 bitmap_insert_list(dlg_ctx *ctx, const u64 *in, u64 n, bool nodiag, Bitmap &
     if (nodiag) {
         if (bm.tr)
             DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 12.0 * n, (bm_insert_list_k<true, true>), g, bs, 0, in, n,
-                       bm.words.p, bm.lo, bm.W, out, cnt);
+                       bm.words.p, bm.lo, bm.S, out, cnt);
         else
             DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 12.0 * n, (bm_insert_list_k<true, false>), g, bs, 0, in, n,
-                       bm.words.p, bm.lo, bm.W, out, cnt);
+                       bm.words.p, bm.lo, bm.S, out, cnt);
     } else {
         if (bm.tr)
             DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 12.0 * n, (bm_insert_list_k<false, true>), g, bs, 0, in, n,
-                       bm.words.p, bm.lo, bm.W, out, cnt);
+                       bm.words.p, bm.lo, bm.S, out, cnt);
         else
             DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 12.0 * n, (bm_insert_list_k<false, false>), g, bs, 0, in, n,
-                       bm.words.p, bm.lo, bm.W, out, cnt);
+                       bm.words.p, bm.lo, bm.S, out, cnt);
     }
     return read_counter(ctx, bm);
 }
@@ -476,19 +472,19 @@ void bitmap_clear_list(dlg_ctx *ctx, const u64 *in, u64 n, Bitmap &bm) {
     if (n == 0) return;
     if (bm.tr)
         DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, bm_clear_list_k<true>, sgrid(ctx, n), ctx->block_size, 0, in, n,
-                   bm.words.p, bm.lo, bm.W);
+                   bm.words.p, bm.lo, bm.S);
     else
         DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, bm_clear_list_k<false>, sgrid(ctx, n), ctx->block_size, 0, in, n,
-                   bm.words.p, bm.lo, bm.W);
+                   bm.words.p, bm.lo, bm.S);
 }
 
-// Count pass + block scan; returns the number of set bits (synchronises).
-static u64 bitmap_count(dlg_ctx *ctx, const Bitmap &bm, DBuf<u64> &bsum, u64 &nb) {
+// Count pass + block scan over a map's words; returns the number of set bits
+// (synchronises).
+static u64 bitmap_count(dlg_ctx *ctx, const Bitmap &bm, const u32 *words, DBuf<u64> &bsum, u64 &nb) {
     nb = (bm.nwords + BE_TILE - 1) / BE_TILE;
     bsum.ensure(nb);
     DBuf<u64> tot(ctx, 1);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * nb * BE_TILE, bm_count_k, (unsigned)nb, BE_T, 0, (const u32 *)bm.words.p,
-               bsum.p);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * nb * BE_TILE, bm_count_k, (unsigned)nb, BE_T, 0, words, bsum.p);
     scan_blocks(ctx, bsum.p, nb, tot.p);
     u64 h = 0;
     ctx->read_scalars(tot.p, 1, &h);
@@ -496,38 +492,65 @@ static u64 bitmap_count(dlg_ctx *ctx, const Bitmap &bm, DBuf<u64> &bsum, u64 &nb
 }
 
 template <bool SWAP, bool CLEAR>
-static void emit_pass(dlg_ctx *ctx, Bitmap &bm, const DBuf<u64> &bsum, u64 nb, u64 h, u64 *out) {
+static void emit_pass(dlg_ctx *ctx, const Bitmap &bm, u32 *words, const DBuf<u64> &bsum, u64 nb, u64 h,
+                      u64 *out) {
     DLG_LAUNCH(ctx, DLG_K_MISC, (CLEAR ? 8.0 : 4.0) * nb * BE_TILE + 8.0 * h, (bm_emit_k<SWAP, CLEAR>),
-               (unsigned)nb, BE_T, 0, bm.words.p, (const u64 *)bsum.p, bm.lo, bm.W, out);
+               (unsigned)nb, BE_T, 0, words, (const u64 *)bsum.p, bm.lo, bm.wpr, out);
 }
 
 u64 bitmap_drain(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, bool swap) {
     DBuf<u64> bsum(ctx);
     u64 nb = 0;
-    u64 h = bitmap_count(ctx, bm, bsum, nb);
+    u64 h = bitmap_count(ctx, bm, bm.words.p, bsum, nb);
     out.ensure(h ? h : 1);
     if (h == 0) return 0;
-    if (swap) emit_pass<true, true>(ctx, bm, bsum, nb, h, out.p);
-    else emit_pass<false, true>(ctx, bm, bsum, nb, h, out.p);
+    if (swap) emit_pass<true, true>(ctx, bm, bm.words.p, bsum, nb, h, out.p);
+    else emit_pass<false, true>(ctx, bm, bm.words.p, bsum, nb, h, out.p);
     return h;
 }
 
-u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, int bits, bool symmetric) {
+// Bit-matrix transpose of a W x S map, 32 x 32-bit blocks: 32 lanes load the
+// 32 rows of a block (one word each), five exchange steps swap the
+// off-diagonal halves of 16-, 8-, 4-, 2- and 1-bit sub-blocks, and lane i
+// stores the block's column i as a row of the destination.
+__global__ void bm_transpose_k(const u32 *src, u32 *dst, u64 W, u64 wpr) {
+    const int lane = threadIdx.x & 31;
+    u64 blk = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 5;   // one block per half wave
+    u64 nblk = wpr * wpr;
+    if (blk >= nblk) return;   // whole half waves leave together
+    u64 R = blk / wpr, C = blk - R * wpr;   // source block: rows 32R.., word column C
+    u64 r = R * 32 + lane;
+    u32 x = r < W ? src[r * wpr + C] : 0u;
+    const u32 m[5] = {0x0000ffffu, 0x00ff00ffu, 0x0f0f0f0fu, 0x33333333u, 0x55555555u};
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        const int j = 16 >> s;
+        u32 y = __shfl_xor(x, j, 32);
+        x = (lane & j) ? ((x & ~m[s]) | ((y & ~m[s]) >> j)) : ((x & m[s]) | ((y & m[s]) << j));
+    }
+    u64 c = C * 32 + lane;   // destination row = source column
+    if (c < W) dst[c * wpr + R] = x;
+}
+
+u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symmetric) {
     out.ensure(expect ? expect : 1);
     if (expect == 0) return 0;
+    u32 *words = bm.words.p;
+    DBuf<u32> tw(ctx);
+    if (bm.tr && !symmetric) {   // back to row-major, so that bit order is tuple order
+        u64 padded = (bm.nwords + BE_TILE - 1) / BE_TILE * BE_TILE;
+        tw.ensure(padded);
+        fill_u32(ctx, tw.p + bm.nwords, padded - bm.nwords, 0u);
+        u64 threads = bm.wpr * bm.wpr * 32;
+        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * bm.nwords, bm_transpose_k, grid_for(threads, 256), 256, 0,
+                   (const u32 *)bm.words.p, tw.p, bm.W, bm.wpr);
+        words = tw.p;
+    }
     DBuf<u64> bsum(ctx);
     u64 nb = 0;
-    u64 h = bitmap_count(ctx, bm, bsum, nb);
+    u64 h = bitmap_count(ctx, bm, words, bsum, nb);
     if (h != expect) fail(DLG_DEVICE_ERROR, "bitmap emission count mismatch");
-    bool swap = bm.tr && !symmetric;
-    if (swap) emit_pass<true, false>(ctx, bm, bsum, nb, h, out.p);
-    else emit_pass<false, false>(ctx, bm, bsum, nb, h, out.p);
-    if (swap) {   // emitted in (value, key) order
-        u32 hi = bm.lo + (u32)(bm.W - 1);
-        u32 bounds[4] = {bm.lo, hi, bm.lo, hi};
-        DBuf<u64> tmp(ctx);
-        radix_sort(ctx, out, h, tmp, bits, bounds);
-    }
+    emit_pass<false, false>(ctx, bm, words, bsum, nb, h, out.p);
     return h;
 }
 

@@ -100,7 +100,8 @@ void unpack_rows(dlg_ctx *ctx, const u64 *t, u64 n, int orient, int32_t *rows) {
 // =============================================================== scan
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 
-__global__ __launch_bounds__(SCAN_T) void scan_reduce_k(const u32 *in, u64 n, u64 *bsum) {
+template <typename T>
+__global__ __launch_bounds__(SCAN_T) void scan_reduce_k(const T *in, u64 n, u64 *bsum) {
     __shared__ u64 sc[SCAN_T / WAVE + 1];
     u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_I;
     u64 s = 0;
@@ -127,11 +128,11 @@ __global__ __launch_bounds__(1024) void scan_blocks_k(u64 *bsum, u64 nb, u64 *d_
     if (threadIdx.x == 0 && d_total) *d_total = carry;
 }
 
-__global__ __launch_bounds__(SCAN_T) void scan_apply_k(const u32 *in, u64 n, const u64 *bpre,
-                                                       u64 *out) {
+template <typename T>
+__global__ __launch_bounds__(SCAN_T) void scan_apply_k(const T *in, u64 n, const u64 *bpre, u64 *out) {
     __shared__ u64 sc[SCAN_T / WAVE + 1];
     u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_I;
-    u32 v[SCAN_I];
+    T v[SCAN_I];
     u64 s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_I; i++) {
@@ -146,15 +147,24 @@ __global__ __launch_bounds__(SCAN_T) void scan_apply_k(const u32 *in, u64 n, con
     }
 }
 
-void scan_u32_u64_async(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *d_total) {
+template <typename T>
+static void scan_to_u64_async(dlg_ctx *ctx, const T *in, u64 n, u64 *out, u64 *d_total) {
     u64 nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb == 0) nb = 1;
     DBuf<u64> bsum(ctx, nb);
-    DLG_LAUNCH(ctx, DLG_K_SCAN, 4.0 * n + 8.0 * nb, scan_reduce_k, (unsigned)nb, SCAN_T, 0, in, n,
-               bsum.p);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, (double)sizeof(T) * n + 8.0 * nb, scan_reduce_k<T>, (unsigned)nb, SCAN_T, 0,
+               in, n, bsum.p);
     DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb, d_total);
-    DLG_LAUNCH(ctx, DLG_K_SCAN, 12.0 * n, scan_apply_k, (unsigned)nb, SCAN_T, 0, in, n, bsum.p,
+    DLG_LAUNCH(ctx, DLG_K_SCAN, (sizeof(T) + 8.0) * n, scan_apply_k<T>, (unsigned)nb, SCAN_T, 0, in, n, bsum.p,
                out);
+}
+
+void scan_u32_u64_async(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *d_total) {
+    scan_to_u64_async<u32>(ctx, in, n, out, d_total);
+}
+
+void scan_u64_async(dlg_ctx *ctx, const u64 *in, u64 n, u64 *out, u64 *d_total) {
+    scan_to_u64_async<u64>(ctx, in, n, out, d_total);
 }
 
 u64 scan_u32_u64(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out) {
@@ -414,6 +424,54 @@ void tuple_bounds(dlg_ctx *ctx, const u64 *data, u64 n, u32 *h) {
     HIPCHK(hipMemcpyAsync(ctx->h_scalars, mm.p, 16, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     std::memcpy(h, ctx->h_scalars, 16);
+}
+
+// Per block: the digest sum and the order breaks of a grid-stride share,
+// reduced through the wave and LDS, then one atomic each.
+__global__ __launch_bounds__(256) void rows_digest_k(const u64 *d, u64 n, int orient, unsigned long long *acc) {
+    __shared__ u64 s_sum[256 / WAVE], s_brk[256 / WAVE];
+    u64 sum = 0, brk = 0;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        u64 t = d[i];
+        u64 k = (u64)(tkey(t) ^ 0x80000000u), v = (u64)(tval(t) ^ 0x80000000u);   // the int32 bits
+        u64 r0 = orient == DLG_ORIENT_KV ? k : v, r1 = orient == DLG_ORIENT_KV ? v : k;
+        sum += r0 * 0x9E3779B97F4A7C15ull + r1 * 0xC2B2AE3D27D4EB4Full + r0 * r1 + 0x632BE59BD9B4E019ull;
+        if (i + 1 < n && t >= d[i + 1]) brk++;
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o, WAVE);
+        brk += __shfl_xor(brk, o, WAVE);
+    }
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    if (lane == 0) {
+        s_sum[w] = sum;
+        s_brk[w] = brk;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 1; j < (int)(blockDim.x / WAVE); j++) {
+            sum += s_sum[j];
+            brk += s_brk[j];
+        }
+        atomicAdd(acc, (unsigned long long)sum);
+        atomicAdd(acc + 1, (unsigned long long)brk);
+    }
+}
+
+void rows_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u64 *sum, u64 *breaks) {
+    DBuf<u64> acc(ctx, 2);
+    fill_u64(ctx, acc.p, 2, 0ull);
+    if (n) {
+        unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 8);
+        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, rows_digest_k, g, 256, 0, d, n, orient,
+                   (unsigned long long *)acc.p);
+    }
+    u64 h[2];
+    ctx->read_scalars(acc.p, 2, h);
+    *sum = h[0];
+    *breaks = h[1];
 }
 
 void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits, const u32 *bounds) {

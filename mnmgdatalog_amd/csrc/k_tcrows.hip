@@ -1,0 +1,342 @@
+// k_tcrows.hip -- TC rounds with one wave per row of the bitmap.
+//
+// With the full relation F as a transposed bitmap (row x = a path's source,
+// column z = its end), a TC round (tc.cu:140-314)
+//     N = { (z, x) : (y, x) in Δ, (y, z) in E } \ F ;  F = F u N
+// only ever writes row x for the probes (y, x) of source x.  So Δ is kept
+// grouped by source -- a segment (x, begin, length) over an array of ends y
+// -- and one wave takes a whole segment (four per workgroup, no barriers):
+//   1 expand: the segment's probes y walk their adjacency in a dense CSR of
+//     E over the id range (no hash probe, no count pass);
+//   2 combine: the bits of each touched word of row x are ORed together in
+//     an LDS hash table;
+//   3 set: one atomicOr per touched word.  No other wave touches row x in
+//     this round, so the atomics are workgroup-scope: they execute in the
+//     L2 of the wave's XCD and are not forwarded to memory; kernel
+//     boundaries make the rows coherent between rounds;
+//   4 emit: the bits that were clear are the segment's new tuples, written
+//     contiguously as the next round's segment for x.
+// A segment whose touched words overflow the table flushes to a wave-private
+// staging area and is copied out as one segment at its end.
+#include "dlg_internal.h"
+
+#include <algorithm>
+
+namespace dlg {
+
+constexpr int TR_T = 256;                    // threads per workgroup: four waves, one segment each
+constexpr int TR_W = TR_T / WAVE;
+constexpr int TR_H = 1024;                   // LDS hash slots per wave
+constexpr int TR_FLUSH = 512;                // flush above this many occupied slots ...
+constexpr int TR_BATCH = 256;                // ... before each batch of at most this many rows
+constexpr u32 TR_EMPTY = 0xffffffffu;
+
+__device__ __forceinline__ u32 tr_hash(u32 w) { return (w * 0x9e3779b1u) >> (32 - 10); }
+static_assert(TR_H == 1 << 10, "tr_hash assumes 1024 slots");
+
+__device__ __forceinline__ u32 atomic_or_wg(u32 *p, u32 v) {
+    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// LDS written by some lanes of a wave and then read by others: order the
+// accesses at wave scope (the wave executes LDS operations in order; this
+// keeps the compiler from reordering them).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ u64 wave_excl_scan(u64 v, u64 *total) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    u64 inc = v;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        u64 t = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc += t;
+    }
+    *total = __shfl(inc, WAVE - 1, WAVE);
+    return inc - v;
+}
+
+struct TrWave {
+    u32 key[TR_H];
+    u32 bits[TR_H];
+    u16 used[TR_H];
+    u64 p_off[WAVE + 1];    // row offsets of the chunk's probes (p_off[WAVE] = total)
+    u64 p_start[WAVE];      // CSR start of each probe's adjacency
+    u32 nused;
+};
+
+// Flush the wave's table into row `rowbase` of F: per occupied slot one
+// atomicOr; the new bits are written as ends z (packed) to dst[0..), in slot
+// order, and their adjacency sizes summed into *next_rows.  Returns the
+// number written.  The table is left empty.
+__device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst, const u64 *csr_off,
+                        u64 *next_rows) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    wave_sync();
+    const u32 n = s.nused;
+    u64 cnt = 0;
+    for (u32 j = lane; j < n; j += WAVE) {
+        u32 slot = s.used[j];
+        u32 w = s.key[slot], b = s.bits[slot];
+        u32 nb = b & ~atomic_or_wg(F + rowbase + w, b);
+        s.bits[slot] = nb;
+        cnt += __popc(nb);
+    }
+    u64 tot;
+    u64 off = wave_excl_scan(cnt, &tot);
+    u64 deg = 0;
+    for (u32 j = lane; j < n; j += WAVE) {
+        u32 slot = s.used[j];
+        u32 w = s.key[slot], nb = s.bits[slot];
+        while (nb) {
+            u32 z = w * 32 + (u32)(__ffs(nb) - 1);
+            nb &= nb - 1;
+            dst[off++] = lo + z;
+            deg += csr_off[z + 1] - csr_off[z];
+        }
+        s.key[slot] = TR_EMPTY;
+        s.bits[slot] = 0u;
+    }
+    *next_rows += deg;
+    wave_sync();
+    if (lane == 0) s.nused = 0;
+    wave_sync();
+    return tot;
+}
+
+// One round.  Segment s (source seg_x[s], ends dy[seg_beg[s] ..][0, seg_len[s]))
+// writes its new ends at ny[seg_out[s] ..] -- seg_out is the exclusive scan
+// of the segments' join rows, which bound their new tuples -- and its next
+// length and next join rows at the same index s.  Workgroups take static
+// ranges of segments; the four waves of a workgroup share the range through
+// an LDS counter, so no global counter is contended.
+__global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_x, const u64 *seg_beg,
+                                                  const u32 *seg_len, const u64 *seg_out, u64 nseg,
+                                                  const u64 *csr_off, const u32 *csr_dst, u32 *F, u32 lo, u64 wpr,
+                                                  u32 *ny, u32 *nseg_len, u64 *next_rows,
+                                                  unsigned long long *new_total) {
+    __shared__ TrWave sw[TR_W];
+    __shared__ u32 s_next;
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    TrWave &s = sw[wv];
+    for (int i = lane; i < TR_H; i += WAVE) {
+        s.key[i] = TR_EMPTY;
+        s.bits[i] = 0u;
+    }
+    if (lane == 0) s.nused = 0;
+    const u64 per = (nseg + gridDim.x - 1) / gridDim.x;
+    const u64 b0 = (u64)blockIdx.x * per, b1 = b0 + per < nseg ? b0 + per : nseg;
+    if (threadIdx.x == 0) s_next = 0;
+    __syncthreads();
+    u64 wave_new = 0;
+    while (true) {
+        u32 k0 = 0;
+        if (lane == 0) k0 = atomicAdd(&s_next, 1u);
+        const u64 sg = b0 + __shfl(k0, 0, WAVE);
+        if (sg >= b1) break;
+        const u32 len = seg_len[sg];
+        u64 rows_next = 0, cnt = 0;
+        if (len) {
+            const u32 x = seg_x[sg];
+            const u64 beg = seg_beg[sg], out0 = seg_out[sg];
+            const u64 rowbase = (u64)(x - lo) * wpr;
+            for (u32 pc = 0; pc < len; pc += WAVE) {
+                // the chunk's probes, their adjacency ranges and row offsets
+                u64 d = 0, st = 0;
+                if (pc + lane < len) {
+                    u32 y = dy[beg + pc + lane] - lo;
+                    st = csr_off[y];
+                    d = csr_off[y + 1] - st;
+                }
+                u64 R;
+                u64 ex = wave_excl_scan(d, &R);
+                wave_sync();
+                s.p_off[lane] = ex;
+                s.p_start[lane] = st;
+                if (lane == 0) s.p_off[WAVE] = R;
+                wave_sync();
+                for (u64 rb = 0; rb < R; rb += TR_BATCH) {
+                    if (s.nused > (u32)TR_FLUSH)   // uniform: every lane reads the same LDS word
+                        cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt, csr_off, &rows_next);
+                    const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
+                    for (u64 r = rb + lane; r < re; r += WAVE) {
+                        u32 a = 0, b = WAVE;   // last probe with p_off <= r
+                        while (b - a > 1) {
+                            u32 m = (a + b) >> 1;
+                            if (s.p_off[m] <= r) a = m; else b = m;
+                        }
+                        u32 z = csr_dst[s.p_start[a] + (r - s.p_off[a])] - lo;
+                        u32 w = z >> 5, mk = 1u << (z & 31);
+                        u32 h = tr_hash(w);
+                        while (true) {
+                            u32 k = atomicCAS(&s.key[h], TR_EMPTY, w);
+                            if (k == TR_EMPTY) {
+                                s.used[atomicAdd(&s.nused, 1u)] = (u16)h;
+                                break;
+                            }
+                            if (k == w) break;
+                            h = (h + 1) & (TR_H - 1);
+                        }
+                        atomicOr(&s.bits[h], mk);
+                    }
+                    wave_sync();
+                }
+            }
+            cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt, csr_off, &rows_next);
+        }
+        // per-lane partial sums of the next join rows -> one value
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) rows_next += __shfl_xor(rows_next, o, WAVE);
+        if (lane == 0) {
+            nseg_len[sg] = (u32)cnt;
+            next_rows[sg] = rows_next;
+        }
+        wave_new += cnt;
+    }
+    if (lane == 0 && wave_new) atomicAdd(new_total, (unsigned long long)wave_new);
+}
+
+// Σ deg over each segment's ends (the first round's join rows per segment).
+__global__ void tc_rows_segrows_k(const u32 *dy, const u64 *seg_beg, const u32 *seg_len, u64 nseg,
+                                  const u64 *csr_off, u32 lo, u64 *rows) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nseg; j += stride) {
+        u64 s = 0;
+        for (u32 i = 0; i < seg_len[j]; i++) {
+            u32 y = dy[seg_beg[j] + i] - lo;
+            s += csr_off[y + 1] - csr_off[y];
+        }
+        rows[j] = s;
+    }
+}
+
+// ---------------------------------------------------------------- set-up
+__global__ void csr_hist_k(const u64 *rows, u64 n, u32 lo, u32 *cnt) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        atomicAdd(cnt + (tkey(rows[i]) - lo), 1u);
+}
+__global__ void csr_dst_k(const u64 *rows, u64 n, u32 *dst) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = tval(rows[i]);
+}
+// Δ0 from the unique sorted edges (src, dst): ends = dst, one segment per src
+// run (heads = run starts), and the edges' bits (row src, column dst) in F.
+__global__ void tc_rows_init_k(const u64 *uniq, u64 n, const u32 *heads, u64 nh, u32 *dy, u32 *seg_x,
+                               u64 *seg_beg, u32 *seg_len, u32 *F, u32 lo, u64 wpr) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        u64 t = uniq[i];
+        u32 sx = tkey(t) - lo, dz = tval(t) - lo;
+        dy[i] = tval(t);
+        atomicOr(F + (u64)sx * wpr + (dz >> 5), 1u << (dz & 31));
+    }
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += stride) {
+        u64 b = heads[j], e = j + 1 < nh ? heads[j + 1] : n;
+        seg_x[j] = tkey(uniq[b]);
+        seg_beg[j] = b;
+        seg_len[j] = (u32)(e - b);
+    }
+}
+// Segments back to (z, x) tuples, compacted (the ends have gaps between
+// segments), for a round hook.
+__global__ void tc_rows_tuples_k(const u32 *dy, const u32 *seg_x, const u64 *seg_beg, const u32 *seg_len,
+                                 const u64 *pos, u64 nseg, u64 *out) {
+    for (u64 j = blockIdx.x; j < nseg; j += gridDim.x)
+        for (u32 i = threadIdx.x; i < seg_len[j]; i += blockDim.x)
+            out[pos[j] + i] = tmake(dy[seg_beg[j] + i], seg_x[j]);
+}
+__global__ void u32_to_u64_k(const u32 *in, u64 n, u64 *out) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = in[i];
+}
+
+void tc_rows_setup(dlg_ctx *ctx, const Index &ix, Bitmap &F, TcRows &tr) {
+    const u64 W = F.W, n = ix.n;
+    // CSR of E over the id range (duplicates kept, as the hash index keeps them)
+    DBuf<u32> cnt(ctx, W + 1);
+    fill_u32(ctx, cnt.p, W + 1, 0u);
+    if (n)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, csr_hist_k, sgrid(ctx, n), ctx->block_size, 0,
+                   (const u64 *)ix.rows.p, n, F.lo, cnt.p);
+    tr.csr_off.ensure(W + 1);
+    scan_u32_u64(ctx, cnt.p, W + 1, tr.csr_off.p);
+    tr.csr_dst.ensure(n ? n : 1);
+    if (n)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, csr_dst_k, sgrid(ctx, n), ctx->block_size, 0,
+                   (const u64 *)ix.rows.p, n, tr.csr_dst.p);
+    // Δ0 = unique(E reversed), grouped by source; one segment per source, for good
+    DBuf<u64> uniq(ctx, n ? n : 1);
+    u64 nu = n ? select(ctx, SEL_UNIQUE, ix.rows.p, n, nullptr, 0, uniq.p) : 0;
+    DBuf<u32> heads(ctx, nu ? nu : 1);
+    u64 nh = nu ? select(ctx, SEL_KEY_HEAD, uniq.p, nu, nullptr, 0, (u64 *)heads.p) : 0;
+    tr.dy.ensure(nu ? nu : 1);
+    tr.seg_x.ensure(nh ? nh : 1);
+    tr.seg_beg.ensure(nh ? nh : 1);
+    tr.seg_len.ensure(nh ? nh : 1);
+    tr.nseg_len.ensure(nh ? nh : 1);
+    tr.seg_out.ensure(nh ? nh : 1);
+    tr.next_rows.ensure(nh ? nh : 1);
+    tr.counters.ensure(4);
+    tr.nd = nu;
+    tr.nseg = nh;
+    tr.rows = 0;
+    if (!nu) return;
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 20.0 * nu, tc_rows_init_k, sgrid(ctx, nu), ctx->block_size, 0,
+               (const u64 *)uniq.p, nu, (const u32 *)heads.p, nh, tr.dy.p, tr.seg_x.p, tr.seg_beg.p,
+               tr.seg_len.p, F.words.p, F.lo, F.wpr);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * nu, tc_rows_segrows_k, sgrid(ctx, nh), ctx->block_size, 0,
+               (const u32 *)tr.dy.p, (const u64 *)tr.seg_beg.p, (const u32 *)tr.seg_len.p, nh,
+               (const u64 *)tr.csr_off.p, F.lo, tr.seg_out.p);
+    scan_u64_async(ctx, tr.seg_out.p, nh, tr.seg_out.p, tr.counters.p + 1);
+    u64 h[2];
+    ctx->read_scalars(tr.counters.p, 2, h);
+    tr.rows = h[1];
+}
+
+u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw) {
+    if (tr.nd == 0 || tr.rows == 0) {
+        tr.nd = 0;
+        return 0;
+    }
+    if (raw) *raw += tr.rows;
+    tr.ny.ensure(tr.rows);
+    fill_u64(ctx, tr.counters.p, 2, 0ull);
+    unsigned g = (unsigned)std::min<u64>((tr.nseg + TR_W - 1) / TR_W, (u64)ctx->num_cus * 3);
+    // per join row: the CSR end (4 B) and an LDS insert; per touched word an
+    // L2 atomic; per new tuple its end (4 B) and two CSR offsets
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 4.0 * tr.rows + 4.0 * tr.nd, tc_rows_k, g, TR_T, 0,
+               (const u32 *)tr.dy.p, (const u32 *)tr.seg_x.p, (const u64 *)tr.seg_beg.p,
+               (const u32 *)tr.seg_len.p, (const u64 *)tr.seg_out.p, tr.nseg, (const u64 *)tr.csr_off.p,
+               (const u32 *)tr.csr_dst.p, F.words.p, F.lo, F.wpr, tr.ny.p, tr.nseg_len.p, tr.next_rows.p,
+               (unsigned long long *)tr.counters.p);
+    // the new ends sit at the old output offsets; the next offsets scan the next rows
+    tr.seg_beg.swap(tr.seg_out);
+    scan_u64_async(ctx, tr.next_rows.p, tr.nseg, tr.seg_out.p, tr.counters.p + 1);
+    u64 h[2];
+    ctx->read_scalars(tr.counters.p, 2, h);
+    tr.dy.swap(tr.ny);
+    tr.seg_len.swap(tr.nseg_len);
+    tr.nd = h[0];
+    tr.rows = h[1];
+    return h[0];
+}
+
+void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out) {
+    out.ensure(tr.nd ? tr.nd : 1);
+    if (!tr.nd) return;
+    DBuf<u64> pos(ctx, tr.nseg);
+    DBuf<u64> tot(ctx, 1);
+    DBuf<u64> lens(ctx, tr.nseg);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * tr.nseg, u32_to_u64_k, sgrid(ctx, tr.nseg), ctx->block_size, 0,
+               (const u32 *)tr.seg_len.p, tr.nseg, lens.p);
+    scan_u64_async(ctx, lens.p, tr.nseg, pos.p, tot.p);
+    unsigned g = (unsigned)std::min<u64>(tr.nseg, (u64)ctx->num_cus * 8);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * tr.nd, tc_rows_tuples_k, g, 256, 0, (const u32 *)tr.dy.p,
+               (const u32 *)tr.seg_x.p, (const u64 *)tr.seg_beg.p, (const u32 *)tr.seg_len.p,
+               (const u64 *)pos.p, tr.nseg, out.p);
+}
+
+}  // namespace dlg

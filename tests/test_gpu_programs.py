@@ -300,3 +300,47 @@ def test_bitmap_strategy_rejects_wide_id_range(ctx):
     with pytest.raises(dlg.DlgError) as e:
         ctx.run("tc", rows, strategy=4)
     assert e.value.status == dlg.DLG_INVALID_INPUT
+    rows = np.array([[0, 1], [1, 370_720]], np.int32)   # W = 370,721: word indexes past 32 bits
+    with pytest.raises(dlg.DlgError) as e:
+        ctx.run("tc", rows, strategy=4)
+    assert e.value.status == dlg.DLG_INVALID_INPUT
+
+
+# ------------------------------------------------------------------ the road-shaped grid
+GRID_SEED = 0x4D4E4D47
+
+
+@pytest.mark.parametrize("L", [20, 45])
+def test_tc_grid_all_layouts(ctx, orc, L):
+    rows = dlg.gen_grid(L, 0.55, 0.01, 20, GRID_SEED)
+    want = orc.tc(rows, threads=4)
+    for s in (0, 1, 2, 3, 4):
+        assert_same(ctx.run("tc", rows, strategy=s), want)
+
+
+def test_output_digest_matches_rows(ctx, orc):
+    from grid_closure import rows_digest
+    rows = read_bin("data_7035.bin")
+    for prog in ("tc", "sg", "wcc", "single_join"):
+        got = ctx.run(prog, rows, check_output=1)
+        assert got["output_sum"] == rows_digest(got["rows"])
+        assert got["output_order_breaks"] == 0
+    got = ctx.run("tc", rows, strategy=4, check_output=1)
+    assert got["output_sum"] == rows_digest(got["rows"]) and got["output_order_breaks"] == 0
+
+
+def test_tc_grid_400_full_size():
+    """TC of the 400 x 400 grid: 20,691,180,264 tuples (166 GB resident),
+    501 iterations, against the closed forms (size, digest) and the diameter
+    computed independently (grid_closure.json)."""
+    import json
+    with open(os.path.join(os.path.dirname(DATA), "grid_closure.json")) as f:
+        fx = json.load(f)
+    rows = dlg.gen_grid(fx["grid"], fx["p"], fx["shortcut_frac"], fx["shortcut_dist"], fx["seed"])
+    assert len(rows) == fx["edges"]
+    with dlg.Ctx(0) as c:   # its own pool: released when the test ends
+        got = c.run("tc", rows, keep_output=False, check_output=1)
+    assert got["output_size"] == fx["tc_size"]
+    assert got["output_sum"] == int(fx["tc_output_sum"])
+    assert got["output_order_breaks"] == 0
+    assert got["iterations"] == fx["iterations"]

@@ -151,3 +151,27 @@ def test_cli_reports_error_without_gpu(tmp_path):
     p = subprocess.run([os.path.join(dlg.BIN_DIR, "tc"), os.path.join(DATA, "hipc_2019.bin")],
                        capture_output=True, text=True)
     assert p.returncode != 0 and "failed" in p.stderr
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """Size and field offsets of every struct the Python binding mirrors, as
+    gcc lays out include/dlg.h."""
+    import ctypes
+    mirrors = {"dlg_opts": dlg.Opts, "dlg_result": dlg.Result, "dlg_batch_stats": dlg.BatchStats,
+               "dlg_server_opts": dlg.ServerOpts, "dlg_request_info": dlg.RequestInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dlg.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "abi"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
