@@ -33,10 +33,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 SEED = 0x4D4E4D47
 
-# Reported HBM traffic per launch of the dominant kernel, from rocprofv3 --pmc
-# FETCH_SIZE + WRITE_SIZE runs of this same command (bytes per launch), keyed
-# by (workload, kernel).  Filled from the measured profile summaries.
-MEASURED_TRAFFIC = {}
+# HBM traffic per launch of the dominant kernel, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of `bench.py --workload W --steps 1
+# --warmup 0 --no-cpu`, scaled by the copy-kernel calibration (FETCH x 2.0,
+# WRITE x 1.0): bytes per launch, keyed by (workload, kernel id).
+MEASURED_TRAFFIC = {("tc_grid", "bitmap_insert"): 2.694e9}
 
 
 def workload_rows(name):
@@ -62,7 +63,7 @@ def workload_rows(name):
 
 
 # bounded CPU sample per workload: number of fixpoint iterations the oracle runs
-CPU_SAMPLE_ITERS = {"tc_grid": 20, "sg_mesh": 3, "tc_powerlaw": 0, "tc_usroad": 40, "wcc_grid": 0, "single_join": 0}
+CPU_SAMPLE_ITERS = {"tc_grid": 30, "sg_mesh": 3, "tc_powerlaw": 0, "tc_usroad": 40, "wcc_grid": 0, "single_join": 0}
 
 
 def cpu_baseline(name, prog, rows, ctx, threads, strategy=0):

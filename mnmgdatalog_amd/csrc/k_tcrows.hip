@@ -10,10 +10,10 @@
 //     E over the id range (no hash probe, no count pass);
 //   2 combine: the bits of each touched word of row x are ORed together in
 //     an LDS hash table;
-//   3 set: one atomicOr per touched word.  No other wave touches row x in
-//     this round, so the atomics are workgroup-scope: they execute in the
-//     L2 of the wave's XCD and are not forwarded to memory; kernel
-//     boundaries make the rows coherent between rounds;
+//   3 set: one plain read and (if a bit is new) one plain write per touched
+//     word -- no other wave touches row x in this round, so no atomics
+//     (device atomics execute at the memory side, ~1e10/s for the whole
+//     chip); kernel boundaries make the rows coherent between rounds;
 //   4 emit: the bits that were clear are the segment's new tuples, written
 //     contiguously as the next round's segment for x.
 // A segment whose touched words overflow the table flushes to a wave-private
@@ -34,15 +34,12 @@ constexpr u32 TR_EMPTY = 0xffffffffu;
 __device__ __forceinline__ u32 tr_hash(u32 w) { return (w * 0x9e3779b1u) >> (32 - 10); }
 static_assert(TR_H == 1 << 10, "tr_hash assumes 1024 slots");
 
-__device__ __forceinline__ u32 atomic_or_wg(u32 *p, u32 v) {
-    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// LDS written by some lanes of a wave and then read by others: order the
-// accesses at wave scope (the wave executes LDS operations in order; this
-// keeps the compiler from reordering them).
+// Memory written by some lanes of a wave and then read by others (the LDS
+// table; F's words across two flushes of one segment): a workgroup-scope
+// fence orders them (the wave's lanes share one CU and its L1), and the
+// wave barrier keeps the compiler from moving accesses across it.
 __device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -67,8 +64,8 @@ struct TrWave {
     u32 nused;
 };
 
-// Flush the wave's table into row `rowbase` of F: per occupied slot one
-// atomicOr; the new bits are written as ends z (packed) to dst[0..), in slot
+// Flush the wave's table into row `rowbase` of F: per occupied slot a read
+// and, for new bits, a write of the word; the new bits are written as ends z (packed) to dst[0..), in slot
 // order, and their adjacency sizes summed into *next_rows.  Returns the
 // number written.  The table is left empty.
 __device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst, const u64 *csr_off,
@@ -80,7 +77,9 @@ __device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst, const 
     for (u32 j = lane; j < n; j += WAVE) {
         u32 slot = s.used[j];
         u32 w = s.key[slot], b = s.bits[slot];
-        u32 nb = b & ~atomic_or_wg(F + rowbase + w, b);
+        u32 cur = F[rowbase + w];   // the row's owner: a plain read-modify-write
+        u32 nb = b & ~cur;
+        if (nb) F[rowbase + w] = cur | nb;
         s.bits[slot] = nb;
         cnt += __popc(nb);
     }
