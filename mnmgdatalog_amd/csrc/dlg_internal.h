@@ -328,11 +328,11 @@ u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symme
 // join rows = where their new ends go; rows = the round's join rows.
 struct TcRows {
     DBuf<u64> csr_off, seg_beg, seg_out, next_rows, counters;
-    DBuf<u32> csr_dst, dy, ny, seg_x, seg_len, nseg_len;
+    DBuf<u32> csr_dst, deg, dy, ny, seg_x, seg_len, nseg_len;
     u64 nd = 0, nseg = 0, rows = 0;
     explicit TcRows(dlg_ctx *c)
-        : csr_off(c), seg_beg(c), seg_out(c), next_rows(c), counters(c), csr_dst(c), dy(c), ny(c), seg_x(c),
-          seg_len(c), nseg_len(c) {}
+        : csr_off(c), seg_beg(c), seg_out(c), next_rows(c), counters(c), csr_dst(c), deg(c), dy(c), ny(c),
+          seg_x(c), seg_len(c), nseg_len(c) {}
 };
 // CSR of the index's rows; Δ0 = unique(E reversed) grouped by source, its
 // bits set in F (F must be transposed and empty).

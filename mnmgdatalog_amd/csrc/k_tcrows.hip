@@ -15,9 +15,10 @@
 //     (device atomics execute at the memory side, ~1e10/s for the whole
 //     chip); kernel boundaries make the rows coherent between rounds;
 //   4 emit: the bits that were clear are the segment's new tuples, written
-//     contiguously as the next round's segment for x.
-// A segment whose touched words overflow the table flushes to a wave-private
-// staging area and is copied out as one segment at its end.
+//     contiguously as the next round's segment for x, into a region reserved
+//     by a scan of the segments' join-row counts (no global counter; a
+//     segment whose touched words overflow the table flushes several times
+//     into its region).
 #include "dlg_internal.h"
 
 #include <algorithm>
@@ -28,7 +29,7 @@ constexpr int TR_T = 256;                    // threads per workgroup: four wave
 constexpr int TR_W = TR_T / WAVE;
 constexpr int TR_H = 1024;                   // LDS hash slots per wave
 constexpr int TR_FLUSH = 512;                // flush above this many occupied slots ...
-constexpr int TR_BATCH = 256;                // ... before each batch of at most this many rows
+constexpr int TR_BATCH = 256;                // ... before each batch of at most this many rows (TR_U per lane)
 constexpr u32 TR_EMPTY = 0xffffffffu;
 
 __device__ __forceinline__ u32 tr_hash(u32 w) { return (w * 0x9e3779b1u) >> (32 - 10); }
@@ -64,41 +65,67 @@ struct TrWave {
     u32 nused;
 };
 
+constexpr int TR_U = 4;   // words / rows per lane with their loads in flight together
+constexpr int TR_SMALL = 8;   // chunks whose probes all have at most this many matches: lane per probe
+constexpr int TR_FLUSH_SMALL = TR_H - TR_SMALL * WAVE - TR_H / 4;   // keeps the table <= 3/4 full
+static_assert(TR_FLUSH_SMALL > 0 && TR_FLUSH + TR_BATCH <= TR_H * 3 / 4, "table load");
+
+// OR bit z of the row into the wave's table (z relative to the id range).
+__device__ __forceinline__ void tr_insert(TrWave &s, u32 z) {
+    u32 w = z >> 5, mk = 1u << (z & 31);
+    u32 h = tr_hash(w);
+    while (true) {
+        u32 k = atomicCAS(&s.key[h], TR_EMPTY, w);
+        if (k == TR_EMPTY) {
+            s.used[atomicAdd(&s.nused, 1u)] = (u16)h;
+            break;
+        }
+        if (k == w) break;
+        h = (h + 1) & (TR_H - 1);
+    }
+    atomicOr(&s.bits[h], mk);
+}
+
 // Flush the wave's table into row `rowbase` of F: per occupied slot a read
-// and, for new bits, a write of the word; the new bits are written as ends z (packed) to dst[0..), in slot
-// order, and their adjacency sizes summed into *next_rows.  Returns the
-// number written.  The table is left empty.
-__device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst, const u64 *csr_off,
-                        u64 *next_rows) {
+// and, for new bits, a write of the word (TR_U words per lane in flight);
+// the new bits are written as ends z (packed) to dst[0..), in slot order.
+// Returns the number written.  The table is left empty.
+__device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst) {
     const int lane = threadIdx.x & (WAVE - 1);
     wave_sync();
     const u32 n = s.nused;
     u64 cnt = 0;
-    for (u32 j = lane; j < n; j += WAVE) {
-        u32 slot = s.used[j];
-        u32 w = s.key[slot], b = s.bits[slot];
-        u32 cur = F[rowbase + w];   // the row's owner: a plain read-modify-write
-        u32 nb = b & ~cur;
-        if (nb) F[rowbase + w] = cur | nb;
-        s.bits[slot] = nb;
-        cnt += __popc(nb);
+    for (u32 j0 = 0; j0 < n; j0 += TR_U * WAVE) {
+        u32 slot[TR_U], w[TR_U], b[TR_U], cur[TR_U];
+#pragma unroll
+        for (int q = 0; q < TR_U; q++) {
+            u32 j = j0 + q * WAVE + lane;
+            slot[q] = j < n ? s.used[j] : 0u;
+            w[q] = j < n ? s.key[slot[q]] : 0u;
+            b[q] = j < n ? s.bits[slot[q]] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < TR_U; q++) cur[q] = b[q] ? F[rowbase + w[q]] : 0u;   // the row's owner:
+#pragma unroll                                                              // plain read-modify-write
+        for (int q = 0; q < TR_U; q++) {
+            u32 nb = b[q] & ~cur[q];
+            if (nb) F[rowbase + w[q]] = cur[q] | nb;
+            if (b[q]) s.bits[slot[q]] = nb;
+            cnt += __popc(nb);
+        }
     }
     u64 tot;
     u64 off = wave_excl_scan(cnt, &tot);
-    u64 deg = 0;
     for (u32 j = lane; j < n; j += WAVE) {
         u32 slot = s.used[j];
         u32 w = s.key[slot], nb = s.bits[slot];
         while (nb) {
-            u32 z = w * 32 + (u32)(__ffs(nb) - 1);
+            dst[off++] = lo + w * 32 + (u32)(__ffs(nb) - 1);
             nb &= nb - 1;
-            dst[off++] = lo + z;
-            deg += csr_off[z + 1] - csr_off[z];
         }
         s.key[slot] = TR_EMPTY;
         s.bits[slot] = 0u;
     }
-    *next_rows += deg;
     wave_sync();
     if (lane == 0) s.nused = 0;
     wave_sync();
@@ -113,8 +140,8 @@ __device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst, const 
 // an LDS counter, so no global counter is contended.
 __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_x, const u64 *seg_beg,
                                                   const u32 *seg_len, const u64 *seg_out, u64 nseg,
-                                                  const u64 *csr_off, const u32 *csr_dst, u32 *F, u32 lo, u64 wpr,
-                                                  u32 *ny, u32 *nseg_len, u64 *next_rows,
+                                                  const u64 *csr_off, const u32 *csr_dst, const u32 *deg, u32 *F,
+                                                  u32 lo, u64 wpr, u32 *ny, u32 *nseg_len, u64 *next_rows,
                                                   unsigned long long *new_total) {
     __shared__ TrWave sw[TR_W];
     __shared__ u32 s_next;
@@ -142,13 +169,31 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
             const u64 beg = seg_beg[sg], out0 = seg_out[sg];
             const u64 rowbase = (u64)(x - lo) * wpr;
             for (u32 pc = 0; pc < len; pc += WAVE) {
-                // the chunk's probes, their adjacency ranges and row offsets
-                u64 d = 0, st = 0;
+                // the chunk's probes and their adjacency ranges
+                u64 st = 0;
+                u32 d = 0;
                 if (pc + lane < len) {
                     u32 y = dy[beg + pc + lane] - lo;
                     st = csr_off[y];
-                    d = csr_off[y + 1] - st;
+                    d = deg[y];
                 }
+                u32 maxd = d;
+#pragma unroll
+                for (int o = WAVE / 2; o > 0; o >>= 1) maxd = max(maxd, (u32)__shfl_xor((int)maxd, o, WAVE));
+                if (maxd <= (u32)TR_SMALL) {
+                    // lane per probe: all of its matches gathered at once, then inserted
+                    if (s.nused > (u32)TR_FLUSH_SMALL)   // uniform: every lane reads the same LDS word
+                        cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
+                    u32 zs[TR_SMALL];
+#pragma unroll
+                    for (int i = 0; i < TR_SMALL; i++) zs[i] = (u32)i < d ? csr_dst[st + i] : 0u;
+#pragma unroll
+                    for (int i = 0; i < TR_SMALL; i++)
+                        if ((u32)i < d) tr_insert(s, zs[i] - lo);
+                    wave_sync();
+                    continue;
+                }
+                // load-balanced rows: row offsets of the chunk's probes in LDS
                 u64 R;
                 u64 ex = wave_excl_scan(d, &R);
                 wave_sync();
@@ -158,34 +203,46 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
                 wave_sync();
                 for (u64 rb = 0; rb < R; rb += TR_BATCH) {
                     if (s.nused > (u32)TR_FLUSH)   // uniform: every lane reads the same LDS word
-                        cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt, csr_off, &rows_next);
+                        cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
                     const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
-                    for (u64 r = rb + lane; r < re; r += WAVE) {
-                        u32 a = 0, b = WAVE;   // last probe with p_off <= r
-                        while (b - a > 1) {
-                            u32 m = (a + b) >> 1;
-                            if (s.p_off[m] <= r) a = m; else b = m;
-                        }
-                        u32 z = csr_dst[s.p_start[a] + (r - s.p_off[a])] - lo;
-                        u32 w = z >> 5, mk = 1u << (z & 31);
-                        u32 h = tr_hash(w);
-                        while (true) {
-                            u32 k = atomicCAS(&s.key[h], TR_EMPTY, w);
-                            if (k == TR_EMPTY) {
-                                s.used[atomicAdd(&s.nused, 1u)] = (u16)h;
-                                break;
+                    u64 ci[TR_U];
+#pragma unroll
+                    for (int q = 0; q < TR_U; q++) {   // locate TR_U rows per lane ...
+                        u64 r = rb + (u64)q * WAVE + lane;
+                        ci[q] = ~0ull;
+                        if (r < re) {
+                            u32 a = 0, b = WAVE;   // last probe with p_off <= r
+                            while (b - a > 1) {
+                                u32 m = (a + b) >> 1;
+                                if (s.p_off[m] <= r) a = m; else b = m;
                             }
-                            if (k == w) break;
-                            h = (h + 1) & (TR_H - 1);
+                            ci[q] = s.p_start[a] + (r - s.p_off[a]);
                         }
-                        atomicOr(&s.bits[h], mk);
                     }
+                    u32 zs[TR_U];
+#pragma unroll
+                    for (int q = 0; q < TR_U; q++) zs[q] = ci[q] != ~0ull ? csr_dst[ci[q]] : 0u;   // ... gather
+#pragma unroll
+                    for (int q = 0; q < TR_U; q++)   // ... insert
+                        if (ci[q] != ~0ull) tr_insert(s, zs[q] - lo);
                     wave_sync();
                 }
             }
-            cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt, csr_off, &rows_next);
+            cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
+            // the next round's join rows of this segment: Σ deg over its new ends
+            wave_sync();
+            for (u64 i0 = 0; i0 < cnt; i0 += TR_U * WAVE) {
+                u32 zz[TR_U];
+#pragma unroll
+                for (int q = 0; q < TR_U; q++) {
+                    u64 i = i0 + (u64)q * WAVE + lane;
+                    zz[q] = i < cnt ? ny[out0 + i] : TR_EMPTY;
+                }
+#pragma unroll
+                for (int q = 0; q < TR_U; q++)
+                    if (zz[q] != TR_EMPTY) rows_next += deg[zz[q] - lo];
+            }
         }
-        // per-lane partial sums of the next join rows -> one value
 #pragma unroll
         for (int o = WAVE / 2; o > 0; o >>= 1) rows_next += __shfl_xor(rows_next, o, WAVE);
         if (lane == 0) {
@@ -255,13 +312,13 @@ __global__ void u32_to_u64_k(const u32 *in, u64 n, u64 *out) {
 void tc_rows_setup(dlg_ctx *ctx, const Index &ix, Bitmap &F, TcRows &tr) {
     const u64 W = F.W, n = ix.n;
     // CSR of E over the id range (duplicates kept, as the hash index keeps them)
-    DBuf<u32> cnt(ctx, W + 1);
-    fill_u32(ctx, cnt.p, W + 1, 0u);
+    tr.deg.ensure(W + 1);
+    fill_u32(ctx, tr.deg.p, W + 1, 0u);
     if (n)
         DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, csr_hist_k, sgrid(ctx, n), ctx->block_size, 0,
-                   (const u64 *)ix.rows.p, n, F.lo, cnt.p);
+                   (const u64 *)ix.rows.p, n, F.lo, tr.deg.p);
     tr.csr_off.ensure(W + 1);
-    scan_u32_u64(ctx, cnt.p, W + 1, tr.csr_off.p);
+    scan_u32_u64(ctx, tr.deg.p, W + 1, tr.csr_off.p);
     tr.csr_dst.ensure(n ? n : 1);
     if (n)
         DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, csr_dst_k, sgrid(ctx, n), ctx->block_size, 0,
@@ -309,7 +366,8 @@ u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw) {
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 4.0 * tr.rows + 4.0 * tr.nd, tc_rows_k, g, TR_T, 0,
                (const u32 *)tr.dy.p, (const u32 *)tr.seg_x.p, (const u64 *)tr.seg_beg.p,
                (const u32 *)tr.seg_len.p, (const u64 *)tr.seg_out.p, tr.nseg, (const u64 *)tr.csr_off.p,
-               (const u32 *)tr.csr_dst.p, F.words.p, F.lo, F.wpr, tr.ny.p, tr.nseg_len.p, tr.next_rows.p,
+               (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, F.words.p, F.lo, F.wpr, tr.ny.p,
+               tr.nseg_len.p, tr.next_rows.p,
                (unsigned long long *)tr.counters.p);
     // the new ends sit at the old output offsets; the next offsets scan the next rows
     tr.seg_beg.swap(tr.seg_out);
