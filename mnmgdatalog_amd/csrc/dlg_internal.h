@@ -101,6 +101,10 @@ struct dlg_ctx {
     void prof_begin();
     void prof_end(int kid, double bytes);
     void prof_flush();
+    // Algorithmic bytes of a launch known only after it ran (profiling on).
+    void prof_add_bytes(int kid, double bytes) {
+        if (prof) kstat[kid].bytes += bytes;
+    }
     void sync();
     // read `count` u64 scalars from device memory into host (synchronises)
     void read_scalars(const uint64_t *d, int count, uint64_t *h);
@@ -325,11 +329,12 @@ u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symme
 // the ends dy[seg_beg[s] ..][0, seg_len[s]) of source seg_x[s] (one segment
 // per source, fixed for the whole fixpoint; empty ones stay), plus the dense
 // CSR of E over F's id range.  seg_out = exclusive scan of the segments'
-// join rows = where their new ends go; rows = the round's join rows.
+// output bounds = where their new ends go.
 struct TcRows {
     DBuf<u64> csr_off, seg_beg, seg_out, next_rows, counters;
     DBuf<u32> csr_dst, deg, dy, ny, seg_x, seg_len, nseg_len;
-    u64 nd = 0, nseg = 0, rows = 0;
+    u64 nd = 0, nseg = 0, rows = 0;   // rows: the round's output bound (Σ of the segments')
+    u32 dmax = 0;                     // nonzero: every degree <= dmax, bounds = new ends * dmax
     explicit TcRows(dlg_ctx *c)
         : csr_off(c), seg_beg(c), seg_out(c), next_rows(c), counters(c), csr_dst(c), deg(c), dy(c), ny(c),
           seg_x(c), seg_len(c), nseg_len(c) {}

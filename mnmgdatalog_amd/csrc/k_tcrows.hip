@@ -22,6 +22,7 @@
 #include "dlg_internal.h"
 
 #include <algorithm>
+#include <vector>
 
 namespace dlg {
 
@@ -66,6 +67,7 @@ struct TrWave {
 };
 
 constexpr int TR_U = 4;   // words / rows per lane with their loads in flight together
+constexpr u32 TR_DMAX_BOUND = 16;   // degrees up to this: output regions bounded by new ends * max degree
 constexpr int TR_SMALL = 8;   // chunks whose probes all have at most this many matches: lane per probe
 constexpr int TR_FLUSH_SMALL = TR_H - TR_SMALL * WAVE - TR_H / 4;   // keeps the table <= 3/4 full
 static_assert(TR_FLUSH_SMALL > 0 && TR_FLUSH + TR_BATCH <= TR_H * 3 / 4, "table load");
@@ -134,15 +136,19 @@ __device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst) {
 
 // One round.  Segment s (source seg_x[s], ends dy[seg_beg[s] ..][0, seg_len[s]))
 // writes its new ends at ny[seg_out[s] ..] -- seg_out is the exclusive scan
-// of the segments' join rows, which bound their new tuples -- and its next
-// length and next join rows at the same index s.  Workgroups take static
-// ranges of segments; the four waves of a workgroup share the range through
-// an LDS counter, so no global counter is contended.
+// of the segments' next-round bounds -- and its next length and bound at the
+// same index s.  The bound is the segment's next join rows, Σ deg over its
+// new ends (exact; `dmax` = 0), or, when every degree is at most dmax,
+// min(new ends * dmax, W) (no degree lookups; the output has gaps).  The
+// round's join rows are counted on the way (counters[1]).  Workgroups take
+// static ranges of segments; the four waves of a workgroup share the range
+// through an LDS counter, so no global counter is contended.
 __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_x, const u64 *seg_beg,
                                                   const u32 *seg_len, const u64 *seg_out, u64 nseg,
-                                                  const u64 *csr_off, const u32 *csr_dst, const u32 *deg, u32 *F,
-                                                  u32 lo, u64 wpr, u32 *ny, u32 *nseg_len, u64 *next_rows,
-                                                  unsigned long long *new_total) {
+                                                  const u64 *csr_off, const u32 *csr_dst, const u32 *deg,
+                                                  u32 dmax, u32 *F, u32 lo, u64 W, u64 wpr, u32 *ny,
+                                                  u32 *nseg_len, u64 *next_rows,
+                                                  unsigned long long *counters) {
     __shared__ TrWave sw[TR_W];
     __shared__ u32 s_next;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
@@ -156,7 +162,7 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
     const u64 b0 = (u64)blockIdx.x * per, b1 = b0 + per < nseg ? b0 + per : nseg;
     if (threadIdx.x == 0) s_next = 0;
     __syncthreads();
-    u64 wave_new = 0;
+    u64 wave_new = 0, wave_rows = 0;
     while (true) {
         u32 k0 = 0;
         if (lane == 0) k0 = atomicAdd(&s_next, 1u);
@@ -168,79 +174,96 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
             const u32 x = seg_x[sg];
             const u64 beg = seg_beg[sg], out0 = seg_out[sg];
             const u64 rowbase = (u64)(x - lo) * wpr;
-            for (u32 pc = 0; pc < len; pc += WAVE) {
-                // the chunk's probes and their adjacency ranges
-                u64 st = 0;
-                u32 d = 0;
-                if (pc + lane < len) {
-                    u32 y = dy[beg + pc + lane] - lo;
-                    st = csr_off[y];
-                    d = deg[y];
+            for (u32 pc = 0; pc < len; pc += 2 * WAVE) {
+                // two probes per lane and their adjacency ranges
+                u64 st[2] = {0, 0};
+                u32 d[2] = {0, 0};
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    u32 i = pc + h * WAVE + lane;
+                    if (i < len) {
+                        u32 y = dy[beg + i] - lo;
+                        st[h] = csr_off[y];
+                        d[h] = deg[y];
+                    }
                 }
-                u32 maxd = d;
+                wave_rows += d[0] + d[1];
+                u32 maxd = max(d[0], d[1]);
 #pragma unroll
                 for (int o = WAVE / 2; o > 0; o >>= 1) maxd = max(maxd, (u32)__shfl_xor((int)maxd, o, WAVE));
                 if (maxd <= (u32)TR_SMALL) {
-                    // lane per probe: all of its matches gathered at once, then inserted
-                    if (s.nused > (u32)TR_FLUSH_SMALL)   // uniform: every lane reads the same LDS word
-                        cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
-                    u32 zs[TR_SMALL];
+                    // lane per probe: the matches of both probes gathered at once, then inserted
+                    u32 zs[2][TR_SMALL];
 #pragma unroll
-                    for (int i = 0; i < TR_SMALL; i++) zs[i] = (u32)i < d ? csr_dst[st + i] : 0u;
+                    for (int h = 0; h < 2; h++)
 #pragma unroll
-                    for (int i = 0; i < TR_SMALL; i++)
-                        if ((u32)i < d) tr_insert(s, zs[i] - lo);
-                    wave_sync();
+                        for (int i = 0; i < TR_SMALL; i++) zs[h][i] = (u32)i < d[h] ? csr_dst[st[h] + i] : 0u;
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        if (s.nused > (u32)TR_FLUSH_SMALL)   // uniform: every lane reads the same LDS word
+                            cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
+#pragma unroll
+                        for (int i = 0; i < TR_SMALL; i++)
+                            if ((u32)i < d[h]) tr_insert(s, zs[h][i] - lo);
+                        wave_sync();
+                    }
                     continue;
                 }
-                // load-balanced rows: row offsets of the chunk's probes in LDS
-                u64 R;
-                u64 ex = wave_excl_scan(d, &R);
-                wave_sync();
-                s.p_off[lane] = ex;
-                s.p_start[lane] = st;
-                if (lane == 0) s.p_off[WAVE] = R;
-                wave_sync();
-                for (u64 rb = 0; rb < R; rb += TR_BATCH) {
-                    if (s.nused > (u32)TR_FLUSH)   // uniform: every lane reads the same LDS word
-                        cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
-                    const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
-                    u64 ci[TR_U];
-#pragma unroll
-                    for (int q = 0; q < TR_U; q++) {   // locate TR_U rows per lane ...
-                        u64 r = rb + (u64)q * WAVE + lane;
-                        ci[q] = ~0ull;
-                        if (r < re) {
-                            u32 a = 0, b = WAVE;   // last probe with p_off <= r
-                            while (b - a > 1) {
-                                u32 m = (a + b) >> 1;
-                                if (s.p_off[m] <= r) a = m; else b = m;
-                            }
-                            ci[q] = s.p_start[a] + (r - s.p_off[a]);
-                        }
-                    }
-                    u32 zs[TR_U];
-#pragma unroll
-                    for (int q = 0; q < TR_U; q++) zs[q] = ci[q] != ~0ull ? csr_dst[ci[q]] : 0u;   // ... gather
-#pragma unroll
-                    for (int q = 0; q < TR_U; q++)   // ... insert
-                        if (ci[q] != ~0ull) tr_insert(s, zs[q] - lo);
+                // load-balanced rows, one half-chunk of 64 probes at a time
+#pragma unroll 1
+                for (int h = 0; h < 2; h++) {
+                    u64 R;
+                    u64 ex = wave_excl_scan(d[h], &R);
                     wave_sync();
+                    s.p_off[lane] = ex;
+                    s.p_start[lane] = st[h];
+                    if (lane == 0) s.p_off[WAVE] = R;
+                    wave_sync();
+                    for (u64 rb = 0; rb < R; rb += TR_BATCH) {
+                        if (s.nused > (u32)TR_FLUSH)   // uniform: every lane reads the same LDS word
+                            cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
+                        const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
+                        u64 ci[TR_U];
+#pragma unroll
+                        for (int q = 0; q < TR_U; q++) {   // locate TR_U rows per lane ...
+                            u64 r = rb + (u64)q * WAVE + lane;
+                            ci[q] = ~0ull;
+                            if (r < re) {
+                                u32 a = 0, b = WAVE;   // last probe with p_off <= r
+                                while (b - a > 1) {
+                                    u32 m = (a + b) >> 1;
+                                    if (s.p_off[m] <= r) a = m; else b = m;
+                                }
+                                ci[q] = s.p_start[a] + (r - s.p_off[a]);
+                            }
+                        }
+                        u32 zq[TR_U];
+#pragma unroll
+                        for (int q = 0; q < TR_U; q++) zq[q] = ci[q] != ~0ull ? csr_dst[ci[q]] : 0u;   // ... gather
+#pragma unroll
+                        for (int q = 0; q < TR_U; q++)   // ... insert
+                            if (ci[q] != ~0ull) tr_insert(s, zq[q] - lo);
+                        wave_sync();
+                    }
                 }
             }
             cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
-            // the next round's join rows of this segment: Σ deg over its new ends
-            wave_sync();
-            for (u64 i0 = 0; i0 < cnt; i0 += TR_U * WAVE) {
-                u32 zz[TR_U];
+            if (dmax) {
+                rows_next = lane == 0 ? min(cnt * dmax, W) : 0;
+            } else {
+                // the next round's join rows of this segment: Σ deg over its new ends
+                wave_sync();
+                for (u64 i0 = 0; i0 < cnt; i0 += TR_U * WAVE) {
+                    u32 zz[TR_U];
 #pragma unroll
-                for (int q = 0; q < TR_U; q++) {
-                    u64 i = i0 + (u64)q * WAVE + lane;
-                    zz[q] = i < cnt ? ny[out0 + i] : TR_EMPTY;
+                    for (int q = 0; q < TR_U; q++) {
+                        u64 i = i0 + (u64)q * WAVE + lane;
+                        zz[q] = i < cnt ? ny[out0 + i] : TR_EMPTY;
+                    }
+#pragma unroll
+                    for (int q = 0; q < TR_U; q++)
+                        if (zz[q] != TR_EMPTY) rows_next += deg[zz[q] - lo];
                 }
-#pragma unroll
-                for (int q = 0; q < TR_U; q++)
-                    if (zz[q] != TR_EMPTY) rows_next += deg[zz[q] - lo];
             }
         }
 #pragma unroll
@@ -251,7 +274,10 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
         }
         wave_new += cnt;
     }
-    if (lane == 0 && wave_new) atomicAdd(new_total, (unsigned long long)wave_new);
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) wave_rows += __shfl_xor(wave_rows, o, WAVE);
+    if (lane == 0 && wave_new) atomicAdd(counters, (unsigned long long)wave_new);
+    if (lane == 0 && wave_rows) atomicAdd(counters + 2, (unsigned long long)wave_rows);
 }
 
 // Σ deg over each segment's ends (the first round's join rows per segment).
@@ -319,6 +345,16 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, Bitmap &F, TcRows &tr) {
                    (const u64 *)ix.rows.p, n, F.lo, tr.deg.p);
     tr.csr_off.ensure(W + 1);
     scan_u32_u64(ctx, tr.deg.p, W + 1, tr.csr_off.p);
+    // the largest degree: small enough, and the next rounds' output regions
+    // are bounded by it instead of by exact degree sums
+    {
+        std::vector<u32> hd(W);
+        HIPCHK(hipMemcpyAsync(hd.data(), tr.deg.p, sizeof(u32) * W, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        u32 m = 0;
+        for (u32 v : hd) m = std::max(m, v);
+        tr.dmax = m <= TR_DMAX_BOUND ? std::max<u32>(m, 1) : 0;
+    }
     tr.csr_dst.ensure(n ? n : 1);
     if (n)
         DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, csr_dst_k, sgrid(ctx, n), ctx->block_size, 0,
@@ -357,27 +393,27 @@ u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw) {
         tr.nd = 0;
         return 0;
     }
-    if (raw) *raw += tr.rows;
     tr.ny.ensure(tr.rows);
-    fill_u64(ctx, tr.counters.p, 2, 0ull);
+    fill_u64(ctx, tr.counters.p, 3, 0ull);
     unsigned g = (unsigned)std::min<u64>((tr.nseg + TR_W - 1) / TR_W, (u64)ctx->num_cus * 3);
     // per join row: the CSR end (4 B) and an LDS insert; per touched word an
     // L2 atomic; per new tuple its end (4 B) and two CSR offsets
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 4.0 * tr.rows + 4.0 * tr.nd, tc_rows_k, g, TR_T, 0,
                (const u32 *)tr.dy.p, (const u32 *)tr.seg_x.p, (const u64 *)tr.seg_beg.p,
                (const u32 *)tr.seg_len.p, (const u64 *)tr.seg_out.p, tr.nseg, (const u64 *)tr.csr_off.p,
-               (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, F.words.p, F.lo, F.wpr, tr.ny.p,
+               (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, tr.dmax, F.words.p, F.lo, F.W, F.wpr, tr.ny.p,
                tr.nseg_len.p, tr.next_rows.p,
                (unsigned long long *)tr.counters.p);
     // the new ends sit at the old output offsets; the next offsets scan the next rows
     tr.seg_beg.swap(tr.seg_out);
     scan_u64_async(ctx, tr.next_rows.p, tr.nseg, tr.seg_out.p, tr.counters.p + 1);
-    u64 h[2];
-    ctx->read_scalars(tr.counters.p, 2, h);
+    u64 h[3];
+    ctx->read_scalars(tr.counters.p, 3, h);
     tr.dy.swap(tr.ny);
     tr.seg_len.swap(tr.nseg_len);
     tr.nd = h[0];
     tr.rows = h[1];
+    if (raw) *raw += h[2];
     return h[0];
 }
 
