@@ -396,9 +396,10 @@ u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw) {
     tr.ny.ensure(tr.rows);
     fill_u64(ctx, tr.counters.p, 3, 0ull);
     unsigned g = (unsigned)std::min<u64>((tr.nseg + TR_W - 1) / TR_W, (u64)ctx->num_cus * 3);
-    // per join row: the CSR end (4 B) and an LDS insert; per touched word an
-    // L2 atomic; per new tuple its end (4 B) and two CSR offsets
-    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 4.0 * tr.rows + 4.0 * tr.nd, tc_rows_k, g, TR_T, 0,
+    // algorithmic bytes, added once the round's join rows are known: 4 B per
+    // join row (its matched end) + 20 B per probe (its end, CSR offset, degree)
+    const u64 nd0 = tr.nd;
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, tc_rows_k, g, TR_T, 0,
                (const u32 *)tr.dy.p, (const u32 *)tr.seg_x.p, (const u64 *)tr.seg_beg.p,
                (const u32 *)tr.seg_len.p, (const u64 *)tr.seg_out.p, tr.nseg, (const u64 *)tr.csr_off.p,
                (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, tr.dmax, F.words.p, F.lo, F.W, F.wpr, tr.ny.p,
@@ -414,6 +415,7 @@ u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw) {
     tr.nd = h[0];
     tr.rows = h[1];
     if (raw) *raw += h[2];
+    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[2] + 20.0 * nd0);
     return h[0];
 }
 
