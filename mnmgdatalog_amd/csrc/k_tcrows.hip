@@ -16,7 +16,7 @@
 //     chip); kernel boundaries make the rows coherent between rounds;
 //   4 emit: the bits that were clear are the segment's new tuples, written
 //     contiguously as the next round's segment for x, into a region reserved
-//     by a scan of the segments' join-row counts (no global counter; a
+//     by a scan of the segments' output bounds (no global counter; a
 //     segment whose touched words overflow the table flushes several times
 //     into its region).
 #include "dlg_internal.h"
@@ -140,7 +140,7 @@ __device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst) {
 // same index s.  The bound is the segment's next join rows, Σ deg over its
 // new ends (exact; `dmax` = 0), or, when every degree is at most dmax,
 // min(new ends * dmax, W) (no degree lookups; the output has gaps).  The
-// round's join rows are counted on the way (counters[1]).  Workgroups take
+// round's join rows are counted on the way (counters[2]).  Workgroups take
 // static ranges of segments; the four waves of a workgroup share the range
 // through an LDS counter, so no global counter is contended.
 __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_x, const u64 *seg_beg,
