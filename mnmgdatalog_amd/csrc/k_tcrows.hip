@@ -174,18 +174,26 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
             const u32 x = seg_x[sg];
             const u64 beg = seg_beg[sg], out0 = seg_out[sg];
             const u64 rowbase = (u64)(x - lo) * wpr;
+            // the chunks' ends are loaded one chunk ahead (software pipelining)
+            u32 yn[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) yn[h] = (u32)(h * WAVE + lane) < len ? dy[beg + h * WAVE + lane] : 0u;
             for (u32 pc = 0; pc < len; pc += 2 * WAVE) {
                 // two probes per lane and their adjacency ranges
                 u64 st[2] = {0, 0};
                 u32 d[2] = {0, 0};
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    u32 i = pc + h * WAVE + lane;
-                    if (i < len) {
-                        u32 y = dy[beg + i] - lo;
+                    if (pc + h * WAVE + lane < len) {
+                        u32 y = yn[h] - lo;
                         st[h] = csr_off[y];
                         d[h] = deg[y];
                     }
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    u32 i = pc + 2 * WAVE + h * WAVE + lane;
+                    yn[h] = i < len ? dy[beg + i] : 0u;
                 }
                 wave_rows += d[0] + d[1];
                 u32 maxd = max(d[0], d[1]);
