@@ -7,9 +7,12 @@
 // union's relations are disjoint unions of the instances' relations, and in
 // every sorted relation an instance occupies one contiguous key range.
 // Per-round accounting:
-//   TC / SG   the round's new tuples N are sorted; 2 binary searches per
-//             instance boundary give |N_i|; an instance's iteration count is
-//             the first round with |N_i| = 0 (its plain run stops there).
+//   TC / SG   2 binary searches per instance boundary in the round's new
+//             tuples N give |N_i| when N is sorted, else a histogram of N's
+//             keys over the instance ranges (LDS-privatised per workgroup);
+//             an instance's iteration count is the first round with
+//             |N_i| = 0 (its plain run stops there).  TC's bitmap layout
+//             keeps one map per instance range (block-diagonal).
 //   WCC       frontier nodes are counted per instance (dense node ids are
 //             sorted by id, so instances are contiguous node ranges); each
 //             instance applies the reference's stop rule (two consecutive
@@ -60,16 +63,30 @@ __device__ __forceinline__ u32 inst_of(const u32 *vstart, u32 B, u32 v) {
 
 constexpr int MAX_LDS_INST = 4096;
 
+// Per-instance counts of frontier nodes: an LDS histogram per workgroup when
+// the instances fit, flushed with one device atomic per nonzero bin.
 __global__ void frontier_hist_k(const u32 *node, u64 n, const u32 *vstart, u32 B, u32 *cnt) {
     __shared__ u32 vs[MAX_LDS_INST + 1];
+    __shared__ u32 s_cnt[MAX_LDS_INST];
     const bool lds = B <= MAX_LDS_INST;
     if (lds)
-        for (u32 i = threadIdx.x; i <= B; i += blockDim.x) vs[i] = vstart[i];
+        for (u32 i = threadIdx.x; i <= B; i += blockDim.x) {
+            vs[i] = vstart[i];
+            if (i < B) s_cnt[i] = 0u;
+        }
     __syncthreads();
     const u32 *tab = lds ? vs : vstart;
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     u64 stride = (u64)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) atomicAdd(&cnt[inst_of(tab, B, node[i])], 1u);
+    for (; i < n; i += stride) {
+        u32 b = inst_of(tab, B, node[i]);
+        if (lds) atomicAdd(&s_cnt[b], 1u);
+        else atomicAdd(&cnt[b], 1u);
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (u32 j = threadIdx.x; j < B; j += blockDim.x)
+        if (s_cnt[j]) atomicAdd(&cnt[j], s_cnt[j]);
 }
 
 // Keep frontier entries of live instances (order is irrelevant: Δ is a set).
@@ -182,11 +199,18 @@ static std::vector<u64> boundaries(dlg_ctx *ctx, const u64 *a, u64 n, const DBuf
 }
 
 // Per-instance count of unordered tuples: instance = last i with ub[i] <= key.
+// Per-instance counts of the keys of a: the instance by binary search over
+// the range starts; counted in an LDS histogram per workgroup (when the
+// instances fit) and flushed with one device atomic per nonzero bin.
 __global__ void key_hist_k(const u64 *a, u64 n, const u32 *ub, u32 B, u32 *cnt) {
     __shared__ u32 s_ub[MAX_LDS_INST + 1];
+    __shared__ u32 s_cnt[MAX_LDS_INST];
     const bool lds = B <= MAX_LDS_INST;
     if (lds)
-        for (u32 i = threadIdx.x; i <= B; i += blockDim.x) s_ub[i] = ub[i];
+        for (u32 i = threadIdx.x; i <= B; i += blockDim.x) {
+            s_ub[i] = ub[i];
+            if (i < B) s_cnt[i] = 0u;
+        }
     __syncthreads();
     const u32 *tab = lds ? s_ub : ub;
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -198,8 +222,13 @@ __global__ void key_hist_k(const u64 *a, u64 n, const u32 *ub, u32 B, u32 *cnt) 
             u32 mid = (lo + hi) >> 1;
             if (tab[mid] <= k) lo = mid; else hi = mid;
         }
-        atomicAdd(&cnt[lo], 1u);
+        if (lds) atomicAdd(&s_cnt[lo], 1u);
+        else atomicAdd(&cnt[lo], 1u);
     }
+    if (!lds) return;
+    __syncthreads();
+    for (u32 j = threadIdx.x; j < B; j += blockDim.x)
+        if (s_cnt[j]) atomicAdd(&cnt[j], s_cnt[j]);
 }
 
 struct BatchHook : RoundHook {
@@ -276,14 +305,14 @@ void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const 
     if (prog == DLG_PROG_TC || prog == DLG_PROG_SG || prog == DLG_PROG_SJ) {
         FixpointOut f(ctx);
         BatchHook hook(ctx, ub, B);
-        if (prog == DLG_PROG_TC) tc_fixpoint(ctx, d_rows.p, N, o, pc, &hook, f);
+        if (prog == DLG_PROG_TC) tc_fixpoint(ctx, d_rows.p, N, o, pc, &hook, f, &L.ub);
         else if (prog == DLG_PROG_SG) sg_fixpoint(ctx, d_rows.p, N, o, pc, &hook, f);
         else sj_fixpoint(ctx, d_rows.p, N, o, pc, f);
         rounds = f.iterations;
         std::vector<u64> b = boundaries(ctx, f.full.p, f.nfull, ub, B, hook.pos);
         ctx->sync();
         st->device_seconds = secs(t0, Clock::now());
-        all.resize(2 * f.nfull + 2);
+        if (o.keep) all.resize(2 * f.nfull + 2);   // host rows only when asked for
         if (o.keep && f.nfull)
             download_rows(ctx, f.full.p, f.nfull, prog == DLG_PROG_SG ? DLG_ORIENT_KV : DLG_ORIENT_VK,
                           all.data());
@@ -381,7 +410,7 @@ void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const 
         HIPCHK(hipMemcpyAsync(hl.data(), largest.p, 4 * B, hipMemcpyDeviceToHost, ctx->stream));
         ctx->sync();
         st->device_seconds = secs(t0, Clock::now());
-        all.resize(2 * ncomp + 2);
+        if (o.keep) all.resize(2 * ncomp + 2);
         if (o.keep && ncomp) download_rows(ctx, out.p, ncomp, DLG_ORIENT_KV, all.data());
         split_rows(all, b, L, o.keep, res);
         for (int i = 0; i < B; i++) {

@@ -324,7 +324,37 @@ void bitmap_clear_list(dlg_ctx *ctx, const u64 *in, u64 n, Bitmap &bm);
 // lies.
 u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symmetric);
 
+// A W-row block of a map, emitted in tuple order at out through a padded
+// scratch copy (transposed back when `transposed`).  Returns its set bits.
+u64 bitmap_emit_block(dlg_ctx *ctx, const u32 *words, u32 lo, u64 W, u64 wpr, bool transposed, DBuf<u32> &scratch,
+                      u64 *out);
+
 // ------------------------------------------------------------- TC rows (k_tcrows.hip)
+// The full TC relation as a block-diagonal set of transposed bitmaps: one
+// block per id range [starts[i], starts[i+1]) (one block for a plain run,
+// one per instance in the batched form, whose instances own disjoint id
+// ranges); row x of a block holds the ends z of the paths from x, columns
+// relative to the block's first id.  row_base / col_lo give each id's row
+// and its block's first id.
+struct RowMap {
+    struct Block {
+        u32 lo;
+        u64 W, wpr, base;   // ids, words per row, first word
+    };
+    DBuf<u32> words;
+    DBuf<u64> row_base;
+    DBuf<u32> col_lo;
+    std::vector<Block> blocks;
+    u32 lo = 0;
+    u64 W = 0, nwords = 0;
+    explicit RowMap(dlg_ctx *c) : words(c), row_base(c), col_lo(c) {}
+};
+// Bits a RowMap over [lo, hi] with the given block starts (null: one block) needs.
+u64 rowmap_bits(u32 lo, u32 hi, const std::vector<u32> *starts);
+void rowmap_init(dlg_ctx *ctx, RowMap &m, u32 lo, u32 hi, const std::vector<u32> *starts);
+// All bits in tuple order (blocks in id order, each transposed back).
+u64 rowmap_emit(dlg_ctx *ctx, RowMap &m, DBuf<u64> &out, u64 expect);
+
 // TC over a transposed bitmap F with Δ grouped by source: segment s holds
 // the ends dy[seg_beg[s] ..][0, seg_len[s]) of source seg_x[s] (one segment
 // per source, fixed for the whole fixpoint; empty ones stay), plus the dense
@@ -335,15 +365,16 @@ struct TcRows {
     DBuf<u32> csr_dst, deg, dy, ny, seg_x, seg_len, nseg_len;
     u64 nd = 0, nseg = 0, rows = 0;   // rows: the round's output bound (Σ of the segments')
     u32 dmax = 0;                     // nonzero: every degree <= dmax, bounds = new ends * dmax
+    u64 wcap = 0;                     // the widest block: no segment has more new ends
     explicit TcRows(dlg_ctx *c)
         : csr_off(c), seg_beg(c), seg_out(c), next_rows(c), counters(c), csr_dst(c), deg(c), dy(c), ny(c),
           seg_x(c), seg_len(c), nseg_len(c) {}
 };
 // CSR of the index's rows; Δ0 = unique(E reversed) grouped by source, its
-// bits set in F (F must be transposed and empty).
-void tc_rows_setup(dlg_ctx *ctx, const Index &ix, Bitmap &F, TcRows &tr);
+// bits set in F (F empty).
+void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr);
 // One round: N = (Δ ⋈ E) \ F into F, Δ = N (grouped).  Returns |N|; *raw += |Δ ⋈ E|.
-u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw);
+u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw);
 // The current Δ as (z, x) tuples.
 void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out);
 

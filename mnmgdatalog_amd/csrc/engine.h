@@ -70,8 +70,10 @@ struct FixpointOut {
 };
 
 // Relations from int32 device rows (n rows).  `hook` may be null.
+// `blocks` (batched form): packed id-range starts of the instances, plus the
+// end; tuples never pair ids of two ranges.
 void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
-                 RoundHook *hook, FixpointOut &out);
+                 RoundHook *hook, FixpointOut &out, const std::vector<u32> *blocks = nullptr);
 void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
                  RoundHook *hook, FixpointOut &out);
 void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,

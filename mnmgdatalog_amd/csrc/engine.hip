@@ -253,8 +253,24 @@ static u64 new_tuples(dlg_ctx *ctx, DBuf<u64> &J, u64 nj, DBuf<u64> &tmp, DBuf<u
 // N = unique(J) \ F.  Adding Δ to F after its join is the reference's
 // "F = F ∪ N" of the previous round (tc.cu:268-286) done one step later,
 // so every set difference sees the same F as the reference's.
+// TC's bitmap layout: one transposed map per id range (one for a plain run).
+static bool tc_use_rows(const Opts &o, u64 n, const u32 *ib, const std::vector<u32> *blocks) {
+    if (n == 0 || (o.strategy != 0 && o.strategy != 4)) return false;
+    u64 bits = rowmap_bits(ib[0], ib[1], blocks), wmax = (u64)(ib[1] - ib[0]) + 1;
+    if (blocks) {
+        wmax = 0;
+        for (size_t i = 0; i + 1 < blocks->size(); i++)
+            wmax = std::max<u64>(wmax, (u64)(*blocks)[i + 1] - (*blocks)[i]);
+    }
+    if (o.strategy == 4) {
+        if (wmax > BITMAP_FORCED_MAX_W) fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
+        return true;
+    }
+    return wmax <= BITMAP_MAX_W && bits <= BITMAP_MAX_W * BITMAP_MAX_W;
+}
+
 void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
-                 RoundHook *hook, FixpointOut &out) {
+                 RoundHook *hook, FixpointOut &out, const std::vector<u32> *blocks) {
     pc.mark(DLG_PH_INIT);
     DBuf<u64> edge(ctx, n), delta(ctx, n), J(ctx, n), N(ctx), tmp(ctx);
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, edge.p);   // e(src, dst) keyed by src
@@ -267,11 +283,11 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     edge.release();
     JoinScratch js(ctx);
     int it = 0;
-    if (use_bitmap(o, n, ib)) {
-        // F as a transposed bitmap (row = source) and Δ grouped by source: one
-        // workgroup per source row and round (k_tcrows.hip).
-        Bitmap F(ctx);
-        bitmap_init(ctx, F, ib[0], ib[1], true);
+    if (tc_use_rows(o, n, ib, blocks)) {
+        // F as transposed bitmaps (row = source; one per id range) and Δ grouped
+        // by source: one wave per source row and round (k_tcrows.hip).
+        RowMap F(ctx);
+        rowmap_init(ctx, F, ib[0], ib[1], blocks);
         J.release();
         TcRows tr(ctx);
         pc.mark(DLG_PH_DEDUP);
@@ -292,7 +308,7 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         pc.mark(DLG_PH_FINAL);
         out.raw += (i64)raw;
         out.nfull = total;
-        bitmap_emit(ctx, F, out.full, total, false);
+        rowmap_emit(ctx, F, out.full, total);
         out.iterations = it;
         return;
     }

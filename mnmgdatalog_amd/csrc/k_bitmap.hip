@@ -478,10 +478,10 @@ void bitmap_clear_list(dlg_ctx *ctx, const u64 *in, u64 n, Bitmap &bm) {
                    bm.words.p, bm.lo, bm.S);
 }
 
-// Count pass + block scan over a map's words; returns the number of set bits
-// (synchronises).
-static u64 bitmap_count(dlg_ctx *ctx, const Bitmap &bm, const u32 *words, DBuf<u64> &bsum, u64 &nb) {
-    nb = (bm.nwords + BE_TILE - 1) / BE_TILE;
+// Count pass + block scan over nwords words (padded to whole BE_TILEs);
+// returns the number of set bits (synchronises).
+static u64 words_count(dlg_ctx *ctx, const u32 *words, u64 nwords, DBuf<u64> &bsum, u64 &nb) {
+    nb = (nwords + BE_TILE - 1) / BE_TILE;
     bsum.ensure(nb);
     DBuf<u64> tot(ctx, 1);
     DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * nb * BE_TILE, bm_count_k, (unsigned)nb, BE_T, 0, words, bsum.p);
@@ -492,20 +492,20 @@ static u64 bitmap_count(dlg_ctx *ctx, const Bitmap &bm, const u32 *words, DBuf<u
 }
 
 template <bool SWAP, bool CLEAR>
-static void emit_pass(dlg_ctx *ctx, const Bitmap &bm, u32 *words, const DBuf<u64> &bsum, u64 nb, u64 h,
+static void emit_pass(dlg_ctx *ctx, u32 lo, u64 wpr, u32 *words, const DBuf<u64> &bsum, u64 nb, u64 h,
                       u64 *out) {
     DLG_LAUNCH(ctx, DLG_K_MISC, (CLEAR ? 8.0 : 4.0) * nb * BE_TILE + 8.0 * h, (bm_emit_k<SWAP, CLEAR>),
-               (unsigned)nb, BE_T, 0, words, (const u64 *)bsum.p, bm.lo, bm.wpr, out);
+               (unsigned)nb, BE_T, 0, words, (const u64 *)bsum.p, lo, wpr, out);
 }
 
 u64 bitmap_drain(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, bool swap) {
     DBuf<u64> bsum(ctx);
     u64 nb = 0;
-    u64 h = bitmap_count(ctx, bm, bm.words.p, bsum, nb);
+    u64 h = words_count(ctx, bm.words.p, bm.nwords, bsum, nb);
     out.ensure(h ? h : 1);
     if (h == 0) return 0;
-    if (swap) emit_pass<true, true>(ctx, bm, bm.words.p, bsum, nb, h, out.p);
-    else emit_pass<false, true>(ctx, bm, bm.words.p, bsum, nb, h, out.p);
+    if (swap) emit_pass<true, true>(ctx, bm.lo, bm.wpr, bm.words.p, bsum, nb, h, out.p);
+    else emit_pass<false, true>(ctx, bm.lo, bm.wpr, bm.words.p, bsum, nb, h, out.p);
     return h;
 }
 
@@ -532,25 +532,43 @@ __global__ void bm_transpose_k(const u32 *src, u32 *dst, u64 W, u64 wpr) {
     if (c < W) dst[c * wpr + R] = x;
 }
 
-u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symmetric) {
-    out.ensure(expect ? expect : 1);
-    if (expect == 0) return 0;
-    u32 *words = bm.words.p;
-    DBuf<u32> tw(ctx);
-    if (bm.tr && !symmetric) {   // back to row-major, so that bit order is tuple order
-        u64 padded = (bm.nwords + BE_TILE - 1) / BE_TILE * BE_TILE;
-        tw.ensure(padded);
-        fill_u32(ctx, tw.p + bm.nwords, padded - bm.nwords, 0u);
-        u64 threads = bm.wpr * bm.wpr * 32;
-        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * bm.nwords, bm_transpose_k, grid_for(threads, 256), 256, 0,
-                   (const u32 *)bm.words.p, tw.p, bm.W, bm.wpr);
-        words = tw.p;
+// A W-row block of wpr words per row, transposed (or copied) into a padded
+// scratch map and emitted in tuple order at out.  Returns the set bits.
+u64 bitmap_emit_block(dlg_ctx *ctx, const u32 *words, u32 lo, u64 W, u64 wpr, bool transposed, DBuf<u32> &scratch,
+                      u64 *out) {
+    const u64 nwords = W * wpr;
+    if (nwords == 0) return 0;
+    const u64 padded = (nwords + BE_TILE - 1) / BE_TILE * BE_TILE;
+    scratch.ensure(padded);
+    fill_u32(ctx, scratch.p + nwords, padded - nwords, 0u);
+    if (transposed) {
+        u64 threads = wpr * wpr * 32;
+        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, bm_transpose_k, grid_for(threads, 256), 256, 0, words,
+                   scratch.p, W, wpr);
+    } else {
+        HIPCHK(hipMemcpyAsync(scratch.p, words, 4 * nwords, hipMemcpyDeviceToDevice, ctx->stream));
     }
     DBuf<u64> bsum(ctx);
     u64 nb = 0;
-    u64 h = bitmap_count(ctx, bm, words, bsum, nb);
+    u64 h = words_count(ctx, scratch.p, nwords, bsum, nb);
+    if (h) emit_pass<false, false>(ctx, lo, wpr, scratch.p, bsum, nb, h, out);
+    return h;
+}
+
+u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symmetric) {
+    out.ensure(expect ? expect : 1);
+    if (expect == 0) return 0;
+    DBuf<u64> bsum(ctx);
+    u64 h;
+    if (bm.tr && !symmetric) {   // back to row-major, so that bit order is tuple order
+        DBuf<u32> tw(ctx);
+        h = bitmap_emit_block(ctx, bm.words.p, bm.lo, bm.W, bm.wpr, true, tw, out.p);
+    } else {
+        u64 nb = 0;
+        h = words_count(ctx, bm.words.p, bm.nwords, bsum, nb);
+        if (h == expect) emit_pass<false, false>(ctx, bm.lo, bm.wpr, bm.words.p, bsum, nb, h, out.p);
+    }
     if (h != expect) fail(DLG_DEVICE_ERROR, "bitmap emission count mismatch");
-    emit_pass<false, false>(ctx, bm, words, bsum, nb, h, out.p);
     return h;
 }
 

@@ -146,8 +146,8 @@ __device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst) {
 __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_x, const u64 *seg_beg,
                                                   const u32 *seg_len, const u64 *seg_out, u64 nseg,
                                                   const u64 *csr_off, const u32 *csr_dst, const u32 *deg,
-                                                  u32 dmax, u32 *F, u32 lo, u64 W, u64 wpr, u32 *ny,
-                                                  u32 *nseg_len, u64 *next_rows,
+                                                  u32 dmax, u32 *F, const u64 *row_base, const u32 *col_lo,
+                                                  u32 lo, u64 wcap, u32 *ny, u32 *nseg_len, u64 *next_rows,
                                                   unsigned long long *counters) {
     __shared__ TrWave sw[TR_W];
     __shared__ u32 s_next;
@@ -173,7 +173,8 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
         if (len) {
             const u32 x = seg_x[sg];
             const u64 beg = seg_beg[sg], out0 = seg_out[sg];
-            const u64 rowbase = (u64)(x - lo) * wpr;
+            const u64 rowbase = row_base[x - lo];   // x's row and its block's first id
+            const u32 clo = col_lo[x - lo];
             // the chunks' ends are loaded one chunk ahead (software pipelining)
             u32 yn[2];
 #pragma unroll
@@ -209,10 +210,10 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
 #pragma unroll
                     for (int h = 0; h < 2; h++) {
                         if (s.nused > (u32)TR_FLUSH_SMALL)   // uniform: every lane reads the same LDS word
-                            cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
+                            cnt += tr_flush(s, F, rowbase, clo, ny + out0 + cnt);
 #pragma unroll
                         for (int i = 0; i < TR_SMALL; i++)
-                            if ((u32)i < d[h]) tr_insert(s, zs[h][i] - lo);
+                            if ((u32)i < d[h]) tr_insert(s, zs[h][i] - clo);
                         wave_sync();
                     }
                     continue;
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
                     wave_sync();
                     for (u64 rb = 0; rb < R; rb += TR_BATCH) {
                         if (s.nused > (u32)TR_FLUSH)   // uniform: every lane reads the same LDS word
-                            cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
+                            cnt += tr_flush(s, F, rowbase, clo, ny + out0 + cnt);
                         const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
                         u64 ci[TR_U];
 #pragma unroll
@@ -250,14 +251,14 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
                         for (int q = 0; q < TR_U; q++) zq[q] = ci[q] != ~0ull ? csr_dst[ci[q]] : 0u;   // ... gather
 #pragma unroll
                         for (int q = 0; q < TR_U; q++)   // ... insert
-                            if (ci[q] != ~0ull) tr_insert(s, zq[q] - lo);
+                            if (ci[q] != ~0ull) tr_insert(s, zq[q] - clo);
                         wave_sync();
                     }
                 }
             }
-            cnt += tr_flush(s, F, rowbase, lo, ny + out0 + cnt);
+            cnt += tr_flush(s, F, rowbase, clo, ny + out0 + cnt);
             if (dmax) {
-                rows_next = lane == 0 ? min(cnt * dmax, W) : 0;
+                rows_next = lane == 0 ? min(cnt * dmax, wcap) : 0;
             } else {
                 // the next round's join rows of this segment: Σ deg over its new ends
                 wave_sync();
@@ -315,13 +316,14 @@ __global__ void csr_dst_k(const u64 *rows, u64 n, u32 *dst) {
 // Δ0 from the unique sorted edges (src, dst): ends = dst, one segment per src
 // run (heads = run starts), and the edges' bits (row src, column dst) in F.
 __global__ void tc_rows_init_k(const u64 *uniq, u64 n, const u32 *heads, u64 nh, u32 *dy, u32 *seg_x,
-                               u64 *seg_beg, u32 *seg_len, u32 *F, u32 lo, u64 wpr) {
+                               u64 *seg_beg, u32 *seg_len, u32 *F, const u64 *row_base, const u32 *col_lo,
+                               u32 lo) {
     u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         u64 t = uniq[i];
-        u32 sx = tkey(t) - lo, dz = tval(t) - lo;
+        u32 sx = tkey(t) - lo, dz = tval(t) - col_lo[sx];
         dy[i] = tval(t);
-        atomicOr(F + (u64)sx * wpr + (dz >> 5), 1u << (dz & 31));
+        atomicOr(F + row_base[sx] + (dz >> 5), 1u << (dz & 31));
     }
     for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += stride) {
         u64 b = heads[j], e = j + 1 < nh ? heads[j + 1] : n;
@@ -343,7 +345,7 @@ __global__ void u32_to_u64_k(const u32 *in, u64 n, u64 *out) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = in[i];
 }
 
-void tc_rows_setup(dlg_ctx *ctx, const Index &ix, Bitmap &F, TcRows &tr) {
+void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
     const u64 W = F.W, n = ix.n;
     // CSR of E over the id range (duplicates kept, as the hash index keeps them)
     tr.deg.ensure(W + 1);
@@ -383,10 +385,12 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, Bitmap &F, TcRows &tr) {
     tr.nd = nu;
     tr.nseg = nh;
     tr.rows = 0;
+    tr.wcap = 0;
+    for (const auto &b : F.blocks) tr.wcap = std::max<u64>(tr.wcap, b.W);
     if (!nu) return;
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 20.0 * nu, tc_rows_init_k, sgrid(ctx, nu), ctx->block_size, 0,
                (const u64 *)uniq.p, nu, (const u32 *)heads.p, nh, tr.dy.p, tr.seg_x.p, tr.seg_beg.p,
-               tr.seg_len.p, F.words.p, F.lo, F.wpr);
+               tr.seg_len.p, F.words.p, (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo);
     DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * nu, tc_rows_segrows_k, sgrid(ctx, nh), ctx->block_size, 0,
                (const u32 *)tr.dy.p, (const u64 *)tr.seg_beg.p, (const u32 *)tr.seg_len.p, nh,
                (const u64 *)tr.csr_off.p, F.lo, tr.seg_out.p);
@@ -396,7 +400,7 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, Bitmap &F, TcRows &tr) {
     tr.rows = h[1];
 }
 
-u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw) {
+u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     if (tr.nd == 0 || tr.rows == 0) {
         tr.nd = 0;
         return 0;
@@ -410,7 +414,8 @@ u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw) {
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, tc_rows_k, g, TR_T, 0,
                (const u32 *)tr.dy.p, (const u32 *)tr.seg_x.p, (const u64 *)tr.seg_beg.p,
                (const u32 *)tr.seg_len.p, (const u64 *)tr.seg_out.p, tr.nseg, (const u64 *)tr.csr_off.p,
-               (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, tr.dmax, F.words.p, F.lo, F.W, F.wpr, tr.ny.p,
+               (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, tr.dmax, F.words.p, (const u64 *)F.row_base.p,
+               (const u32 *)F.col_lo.p, F.lo, tr.wcap, tr.ny.p,
                tr.nseg_len.p, tr.next_rows.p,
                (unsigned long long *)tr.counters.p);
     // the new ends sit at the old output offsets; the next offsets scan the next rows
@@ -425,6 +430,63 @@ u64 tc_rows_round(dlg_ctx *ctx, Bitmap &F, TcRows &tr, u64 *raw) {
     if (raw) *raw += h[2];
     ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[2] + 20.0 * nd0);
     return h[0];
+}
+
+u64 rowmap_bits(u32 lo, u32 hi, const std::vector<u32> *starts) {
+    if (!starts) {
+        u64 W = (u64)(hi - lo) + 1;
+        return W * ((W + 31) / 32) * 32;
+    }
+    u64 bits = 0;
+    for (size_t i = 0; i + 1 < starts->size(); i++) {
+        u64 e = (*starts)[i + 1] == 0xffffffffu && i + 2 == starts->size() ? 0x100000000ull : (*starts)[i + 1];
+        u64 b0 = std::max<u64>((*starts)[i], lo), b1 = std::min<u64>(e, (u64)hi + 1);
+        if (b1 > b0) bits += (b1 - b0) * ((b1 - b0 + 31) / 32) * 32;
+    }
+    return bits;
+}
+
+void rowmap_init(dlg_ctx *ctx, RowMap &m, u32 lo, u32 hi, const std::vector<u32> *starts) {
+    m.lo = lo;
+    m.W = (u64)(hi - lo) + 1;
+    m.blocks.clear();
+    std::vector<u64> st;
+    if (starts) st.assign(starts->begin(), starts->end());
+    else st = {lo, (u64)hi + 1};
+    if (st.back() == 0xffffffffull) st.back() = 0x100000000ull;   // a range that ends at the last id
+    // block i covers the ids of [st[i], st[i+1]) that fall in [lo, hi]
+    std::vector<u64> hb(m.W);
+    std::vector<u32> hl(m.W);
+    u64 base = 0;
+    for (size_t i = 0; i + 1 < st.size(); i++) {
+        u64 b0 = std::max<u64>(st[i], lo), b1 = std::min<u64>(st[i + 1], (u64)hi + 1);
+        if (b1 <= b0) continue;
+        RowMap::Block b{(u32)b0, b1 - b0, (b1 - b0 + 31) / 32, base};
+        for (u64 r = 0; r < b.W; r++) {
+            hb[b0 - lo + r] = base + r * b.wpr;
+            hl[b0 - lo + r] = b.lo;
+        }
+        base += b.W * b.wpr;
+        m.blocks.push_back(b);
+    }
+    m.nwords = base;
+    m.words.ensure(m.nwords ? m.nwords : 1);
+    fill_u32(ctx, m.words.p, m.nwords ? m.nwords : 1, 0u);
+    m.row_base.ensure(m.W);
+    m.col_lo.ensure(m.W);
+    HIPCHK(hipMemcpyAsync(m.row_base.p, hb.data(), 8 * m.W, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(m.col_lo.p, hl.data(), 4 * m.W, hipMemcpyHostToDevice, ctx->stream));
+    ctx->sync();   // the host arrays go out of scope
+}
+
+u64 rowmap_emit(dlg_ctx *ctx, RowMap &m, DBuf<u64> &out, u64 expect) {
+    out.ensure(expect ? expect : 1);
+    DBuf<u32> scratch(ctx);
+    u64 pos = 0;
+    for (const auto &b : m.blocks)
+        pos += bitmap_emit_block(ctx, m.words.p + b.base, b.lo, b.W, b.wpr, true, scratch, out.p + pos);
+    if (pos != expect) fail(DLG_DEVICE_ERROR, "bitmap emission count mismatch");
+    return pos;
 }
 
 void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out) {
