@@ -340,31 +340,63 @@ __global__ __launch_bounds__(BE_T) void bm_count_k(const u32 *words, u64 *bsum) 
 }
 
 // SWAP: write (column, row) instead of (row, column).  CLEAR: zero the words read.
+// Emission of a tile: per-thread popcounts -> block scan.  A sparse tile
+// (fewer set bits than words) is written thread by thread; a dense one gets a
+// position for every word (in LDS), then each wave walks its 512 words two at
+// a time, one lane per bit, so the set bits of a word pair are written to
+// consecutive addresses (coalesced).
 template <bool SWAP, bool CLEAR>
 __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u32 lo, u64 wpr, u64 *out) {
     __shared__ u64 sc[BE_T / WAVE + 1];
-    u64 base = (u64)blockIdx.x * BE_TILE + (u64)threadIdx.x * BE_W;
+    __shared__ u32 s_w[BE_TILE];
+    __shared__ u64 s_pos[BE_TILE];
+    const u64 tile = (u64)blockIdx.x * BE_TILE;
+    const u64 base = tile + (u64)threadIdx.x * BE_W;
     u32 w[BE_W];
     load8(words, base, w);
     u64 c = 0;
 #pragma unroll
     for (int i = 0; i < BE_W; i++) c += __popc(w[i]);
-    u64 pos = block_exclusive_scan<u64>(c, sc, nullptr) + bpre[blockIdx.x];
+    u64 tot;
+    u64 pos = block_exclusive_scan<u64>(c, sc, &tot) + bpre[blockIdx.x];
     if (CLEAR && c) {
         *(uint4 *)(words + base) = make_uint4(0u, 0u, 0u, 0u);
         *(uint4 *)(words + base + 4) = make_uint4(0u, 0u, 0u, 0u);
     }
+    if (tot < (u64)BE_TILE) {   // sparse tile (uniform): each thread writes its own bits
+#pragma unroll
+        for (int i = 0; i < BE_W; i++) {
+            u32 x = w[i];
+            if (!x) continue;
+            u64 k = (base + i) / wpr;
+            u32 c0 = (u32)((base + i - k * wpr) * 32);
+            while (x) {
+                u32 v = c0 + (u32)(__ffs(x) - 1);
+                x &= x - 1;
+                out[pos++] = SWAP ? tmake(lo + v, lo + (u32)k) : tmake(lo + (u32)k, lo + v);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < BE_W; i++) {
-        u32 x = w[i];
-        if (!x) continue;
-        u64 k = (base + i) / wpr;               // row
-        u32 c0 = (u32)((base + i - k * wpr) * 32);   // first column of the word
-        while (x) {
-            u32 v = c0 + (u32)(__ffs(x) - 1);
-            x &= x - 1;
-            out[pos++] = SWAP ? tmake(lo + v, lo + (u32)k) : tmake(lo + (u32)k, lo + v);
-        }
+        s_w[threadIdx.x * BE_W + i] = w[i];
+        s_pos[threadIdx.x * BE_W + i] = pos;
+        pos += __popc(w[i]);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const u32 half = lane >> 5, bit = lane & 31;
+    const u32 below = (1u << bit) - 1u;
+    for (int j = wv * (BE_TILE / (BE_T / WAVE)); j < (wv + 1) * (BE_TILE / (BE_T / WAVE)); j += 2) {
+        const u32 x0 = s_w[j], x1 = s_w[j + 1];
+        if ((x0 | x1) == 0u) continue;   // uniform across the wave
+        const u32 x = half ? x1 : x0;
+        if (!((x >> bit) & 1u)) continue;
+        const u64 wi = tile + j + half;
+        const u64 k = wi / wpr;                                   // row
+        const u32 v = (u32)((wi - k * wpr) * 32) + bit;           // column
+        out[s_pos[j + half] + __popc(x & below)] = SWAP ? tmake(lo + v, lo + (u32)k) : tmake(lo + (u32)k, lo + v);
     }
 }
 
