@@ -162,7 +162,11 @@ typedef struct dlg_opts {
     int32_t keep_output;     /* 1: copy the output rows to host (dlg_result.rows)      */
     int32_t check_output;    /* 1: fill dlg_result.output_sum / output_order_breaks
                                 (one pass over the device output, after the timing) */
-    int32_t reserved[5];
+    int32_t join_method;     /* 0 hash index (tc.cu, sg.cu); 1 sorted index with a
+                                binary search per probe, TC's edges deduplicated first
+                                (tc_nl.cu, sg_nl.cu; common/join.cu:89-180).  Same
+                                relations and iteration counts either way             */
+    int32_t reserved[4];
 } dlg_opts;
 
 /* Phase timers in the reference's CSV order (tc.cu:413-420). */

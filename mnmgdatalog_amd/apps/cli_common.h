@@ -36,6 +36,7 @@ inline void apply_kv(dlg_opts &o, const char *k, const char *v) {
     else if (!std::strcmp(k, "radix_bits")) o.radix_bits = std::atoi(v);
     else if (!std::strcmp(k, "strategy")) o.strategy = std::atoi(v);
     else if (!std::strcmp(k, "block_size")) o.block_size = std::atoi(v);
+    else if (!std::strcmp(k, "join_method")) o.join_method = std::atoi(v);
 }
 
 inline dlg_opts tuning_from_env() {
@@ -90,7 +91,7 @@ inline double now_s() {
 // tc / sg / wcc driver.  `suffix` names the output file; `wcc` selects the
 // WCC CSV columns.
 inline int run_graph_app(int argc, char **argv, const char *default_input, const char *suffix,
-                         runner_t fn, bool wcc) {
+                         runner_t fn, bool wcc, int join_method = 0) {
     Args a = parse(argc, argv, default_input, 0);
     std::string out_name = std::string(a.input) + suffix;
     dlg_ctx *ctx = nullptr;
@@ -112,6 +113,7 @@ inline int run_graph_app(int argc, char **argv, const char *default_input, const
 
     dlg_opts o = tuning_from_env();
     o.keep_output = 1;
+    if (join_method) o.join_method = join_method;   // the *_nl executables
     dlg_result r;
     s = fn(ctx, rows, n, &o, &r);
     if (s != DLG_OK) die(ctx, s, "fixpoint");

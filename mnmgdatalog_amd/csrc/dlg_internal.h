@@ -406,11 +406,13 @@ struct Index {
     DBuf<u32> gstart;    // start row of each key group, plus gstart[G] = n
     DBuf<u64> slots;     // open-addressed (key << 32 | group) slots, EMPTY_SLOT = free
     u64 n = 0, groups = 0, nslots = 0;
+    bool sorted = false; // probes binary-search `rows` instead of the slots (no slots built)
     explicit Index(dlg_ctx *c) : rows(c), gstart(c), slots(c) {}
 };
-// Build from a device relation (copied); `presorted` skips the sort.
+// Build from a device relation (copied); `presorted` skips the sort;
+// `sorted_lookup` builds no hash slots (probes binary-search the rows).
 void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double load_factor,
-                 bool presorted);
+                 bool presorted, bool sorted_lookup = false);
 // Count phase: per probe, the matching group's row start and count.  Returns |J|.
 struct JoinScratch {
     DBuf<u32> cnt, pstart;

@@ -18,6 +18,7 @@ struct Opts {
     int strategy;
     int block_size;  // 0 = the context's
     bool check;      // output digest
+    int join;        // 0 hash index, 1 sorted index (binary search)
 };
 Opts read_opts(dlg_ctx *ctx, const dlg_opts *o);
 

@@ -47,7 +47,7 @@ void PhaseClock::finish(double *out) {
 }
 
 Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
-    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0, false};
+    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0, false, 0};
     if (o) {
         if (o->load_factor < 0 || o->load_factor > 0.95)
             fail(DLG_INVALID_INPUT, "load_factor out of (0, 0.95]");
@@ -63,6 +63,8 @@ Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
         r.max_it = o->max_iterations;
         r.keep = o->keep_output != 0;
         r.check = o->check_output != 0;
+        if (o->join_method < 0 || o->join_method > 1) fail(DLG_INVALID_INPUT, "join_method out of 0..1");
+        r.join = o->join_method;
         r.strategy = o->strategy;
     }
     return r;
@@ -279,7 +281,14 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     id_bounds(ctx, edge.p, n, ib);
     pc.mark(DLG_PH_HASHTABLE);
     Index ix(ctx);
-    index_build(ctx, ix, edge.p, n, o.lf, false);      // duplicates kept, as the reference
+    if (o.join == 1) {
+        // tc_nl.cu:182-185 sorts and deduplicates the edge relation it probes
+        DBuf<u64> ue(ctx);
+        u64 ne = sort_unique_into(ctx, edge, n, tmp, ue, SEL_UNIQUE, nullptr, 0, o.bits, ib);
+        index_build(ctx, ix, ue.p, ne, o.lf, true, true);
+    } else {
+        index_build(ctx, ix, edge.p, n, o.lf, false);  // duplicates kept, as tc.cu:210
+    }
     edge.release();
     JoinScratch js(ctx);
     int it = 0;
@@ -350,7 +359,7 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     u64 ne = sort_unique_into(ctx, J, n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits, ib);
     pc.mark(DLG_PH_HASHTABLE);
     Index ix(ctx);
-    index_build(ctx, ix, edge.p, ne, o.lf, true);
+    index_build(ctx, ix, edge.p, ne, o.lf, true, o.join == 1);
     JoinScratch js(ctx);
     int it = 0;
     if (use_bitmap(o, ne, ib)) {
@@ -443,7 +452,7 @@ void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     a.release();
     pc.mark(DLG_PH_HASHTABLE);
     Index ix(ctx);
-    index_build(ctx, ix, R.p, nr, o.lf, true);
+    index_build(ctx, ix, R.p, nr, o.lf, true, o.join == 1);
     R.release();
     pc.mark(DLG_PH_JOIN);
     JoinScratch js(ctx);
@@ -465,7 +474,7 @@ void wcc_graph_build(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, 
     sym.release();
     tmp.release();
     pc.mark(DLG_PH_HASHTABLE);
-    index_build(ctx, g.ix, edge.p, g.ne, o.lf, true);
+    index_build(ctx, g.ix, edge.p, g.ne, o.lf, true, o.join == 1);
     edge.release();
     g.V = g.ix.groups;
     g.nbr.ensure(g.ne ? g.ne : 1);

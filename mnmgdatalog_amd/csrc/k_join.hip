@@ -59,10 +59,12 @@ __global__ void hash_insert_k(const u64 *rows, const u32 *gstart, u64 G, u64 *sl
     }
 }
 
-void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double lf, bool presorted) {
+void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double lf, bool presorted,
+                 bool sorted_lookup) {
     ix.n = n;
     ix.groups = 0;
     ix.nslots = 0;
+    ix.sorted = sorted_lookup;
     if (n == 0) return;
     if (n >= 0xffffffffull) fail(DLG_OVERFLOW, "index: relation larger than 2^32-2 rows");
     if (!(lf > 0.0)) lf = ctx->load_factor;
@@ -77,6 +79,7 @@ void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double lf, bool
     u64 G = select(ctx, SEL_KEY_HEAD, ix.rows.p, n, nullptr, 0, ix.gstart.p);
     DLG_LAUNCH(ctx, DLG_K_MISC, 4.0, set_last_k, 1, 64, 0, ix.gstart.p, G, (u32)n);
     ix.groups = G;
+    if (sorted_lookup) return;
     u64 want = (u64)std::ceil((double)G / lf);
     u64 S = 64;
     while (S < want || S <= G) S <<= 1;
@@ -105,11 +108,33 @@ __global__ __launch_bounds__(1024) void join_count_k(const u64 *probe, u64 n, co
     }
 }
 
+// Sorted-index variant (common/join.cu:89-142): the probe key's run of rows
+// by two binary searches over the sorted relation.
+__global__ __launch_bounds__(1024) void join_count_sorted_k(const u64 *probe, u64 n, const u64 *rows, u64 nr,
+                                                           u32 *cnt, u32 *pstart) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        u64 k = (u64)tkey(probe[i]) << 32;
+        u64 s = lower_bound_u64(rows, 0, nr, k);
+        u64 e = upper_bound_u64(rows, s, nr, k | 0xffffffffull);
+        cnt[i] = (u32)(e - s);
+        pstart[i] = (u32)s;
+    }
+}
+
 u64 join_count(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js) {
     if (n == 0 || ix.n == 0) return 0;
     js.cnt.ensure(n);
     js.pstart.ensure(n);
     js.offs.ensure(n + 1);
+    if (ix.sorted) {
+        // probe read + ~2 log2(n) row reads per probe + count/start writes
+        double lg = std::log2((double)ix.n + 1.0);
+        DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 16.0 * lg * n + 8.0 * n, join_count_sorted_k, sgrid(ctx, n),
+                   ctx->block_size, 0, probe, n, (const u64 *)ix.rows.p, ix.n, js.cnt.p, js.pstart.p);
+        return scan_u32_u64(ctx, js.cnt.p, n, js.offs.p);
+    }
     // probe read + one slot line per probe + gstart pair + count/start writes
     DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 8.0 * n + 8.0 * n + 8.0 * n, join_count_k,
                sgrid(ctx, n), ctx->block_size, 0, probe, n, (const u64 *)ix.slots.p, ix.nslots - 1, (const u32 *)ix.gstart.p,
@@ -201,8 +226,34 @@ __global__ void lookup_values_k(const u64 *rows, u64 n, const u64 *slots, u64 ma
     for (; i < n; i += stride) out[i] = ix_find(slots, mask, tval(rows[i]));
 }
 
+// Group of each row's value by binary search: its first row, then that row's
+// group among the group starts.
+__global__ void lookup_values_sorted_k(const u64 *rows, u64 n, const u32 *gstart, u64 G, u32 *out) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        u64 k = (u64)tval(rows[i]) << 32;
+        u64 r = lower_bound_u64(rows, 0, n, k);
+        u32 g = 0xffffffffu;
+        if (r < n && (rows[r] >> 32) == (k >> 32)) {
+            u64 lo = 0, hi = G;   // the group whose start is r
+            while (lo < hi) {
+                u64 mid = (lo + hi) >> 1;
+                if (gstart[mid] < r) lo = mid + 1; else hi = mid;
+            }
+            g = (u32)lo;
+        }
+        out[i] = g;
+    }
+}
+
 void index_lookup_values(dlg_ctx *ctx, const Index &ix, u32 *out) {
     if (ix.n == 0) return;
+    if (ix.sorted) {
+        DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * ix.n, lookup_values_sorted_k, sgrid(ctx, ix.n), ctx->block_size, 0,
+                   (const u64 *)ix.rows.p, ix.n, (const u32 *)ix.gstart.p, ix.groups, out);
+        return;
+    }
     DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * ix.n, lookup_values_k, sgrid(ctx, ix.n), ctx->block_size, 0,
                (const u64 *)ix.rows.p,
                ix.n, (const u64 *)ix.slots.p, ix.nslots - 1, out);

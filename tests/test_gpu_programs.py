@@ -212,14 +212,15 @@ def run_cli(prog, *args):
     return p.stdout.strip().splitlines()
 
 
-@pytest.mark.parametrize("prog,suffix", [("tc", "_tc.bin"), ("sg", "_sg.bin"), ("wcc", "_cc.bin")])
+@pytest.mark.parametrize("prog,suffix", [("tc", "_tc.bin"), ("sg", "_sg.bin"), ("wcc", "_cc.bin"),
+                                         ("tc_nl", "_tc.bin"), ("sg_nl", "_sg.bin")])
 @pytest.mark.parametrize("name", ["hipc_2019.bin", "dummy.bin", "data_7035.bin"])
 def test_cli_output_file_and_csv(orc, tmp_path, prog, suffix, name):
     src = tmp_path / name
     shutil.copy(os.path.join(DATA, name), src)
     out = run_cli(prog, src, 0, 0, 0)
     assert out[0] == f"Rank 0 wrote local tuples on file: {src}{suffix}"
-    want = getattr(orc, prog)(read_bin(name))
+    want = getattr(orc, prog.replace("_nl", ""))(read_bin(name))
     header, row = out[1].split(","), out[2].split(",")
     assert header[:4] == ["# Input", "# Process", "# Iterations", "# CC" if prog == "wcc" else "# TC"]
     assert len(row) == len(header) == (16 if prog == "wcc" else 15)
@@ -249,6 +250,38 @@ def test_cli_missing_file_fails(tmp_path):
     p = subprocess.run([os.path.join(dlg.BIN_DIR, "tc"), str(tmp_path / "nope.bin")], capture_output=True,
                        text=True, timeout=120)
     assert p.returncode != 0 and "Error opening file" in p.stderr
+
+
+# ------------------------------------------------------------------ sorted-index join (tc_nl / sg_nl)
+@pytest.mark.parametrize("name", ["data_7035.bin", "data_23874.bin", "paper.bin"])
+def test_sorted_index_join_matches_oracle(ctx, orc, name):
+    rows = read_bin(name)
+    uniq = np.unique(rows, axis=0)
+    want_tc, want_sg = orc.tc(rows), orc.sg(rows)
+    for strategy in (0, 1, 3, 4):
+        got = ctx.run("tc", rows, join_method=1, strategy=strategy)
+        assert_same(got, want_tc)
+        # tc_nl.cu probes the deduplicated edges: its |J| is TC's on unique(E)
+        assert got["raw_join_rows"] == orc.tc(uniq)["raw_join_rows"]
+        got = ctx.run("sg", rows, join_method=1, strategy=strategy)
+        assert_same(got, want_sg)
+        assert got["raw_join_rows"] == want_sg["raw_join_rows"]
+    assert_same(ctx.run("wcc", rows, join_method=1), orc.wcc(rows), secondary=True)
+    assert_same(ctx.run("single_join", rows, join_method=1), orc.single_join(rows))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_sorted_index_join_random(ctx, orc, seed):
+    rows = rand_rows(700 + seed, 200, 400, neg=seed == 1)
+    assert_same(ctx.run("tc", rows, join_method=1), orc.tc(rows))
+    assert_same(ctx.run("sg", rows, join_method=1), orc.sg(rows))
+    assert_same(ctx.run("wcc", rows, join_method=1), orc.wcc(rows), secondary=True)
+
+
+def test_join_method_rejects_unknown(ctx):
+    with pytest.raises(dlg.DlgError) as e:
+        ctx.run("tc", read_bin("dummy.bin"), join_method=2)
+    assert e.value.status == dlg.DLG_INVALID_INPUT
 
 
 # ------------------------------------------------------------------ strategies
