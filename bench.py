@@ -92,6 +92,26 @@ def cpu_baseline(name, prog, rows, ctx, threads, strategy=0):
 METRIC = "derived tuples/s, 1 MI355X, bit-exact; batched/served vs plain speedup + p99"
 
 
+def energy_meter(ctx):
+    """Reader of the device's energy accumulator (joules) through amdsmi, matched
+    to the context's device by PCI address; None when it is not available.
+    Reading it changes nothing on the device."""
+    try:
+        import amdsmi
+        amdsmi.amdsmi_init()
+        bdf = ctx.pci_bus_id().lower()
+        for h in amdsmi.amdsmi_get_processor_handles():
+            if amdsmi.amdsmi_get_gpu_device_bdf(h).lower() == bdf:
+                def read(h=h):
+                    e = amdsmi.amdsmi_get_energy_count(h)
+                    return e["energy_accumulator"] * e["counter_resolution"] * 1e-6
+                read()
+                return read
+    except Exception as e:   # no monitor on this host: the line reports null
+        log(f"energy: not available ({type(e).__name__}: {e})")
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -333,9 +353,11 @@ def main():
         res = None
         for _ in range(args.warmup):
             res = ctx.run(prog, None, keep_output=False, device_rows=d_rows, n=n, strategy=args.strategy)
+        meter = energy_meter(ctx)
         ctx.kernel_stats_reset()
         ctx.profiling(True)
         ctx.check(ctx.L.dlg_ctx_sync(ctx.h), "sync")
+        e0 = meter() if meter else None
         t0 = time.perf_counter()
         dev_s = 0.0
         for _ in range(args.steps):
@@ -343,6 +365,7 @@ def main():
             dev_s += res["device_seconds"]
         ctx.check(ctx.L.dlg_ctx_sync(ctx.h), "sync")
         t1 = time.perf_counter()
+        e1 = meter() if meter else None
         ctx.profiling(False)
         ks = ctx.kernel_stats()
         elapsed = t1 - t0
@@ -374,6 +397,10 @@ def main():
                      "launches": st["launches"]},
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "energy": None if e0 is None else {
+            "joules_per_step": (e1 - e0) / args.steps, "avg_watts": (e1 - e0) / elapsed,
+            "tuples_per_joule": out_tuples * args.steps / (e1 - e0) if e1 > e0 else None,
+            "source": "amdsmi energy accumulator of the device, over the timed steps"},
         "device": info,
     }
     print(json.dumps(line), flush=True)

@@ -56,6 +56,10 @@ dlg_status dlg_ctx_sync(dlg_ctx *ctx);
 dlg_status dlg_device_info(dlg_ctx *ctx, int32_t *num_cus, int64_t *hbm_bytes,
                            int32_t *lds_per_block, char *arch_name, int32_t arch_len);
 
+/* The context's device as a PCI address "dddd:bb:dd.f" (for matching power and
+ * energy monitors to the device); `len` >= 13. */
+dlg_status dlg_device_pci_bus_id(dlg_ctx *ctx, char *bdf, int32_t len);
+
 /* Per-kernel event timing on the context's stream (off by default).  While on,
  * every launch is bracketed by hipEvents and its algorithmic bytes recorded. */
 dlg_status dlg_profiling(dlg_ctx *ctx, int enable);

@@ -92,6 +92,7 @@ def lib() -> ctypes.CDLL:
         "dlg_last_error": (c_char_p, [vp]),
         "dlg_ctx_sync": (c_int, [vp]),
         "dlg_device_info": (c_int, [vp, POINTER(c_int32), POINTER(c_int64), POINTER(c_int32), c_char_p, c_int32]),
+        "dlg_device_pci_bus_id": (c_int, [vp, c_char_p, c_int32]),
         "dlg_profiling": (c_int, [vp, c_int]),
         "dlg_kernel_name": (c_char_p, [c_int]),
         "dlg_kernel_stats": (c_int, [vp, c_int, POINTER(c_int64), POINTER(c_double), POINTER(c_double)]),
@@ -264,6 +265,11 @@ class Ctx:
         self.check(self.L.dlg_device_info(self.h, byref(cu), byref(hbm), byref(lds), arch, 64), "device_info")
         return {"num_cus": cu.value, "hbm_bytes": hbm.value, "lds_per_block": lds.value,
                 "arch": arch.value.decode()}
+
+    def pci_bus_id(self) -> str:
+        b = ctypes.create_string_buffer(64)
+        self.check(self.L.dlg_device_pci_bus_id(self.h, b, 64), "device_pci_bus_id")
+        return b.value.decode()
 
     # ---- device memory
     def alloc(self, nbytes: int) -> int:

@@ -224,6 +224,13 @@ dlg_status dlg_device_info(dlg_ctx *c, int32_t *num_cus, int64_t *hbm, int32_t *
     return DLG_OK;
 }
 
+dlg_status dlg_device_pci_bus_id(dlg_ctx *c, char *bdf, int32_t len) {
+    if (!c || !bdf || len < 13) return DLG_INVALID_INPUT;
+    API_BEGIN(c)
+    HIPCHK(hipDeviceGetPCIBusId(bdf, len, c->device));
+    API_END(c)
+}
+
 dlg_status dlg_profiling(dlg_ctx *c, int enable) {
     if (!c) return DLG_INVALID_INPUT;
     API_BEGIN(c)

@@ -175,3 +175,58 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+# ------------------------------------------------------------------ format tools
+TWINS = ["data_5", "data_10", "paper", "quiz3", "tc_paper", "data_22", "data_7035"]
+
+
+def tool(*args):
+    return subprocess.run([os.path.join(dlg.BIN_DIR, args[0]), *map(str, args[1:])], capture_output=True,
+                          text=True, timeout=60)
+
+
+@pytest.mark.parametrize("name", TWINS)
+def test_txt_to_bin_matches_reference_twins(tmp_path, name):
+    """The reference ships each small graph as .txt and .bin; converting the
+    text must give the binary byte for byte (binary_file_utils.py:5-18)."""
+    out = tmp_path / f"{name}.bin"
+    p = tool("binary_file_utils", "txt_to_bin", os.path.join(DATA, f"{name}.txt"), out)
+    assert p.returncode == 0 and "Converted" in p.stdout
+    assert out.read_bytes() == open(os.path.join(DATA, f"{name}.bin"), "rb").read()
+
+
+@pytest.mark.parametrize("name", TWINS + ["hipc_2019", "dummy"])
+def test_bin_to_txt_round_trip(tmp_path, name):
+    src = os.path.join(DATA, f"{name}.bin")
+    txt, back = tmp_path / "x.txt", tmp_path / "x.bin"
+    assert tool("binary_file_utils", "bin_to_txt", src, txt).returncode == 0
+    rows = read_bin(f"{name}.bin")
+    assert txt.read_text() == "".join(f"{a}\t{b}\n" for a, b in rows)   # binary_file_utils.py:21-33
+    assert tool("binary_file_utils", "txt_to_bin", txt, back).returncode == 0
+    assert back.read_bytes() == open(src, "rb").read()
+
+
+def test_txt_to_bin_field_rules(tmp_path):
+    src, out = tmp_path / "in.txt", tmp_path / "out.bin"
+    src.write_text("1 2 3\n-4\t+5\nx 7 y 8\n\n9\n1_0 2_000\nfoo\n")
+    assert tool("binary_file_utils", "txt_to_bin", src, out).returncode == 0
+    assert np.fromfile(out, np.int32).tolist() == [1, 2, -4, 5, 7, 8, 9, 10, 2000]
+
+
+def test_format_tool_errors(tmp_path):
+    src, out = tmp_path / "in.txt", tmp_path / "out.bin"
+    src.write_text("1 2147483648\n")
+    assert tool("binary_file_utils", "txt_to_bin", src, out).returncode != 0
+    odd = tmp_path / "odd.bin"
+    odd.write_bytes(b"\x01\x00\x00\x00\x02\x00\x00\x00\x03")
+    assert tool("binary_file_utils", "bin_to_txt", odd, tmp_path / "o.txt").returncode != 0
+    p = tool("binary_file_utils", "txt_to_bin", src)
+    assert p.returncode == 0 and p.stdout.startswith("Usage:")
+    p = tool("binary_file_utils", "nope", src, out)
+    assert p.returncode == 0 and "Invalid conversion type" in p.stdout
+
+
+def test_row_size():
+    p = tool("row_size", os.path.join(DATA, "data_7035.bin"))
+    assert p.returncode == 0 and p.stdout.strip() == f"{os.path.join(DATA, 'data_7035.bin')}: 7035"
