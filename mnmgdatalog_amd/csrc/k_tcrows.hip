@@ -28,13 +28,25 @@ namespace dlg {
 
 constexpr int TR_T = 256;                    // threads per workgroup: four waves, one segment each
 constexpr int TR_W = TR_T / WAVE;
-constexpr int TR_H = 1024;                   // LDS hash slots per wave
-constexpr int TR_FLUSH = 512;                // flush above this many occupied slots ...
-constexpr int TR_BATCH = 256;                // ... before each batch of at most this many rows (TR_U per lane)
+constexpr int TR_BATCH = 256;                // rows per batch of the load-balanced path (TR_U per lane)
 constexpr u32 TR_EMPTY = 0xffffffffu;
+constexpr int TR_SMALL = 8;   // chunks whose probes all have at most this many matches: lane per probe
 
-__device__ __forceinline__ u32 tr_hash(u32 w) { return (w * 0x9e3779b1u) >> (32 - 10); }
-static_assert(TR_H == 1 << 10, "tr_hash assumes 1024 slots");
+// The word table of a wave: 2^LOG slots.  A larger table flushes a long
+// segment fewer times; a smaller one leaves LDS for more waves per CU (short
+// segments).  Insert batches are sized so the table stays <= 3/4 full.
+template <int LOG>
+struct TrCfg {
+    static constexpr int H = 1 << LOG;
+    static constexpr int STEP = LOG >= 10 ? 8 : 2;   // matches per lane between two flush checks
+    static constexpr int FLUSH = H * 3 / 4 - TR_BATCH;              // before a load-balanced batch
+    static constexpr int FLUSH_SMALL = H * 3 / 4 - STEP * WAVE;     // before a lane-per-probe step
+    static constexpr int BLOCKS_PER_CU = LOG >= 10 ? 3 : 5;         // what LDS allows
+    static_assert(FLUSH > 0 && FLUSH_SMALL > 0 && TR_SMALL % STEP == 0, "table load");
+};
+
+template <int LOG>
+__device__ __forceinline__ u32 tr_hash(u32 w) { return (w * 0x9e3779b1u) >> (32 - LOG); }
 
 // Memory written by some lanes of a wave and then read by others (the LDS
 // table; F's words across two flushes of one segment): a workgroup-scope
@@ -57,10 +69,11 @@ __device__ __forceinline__ u64 wave_excl_scan(u64 v, u64 *total) {
     return inc - v;
 }
 
+template <int LOG>
 struct TrWave {
-    u32 key[TR_H];
-    u32 bits[TR_H];
-    u16 used[TR_H];
+    u32 key[TrCfg<LOG>::H];
+    u32 bits[TrCfg<LOG>::H];
+    u16 used[TrCfg<LOG>::H];
     u64 p_off[WAVE + 1];    // row offsets of the chunk's probes (p_off[WAVE] = total)
     u64 p_start[WAVE];      // CSR start of each probe's adjacency
     u32 nused;
@@ -68,14 +81,12 @@ struct TrWave {
 
 constexpr int TR_U = 4;   // words / rows per lane with their loads in flight together
 constexpr u32 TR_DMAX_BOUND = 16;   // degrees up to this: output regions bounded by new ends * max degree
-constexpr int TR_SMALL = 8;   // chunks whose probes all have at most this many matches: lane per probe
-constexpr int TR_FLUSH_SMALL = TR_H - TR_SMALL * WAVE - TR_H / 4;   // keeps the table <= 3/4 full
-static_assert(TR_FLUSH_SMALL > 0 && TR_FLUSH + TR_BATCH <= TR_H * 3 / 4, "table load");
 
 // OR bit z of the row into the wave's table (z relative to the id range).
-__device__ __forceinline__ void tr_insert(TrWave &s, u32 z) {
+template <int LOG>
+__device__ __forceinline__ void tr_insert(TrWave<LOG> &s, u32 z) {
     u32 w = z >> 5, mk = 1u << (z & 31);
-    u32 h = tr_hash(w);
+    u32 h = tr_hash<LOG>(w);
     while (true) {
         u32 k = atomicCAS(&s.key[h], TR_EMPTY, w);
         if (k == TR_EMPTY) {
@@ -83,7 +94,7 @@ __device__ __forceinline__ void tr_insert(TrWave &s, u32 z) {
             break;
         }
         if (k == w) break;
-        h = (h + 1) & (TR_H - 1);
+        h = (h + 1) & (TrCfg<LOG>::H - 1);
     }
     atomicOr(&s.bits[h], mk);
 }
@@ -92,7 +103,8 @@ __device__ __forceinline__ void tr_insert(TrWave &s, u32 z) {
 // and, for new bits, a write of the word (TR_U words per lane in flight);
 // the new bits are written as ends z (packed) to dst[0..), in slot order.
 // Returns the number written.  The table is left empty.
-__device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst) {
+template <int LOG>
+__device__ u64 tr_flush(TrWave<LOG> &s, u32 *F, u64 rowbase, u32 lo, u32 *dst) {
     const int lane = threadIdx.x & (WAVE - 1);
     wave_sync();
     const u32 n = s.nused;
@@ -143,17 +155,19 @@ __device__ u64 tr_flush(TrWave &s, u32 *F, u64 rowbase, u32 lo, u32 *dst) {
 // round's join rows are counted on the way (counters[2]).  Workgroups take
 // static ranges of segments; the four waves of a workgroup share the range
 // through an LDS counter, so no global counter is contended.
+template <int LOG>
 __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_x, const u64 *seg_beg,
                                                   const u32 *seg_len, const u64 *seg_out, u64 nseg,
                                                   const u64 *csr_off, const u32 *csr_dst, const u32 *deg,
                                                   u32 dmax, u32 *F, const u64 *row_base, const u32 *col_lo,
                                                   u32 lo, u64 wcap, u32 *ny, u32 *nseg_len, u64 *next_rows,
                                                   unsigned long long *counters) {
-    __shared__ TrWave sw[TR_W];
+    using C = TrCfg<LOG>;
+    __shared__ TrWave<LOG> sw[TR_W];
     __shared__ u32 s_next;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    TrWave &s = sw[wv];
-    for (int i = lane; i < TR_H; i += WAVE) {
+    TrWave<LOG> &s = sw[wv];
+    for (int i = lane; i < C::H; i += WAVE) {
         s.key[i] = TR_EMPTY;
         s.bits[i] = 0u;
     }
@@ -209,12 +223,17 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
                         for (int i = 0; i < TR_SMALL; i++) zs[h][i] = (u32)i < d[h] ? csr_dst[st[h] + i] : 0u;
 #pragma unroll
                     for (int h = 0; h < 2; h++) {
-                        if (s.nused > (u32)TR_FLUSH_SMALL)   // uniform: every lane reads the same LDS word
-                            cnt += tr_flush(s, F, rowbase, clo, ny + out0 + cnt);
 #pragma unroll
-                        for (int i = 0; i < TR_SMALL; i++)
-                            if ((u32)i < d[h]) tr_insert(s, zs[h][i] - clo);
-                        wave_sync();
+                        for (int i0 = 0; i0 < TR_SMALL; i0 += C::STEP) {
+                            if ((u32)i0 < maxd) {   // uniform
+                                if (s.nused > (u32)C::FLUSH_SMALL)   // uniform: every lane reads the same LDS word
+                                    cnt += tr_flush(s, F, rowbase, clo, ny + out0 + cnt);
+#pragma unroll
+                                for (int i = i0; i < i0 + C::STEP; i++)
+                                    if ((u32)i < d[h]) tr_insert(s, zs[h][i] - clo);
+                                wave_sync();
+                            }
+                        }
                     }
                     continue;
                 }
@@ -229,7 +248,7 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
                     if (lane == 0) s.p_off[WAVE] = R;
                     wave_sync();
                     for (u64 rb = 0; rb < R; rb += TR_BATCH) {
-                        if (s.nused > (u32)TR_FLUSH)   // uniform: every lane reads the same LDS word
+                        if (s.nused > (u32)C::FLUSH)   // uniform: every lane reads the same LDS word
                             cnt += tr_flush(s, F, rowbase, clo, ny + out0 + cnt);
                         const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
                         u64 ci[TR_U];
@@ -400,6 +419,18 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
     tr.rows = h[1];
 }
 
+constexpr u64 TR_LONG_SEGMENTS = 64;   // average probes per segment above which the big table wins
+
+template <int LOG>
+static void launch_rows(dlg_ctx *ctx, RowMap &F, TcRows &tr) {
+    unsigned g = (unsigned)std::min<u64>((tr.nseg + TR_W - 1) / TR_W, (u64)ctx->num_cus * TrCfg<LOG>::BLOCKS_PER_CU);
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, tc_rows_k<LOG>, g, TR_T, 0, (const u32 *)tr.dy.p, (const u32 *)tr.seg_x.p,
+               (const u64 *)tr.seg_beg.p, (const u32 *)tr.seg_len.p, (const u64 *)tr.seg_out.p, tr.nseg,
+               (const u64 *)tr.csr_off.p, (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, tr.dmax, F.words.p,
+               (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo, tr.wcap, tr.ny.p, tr.nseg_len.p,
+               tr.next_rows.p, (unsigned long long *)tr.counters.p);
+}
+
 u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     if (tr.nd == 0 || tr.rows == 0) {
         tr.nd = 0;
@@ -407,17 +438,11 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     }
     tr.ny.ensure(tr.rows);
     fill_u64(ctx, tr.counters.p, 3, 0ull);
-    unsigned g = (unsigned)std::min<u64>((tr.nseg + TR_W - 1) / TR_W, (u64)ctx->num_cus * 3);
     // algorithmic bytes, added once the round's join rows are known: 4 B per
     // join row (its matched end) + 20 B per probe (its end, CSR offset, degree)
     const u64 nd0 = tr.nd;
-    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, tc_rows_k, g, TR_T, 0,
-               (const u32 *)tr.dy.p, (const u32 *)tr.seg_x.p, (const u64 *)tr.seg_beg.p,
-               (const u32 *)tr.seg_len.p, (const u64 *)tr.seg_out.p, tr.nseg, (const u64 *)tr.csr_off.p,
-               (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, tr.dmax, F.words.p, (const u64 *)F.row_base.p,
-               (const u32 *)F.col_lo.p, F.lo, tr.wcap, tr.ny.p,
-               tr.nseg_len.p, tr.next_rows.p,
-               (unsigned long long *)tr.counters.p);
+    if (tr.nd >= TR_LONG_SEGMENTS * tr.nseg) launch_rows<10>(ctx, F, tr);   // long segments: big tables
+    else launch_rows<9>(ctx, F, tr);                                        // short ones: more waves
     // the new ends sit at the old output offsets; the next offsets scan the next rows
     tr.seg_beg.swap(tr.seg_out);
     scan_u64_async(ctx, tr.next_rows.p, tr.nseg, tr.seg_out.p, tr.counters.p + 1);
