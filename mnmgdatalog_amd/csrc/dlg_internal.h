@@ -222,6 +222,18 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T *scratch, T *total) {
     return r;
 }
 
+__device__ __forceinline__ u32 wave_excl_scan32(u32 v, u32 *total) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    u32 inc = v;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        u32 t = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc += t;
+    }
+    *total = __shfl(inc, WAVE - 1, WAVE);
+    return inc - v;
+}
+
 // Binary searches over sorted u64 arrays in global memory.
 __device__ __forceinline__ u64 lower_bound_u64(const u64 *a, u64 lo, u64 hi, u64 x) {
     while (lo < hi) {
@@ -361,14 +373,20 @@ u64 rowmap_emit(dlg_ctx *ctx, RowMap &m, DBuf<u64> &out, u64 expect);
 // CSR of E over F's id range.  seg_out = exclusive scan of the segments'
 // output bounds = where their new ends go.
 struct TcRows {
-    DBuf<u64> csr_off, seg_beg, seg_out, next_rows, counters;
-    DBuf<u32> csr_dst, deg, dy, ny, seg_x, seg_len, nseg_len;
-    u64 nd = 0, nseg = 0, rows = 0;   // rows: the round's output bound (Σ of the segments')
+    DBuf<u64> csr_off;
+    DBuf<u32> csr_dst, deg;
+    // Δ: ends dy grouped by source, and the active list (the sources with
+    // ends): source ax, its ends at dy[abeg ..][0, alen), its output offset
+    // aoff (exclusive scan of the bounds); n*: the next round's.  rcnt,
+    // rbound: each entry's new ends and next bound; part: chunk sums.
+    DBuf<u32> dy, ny, ax, nx, alen, nlen, rcnt;
+    DBuf<u64> abeg, nbeg, aoff, noff, rbound, part, counters;
+    u64 nd = 0, nact = 0, rows = 0;   // rows: the round's output bound (Σ of the entries')
     u32 dmax = 0;                     // nonzero: every degree <= dmax, bounds = new ends * dmax
-    u64 wcap = 0;                     // the widest block: no segment has more new ends
+    u64 wcap = 0;                     // the widest block: no source has more new ends
     explicit TcRows(dlg_ctx *c)
-        : csr_off(c), seg_beg(c), seg_out(c), next_rows(c), counters(c), csr_dst(c), deg(c), dy(c), ny(c),
-          seg_x(c), seg_len(c), nseg_len(c) {}
+        : csr_off(c), csr_dst(c), deg(c), dy(c), ny(c), ax(c), nx(c), alen(c), nlen(c), rcnt(c), abeg(c), nbeg(c),
+          aoff(c), noff(c), rbound(c), part(c), counters(c) {}
 };
 // CSR of the index's rows; Δ0 = unique(E reversed) grouped by source, its
 // bits set in F (F empty).

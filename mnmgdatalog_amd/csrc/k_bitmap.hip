@@ -82,10 +82,10 @@ static_assert(BH == 1 << 12, "bh_hash assumes 4096 slots");
 //     that many rows of the tile hit costs one global read and at most one
 //     atomicOr (device-scope atomics execute at the memory side);
 //   4 per distinct word: a plain read; the bits not yet set go in with one
-//     atomicOr, whose result tells which of them this tile set;
-//   5 MARK: those bits are ORed into `mark`; APPEND: each such bit is
-//     claimed by the first of the tile's rows that carries it, which appends
-//     its tuple through an LDS buffer: one counter atomic per tile.
+//     atomicOr (APPEND reads its result: the bits this tile set);
+//   5 MARK: the same bits are ORed into `mark`; APPEND: each bit this tile
+//     set is claimed by the first of the tile's rows that carries it, which
+//     appends its tuple through an LDS buffer: one counter atomic per tile.
 template <int PROJ, bool NODIAG, bool TR, int MODE>
 __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 np, const u32 *pstart,
                                                          const u64 *offs, u64 total, const u64 *rows,
@@ -244,26 +244,32 @@ __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 n
         u32 cur[BJ_I];
 #pragma unroll
         for (int i = 0; i < BJ_I; i++) cur[i] = sb[i] ? words[sk[i]] : 0u;
+        // MARK needs no atomic results: the plain read sees at least the
+        // words as the launch began (every earlier round's bits), so a bit it
+        // finds clear is new this round, and marking it twice is harmless;
+        // the first tile to set a bit read it clear, so every new bit is
+        // marked.  APPEND needs the result: each bit is appended once.
         u32 nb[BJ_I];
 #pragma unroll
         for (int i = 0; i < BJ_I; i++) {
             nb[i] = sb[i] & ~cur[i];
             if (nb[i]) {
-                if (MODE == BM_SET) atomicOr(words + sk[i], nb[i]);
+                if (MODE != BM_APPEND) atomicOr(words + sk[i], nb[i]);
                 else cur[i] = atomicOr(words + sk[i], nb[i]);
             }
         }
         if (MODE == BM_SET) continue;
+        if (MODE == BM_MARK) {
+#pragma unroll
+            for (int i = 0; i < BJ_I; i++)
+                if (nb[i]) atomicOr(mark + sk[i], nb[i]);
+            continue;
+        }
 #pragma unroll
         for (int i = 0; i < BJ_I; i++) {
             if (nb[i]) nb[i] &= ~cur[i];   // the bits this tile set
-            if (MODE == BM_MARK) {
-                if (nb[i]) atomicOr(mark + sk[i], nb[i]);
-            } else if (sb[i]) {
-                h_bits[sh[i]] = nb[i];
-            }
+            if (sb[i]) h_bits[sh[i]] = nb[i];
         }
-        if (MODE == BM_MARK) continue;
         __syncthreads();
         // 5: each set bit goes to the first row of the tile that claims it
 #pragma unroll

@@ -28,28 +28,32 @@ namespace dlg {
 
 constexpr int TR_T = 256;                    // threads per workgroup: four waves, one segment each
 constexpr int TR_W = TR_T / WAVE;
-constexpr int TR_BATCH = 256;                // rows per batch of the load-balanced path (TR_U per lane)
 constexpr u32 TR_EMPTY = 0xffffffffu;
 constexpr int TR_SMALL = 8;   // chunks whose probes all have at most this many matches: lane per probe
+constexpr int TR_U = 4;       // words / rows per lane with their loads in flight together
+constexpr int TR_BATCH = TR_U * WAVE;        // rows per batch of the load-balanced path
+constexpr u32 TR_DMAX_BOUND = 16;   // degrees up to this: output regions bounded by new ends * max degree
+constexpr u64 TR_ENTRY_COST = 256;  // an entry's fixed cost (loads, final flush) in join rows
 
-// The word table of a wave: 2^LOG slots.  A larger table flushes a long
-// segment fewer times; a smaller one leaves LDS for more waves per CU (short
-// segments).  Insert batches are sized so the table stays <= 3/4 full.
+// The word table of a wave: 2^LOG slots, linear probing.  A larger table
+// flushes a long segment fewer times; a smaller one leaves LDS for more waves
+// per CU (short segments).  A step inserts at most NI (lane-per-probe path)
+// or TR_U (load-balanced path) entries per lane; the table is flushed before
+// a step's entries could take it past LIMIT occupied slots.
 template <int LOG>
 struct TrCfg {
     static constexpr int H = 1 << LOG;
-    static constexpr int STEP = LOG >= 10 ? 8 : 2;   // matches per lane between two flush checks
-    static constexpr int FLUSH = H * 3 / 4 - TR_BATCH;              // before a load-balanced batch
-    static constexpr int FLUSH_SMALL = H * 3 / 4 - STEP * WAVE;     // before a lane-per-probe step
-    static constexpr int BLOCKS_PER_CU = LOG >= 10 ? 3 : 5;         // what LDS allows
-    static_assert(FLUSH > 0 && FLUSH_SMALL > 0 && TR_SMALL % STEP == 0, "table load");
+    static constexpr int NI = LOG >= 10 ? 8 : 4;
+    static constexpr u32 LIMIT = H * 3 / 4;
+    static constexpr int BLOCKS_PER_CU = LOG >= 10 ? 3 : 4;   // what LDS (big table) or registers allow
+    static_assert(NI * WAVE <= (int)LIMIT && TR_BATCH <= (int)LIMIT && TR_SMALL % NI == 0, "table load");
 };
 
 template <int LOG>
 __device__ __forceinline__ u32 tr_hash(u32 w) { return (w * 0x9e3779b1u) >> (32 - LOG); }
 
-// Memory written by some lanes of a wave and then read by others (the LDS
-// table; F's words across two flushes of one segment): a workgroup-scope
+// Memory written by some lanes of a wave and then read by others (the slot
+// list; F's words across two flushes of one segment): a workgroup-scope
 // fence orders them (the wave's lanes share one CU and its L1), and the
 // wave barrier keeps the compiler from moving accesses across it.
 __device__ __forceinline__ void wave_sync() {
@@ -73,97 +77,127 @@ template <int LOG>
 struct TrWave {
     u32 key[TrCfg<LOG>::H];
     u32 bits[TrCfg<LOG>::H];
-    u16 used[TrCfg<LOG>::H];
-    u64 p_off[WAVE + 1];    // row offsets of the chunk's probes (p_off[WAVE] = total)
-    u64 p_start[WAVE];      // CSR start of each probe's adjacency
-    u32 nused;
+    u16 used[TrCfg<LOG>::H];   // occupied slots, in insertion order
+    u64 p_off[WAVE + 1];       // row offsets of the chunk's probes (p_off[WAVE] = total)
+    u64 p_start[WAVE];         // CSR start of each probe's adjacency
 };
 
-constexpr int TR_U = 4;   // words / rows per lane with their loads in flight together
-constexpr u32 TR_DMAX_BOUND = 16;   // degrees up to this: output regions bounded by new ends * max degree
-
-// OR bit z of the row into the wave's table (z relative to the id range).
-template <int LOG>
-__device__ __forceinline__ void tr_insert(TrWave<LOG> &s, u32 z) {
-    u32 w = z >> 5, mk = 1u << (z & 31);
-    u32 h = tr_hash<LOG>(w);
-    while (true) {
-        u32 k = atomicCAS(&s.key[h], TR_EMPTY, w);
-        if (k == TR_EMPTY) {
-            s.used[atomicAdd(&s.nused, 1u)] = (u16)h;
-            break;
+// OR the lane's entries (word w[i], bits mk[i]; mk[i] = 0: no entry) into
+// the wave's table, one entry at a time (a CAS claims or finds the word's
+// slot, an OR with no result sets the bits), and append the new slots to
+// `used`; nused is the wave's (uniform) count.
+template <int LOG, int N>
+__device__ __forceinline__ void tr_insert(TrWave<LOG> &s, u32 &nused, const u32 (&w)[N], const u32 (&mk)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const u64 any = __ballot(mk[i] != 0);
+        if (!any) continue;   // uniform
+        bool fresh = false;
+        u32 h = tr_hash<LOG>(w[i]);
+        if (mk[i]) {
+            while (true) {
+                const u32 k = atomicCAS(&s.key[h], TR_EMPTY, w[i]);
+                if (k == TR_EMPTY || k == w[i]) {
+                    fresh = k == TR_EMPTY;
+                    break;
+                }
+                h = (h + 1) & (TrCfg<LOG>::H - 1);
+            }
+            atomicOr(&s.bits[h], mk[i]);
         }
-        if (k == w) break;
-        h = (h + 1) & (TrCfg<LOG>::H - 1);
+        const u64 bal = __ballot(fresh);
+        if (fresh)
+            s.used[nused + __builtin_amdgcn_mbcnt_hi((u32)(bal >> 32), __builtin_amdgcn_mbcnt_lo((u32)bal, 0u))] =
+                (u16)h;
+        nused += (u32)__popcll(bal);
     }
-    atomicOr(&s.bits[h], mk);
 }
 
 // Flush the wave's table into row `rowbase` of F: per occupied slot a read
 // and, for new bits, a write of the word (TR_U words per lane in flight);
-// the new bits are written as ends z (packed) to dst[0..), in slot order.
-// Returns the number written.  The table is left empty.
+// the new bits are written as ends z (packed) to dst[0..).  Returns the
+// number written.  The table is left empty.
 template <int LOG>
-__device__ u64 tr_flush(TrWave<LOG> &s, u32 *F, u64 rowbase, u32 lo, u32 *dst) {
+__device__ u64 tr_flush(TrWave<LOG> &s, u32 &nused, u32 *F, u64 rowbase, u32 lo, u32 *dst) {
     const int lane = threadIdx.x & (WAVE - 1);
     wave_sync();
-    const u32 n = s.nused;
+    const u32 n = nused;
     u64 cnt = 0;
     for (u32 j0 = 0; j0 < n; j0 += TR_U * WAVE) {
-        u32 slot[TR_U], w[TR_U], b[TR_U], cur[TR_U];
+        u32 slot[TR_U], w[TR_U], b[TR_U], nb[TR_U];
 #pragma unroll
         for (int q = 0; q < TR_U; q++) {
-            u32 j = j0 + q * WAVE + lane;
-            slot[q] = j < n ? s.used[j] : 0u;
-            w[q] = j < n ? s.key[slot[q]] : 0u;
-            b[q] = j < n ? s.bits[slot[q]] : 0u;
+            const u32 j = j0 + q * WAVE + lane;
+            slot[q] = j < n ? s.used[j] : TR_EMPTY;
         }
 #pragma unroll
-        for (int q = 0; q < TR_U; q++) cur[q] = b[q] ? F[rowbase + w[q]] : 0u;   // the row's owner:
-#pragma unroll                                                              // plain read-modify-write
         for (int q = 0; q < TR_U; q++) {
-            u32 nb = b[q] & ~cur[q];
-            if (nb) F[rowbase + w[q]] = cur[q] | nb;
-            if (b[q]) s.bits[slot[q]] = nb;
-            cnt += __popc(nb);
+            w[q] = slot[q] != TR_EMPTY ? s.key[slot[q]] : 0u;
+            b[q] = slot[q] != TR_EMPTY ? s.bits[slot[q]] : 0u;
         }
-    }
-    u64 tot;
-    u64 off = wave_excl_scan(cnt, &tot);
-    for (u32 j = lane; j < n; j += WAVE) {
-        u32 slot = s.used[j];
-        u32 w = s.key[slot], nb = s.bits[slot];
-        while (nb) {
-            dst[off++] = lo + w * 32 + (u32)(__ffs(nb) - 1);
-            nb &= nb - 1;
+#pragma unroll
+        for (int q = 0; q < TR_U; q++) nb[q] = b[q] ? F[rowbase + w[q]] : 0u;   // the row's owner:
+        u32 c = 0;                                                           // plain read-modify-write
+#pragma unroll
+        for (int q = 0; q < TR_U; q++) {
+            const u32 cur = nb[q];
+            nb[q] = b[q] & ~cur;
+            if (nb[q]) F[rowbase + w[q]] = cur | nb[q];
+            if (slot[q] != TR_EMPTY) {
+                s.key[slot[q]] = TR_EMPTY;
+                s.bits[slot[q]] = 0u;
+            }
+            c += (u32)__popc(nb[q]);
         }
-        s.key[slot] = TR_EMPTY;
-        s.bits[slot] = 0u;
+        u32 tot;
+        u32 off = wave_excl_scan32(c, &tot);
+        u32 *d = dst + cnt;
+#pragma unroll
+        for (int q = 0; q < TR_U; q++) {
+            for (u32 v = nb[q]; v; v &= v - 1) d[off++] = lo + w[q] * 32 + (u32)(__ffs(v) - 1);
+        }
+        cnt += tot;
     }
     wave_sync();
-    if (lane == 0) s.nused = 0;
-    wave_sync();
-    return tot;
+    nused = 0;
+    return cnt;
 }
 
-// One round.  Segment s (source seg_x[s], ends dy[seg_beg[s] ..][0, seg_len[s]))
-// writes its new ends at ny[seg_out[s] ..] -- seg_out is the exclusive scan
-// of the segments' next-round bounds -- and its next length and bound at the
-// same index s.  The bound is the segment's next join rows, Σ deg over its
-// new ends (exact; `dmax` = 0), or, when every degree is at most dmax,
+// One step: flush first if the step's entries could take the table past
+// LIMIT, then insert them.  Returns the ends the flush wrote to dst.
+template <int LOG, int N>
+__device__ __forceinline__ u64 tr_step(TrWave<LOG> &s, u32 &nused, const u32 (&w)[N], const u32 (&mk)[N], u32 *F,
+                                       u64 rowbase, u32 lo, u32 *dst) {
+    u32 m = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) m += (u32)__popcll(__ballot(mk[i] != 0));
+    u64 c = 0;
+    if (nused + m > TrCfg<LOG>::LIMIT) c = tr_flush(s, nused, F, rowbase, lo, dst);
+    tr_insert(s, nused, w, mk);
+    return c;
+}
+
+// One round over the active list (the sources with ends): entry i (source
+// ax[i], ends dy[abeg[i] ..][0, alen[i])) writes its new ends at
+// ny[aoff[i] ..] -- aoff is the exclusive scan of the entries' output
+// bounds, span its total -- and its new end count and next bound at rcnt[i],
+// rbound[i].  The bound is the source's next join rows, Σ deg over its new
+// ends (exact; `dmax` = 0), or, when every degree is at most dmax,
 // min(new ends * dmax, W) (no degree lookups; the output has gaps).  The
-// round's join rows are counted on the way (counters[2]).  Workgroups take
-// static ranges of segments; the four waves of a workgroup share the range
-// through an LDS counter, so no global counter is contended.
+// round's new ends and join rows are counted on the way (counters[0],
+// counters[2]).  An entry costs about its bound (join rows) plus a fixed
+// TR_ENTRY_COST, so with cost prefix c(i) = aoff[i] + TR_ENTRY_COST * i,
+// workgroup b takes the entries with c(i) in [b, b + 1) * c(nact) / grid (a
+// binary search; no global counter), and its four waves share them through
+// an LDS counter.
 template <int LOG>
-__global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_x, const u64 *seg_beg,
-                                                  const u32 *seg_len, const u64 *seg_out, u64 nseg,
-                                                  const u64 *csr_off, const u32 *csr_dst, const u32 *deg,
-                                                  u32 dmax, u32 *F, const u64 *row_base, const u32 *col_lo,
-                                                  u32 lo, u64 wcap, u32 *ny, u32 *nseg_len, u64 *next_rows,
-                                                  unsigned long long *counters) {
+__global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
+    const u32 *dy, const u32 *ax, const u64 *abeg, const u32 *alen, const u64 *aoff, u64 nact, u64 span,
+    const u64 *csr_off, const u32 *csr_dst, const u32 *deg, u32 dmax, u32 *F, const u64 *row_base,
+    const u32 *col_lo, u32 lo, u64 wcap, u32 *ny, u32 *rcnt, u64 *rbound, unsigned long long *counters) {
     using C = TrCfg<LOG>;
     __shared__ TrWave<LOG> sw[TR_W];
+    __shared__ u64 s_rng[2];
     __shared__ u32 s_next;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     TrWave<LOG> &s = sw[wv];
@@ -171,136 +205,155 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
         s.key[i] = TR_EMPTY;
         s.bits[i] = 0u;
     }
-    if (lane == 0) s.nused = 0;
-    const u64 per = (nseg + gridDim.x - 1) / gridDim.x;
-    const u64 b0 = (u64)blockIdx.x * per, b1 = b0 + per < nseg ? b0 + per : nseg;
+    if (threadIdx.x < 2) {   // the first entries of this workgroup's share and of the next one's
+        const u64 b = (u64)blockIdx.x + threadIdx.x, tot = span + TR_ENTRY_COST * nact;
+        const u64 key = (b * tot + gridDim.x - 1) / gridDim.x;
+        u64 l = 0, h = nact;
+        while (l < h) {
+            const u64 m = (l + h) >> 1;
+            if (aoff[m] + TR_ENTRY_COST * m < key) l = m + 1; else h = m;
+        }
+        s_rng[threadIdx.x] = b >= gridDim.x ? nact : l;
+    }
     if (threadIdx.x == 0) s_next = 0;
     __syncthreads();
+    const u64 r0 = s_rng[0], r1 = s_rng[1];
+    u32 nused = 0;   // the wave's occupied slots (uniform)
     u64 wave_new = 0, wave_rows = 0;
     while (true) {
         u32 k0 = 0;
         if (lane == 0) k0 = atomicAdd(&s_next, 1u);
-        const u64 sg = b0 + __shfl(k0, 0, WAVE);
-        if (sg >= b1) break;
-        const u32 len = seg_len[sg];
-        u64 rows_next = 0, cnt = 0;
-        if (len) {
-            const u32 x = seg_x[sg];
-            const u64 beg = seg_beg[sg], out0 = seg_out[sg];
-            const u64 rowbase = row_base[x - lo];   // x's row and its block's first id
-            const u32 clo = col_lo[x - lo];
-            // the chunks' ends are loaded one chunk ahead (software pipelining)
-            u32 yn[2];
+        const u64 ai = r0 + __shfl(k0, 0, WAVE);
+        if (ai >= r1) break;
+        {
+            const u32 len = alen[ai];
+            u64 rows_next = 0, cnt = 0;
+            if (len) {
+                const u32 x = ax[ai];
+                const u64 beg = abeg[ai], out0 = aoff[ai];
+                const u64 rowbase = row_base[x - lo];   // x's row and its block's first id
+                const u32 clo = col_lo[x - lo];
+                // the chunks' ends are loaded one chunk ahead (software pipelining)
+                u32 yn[2];
 #pragma unroll
-            for (int h = 0; h < 2; h++) yn[h] = (u32)(h * WAVE + lane) < len ? dy[beg + h * WAVE + lane] : 0u;
-            for (u32 pc = 0; pc < len; pc += 2 * WAVE) {
-                // two probes per lane and their adjacency ranges
-                u64 st[2] = {0, 0};
-                u32 d[2] = {0, 0};
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    if (pc + h * WAVE + lane < len) {
-                        u32 y = yn[h] - lo;
-                        st[h] = csr_off[y];
-                        d[h] = deg[y];
-                    }
-                }
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    u32 i = pc + 2 * WAVE + h * WAVE + lane;
-                    yn[h] = i < len ? dy[beg + i] : 0u;
-                }
-                wave_rows += d[0] + d[1];
-                u32 maxd = max(d[0], d[1]);
-#pragma unroll
-                for (int o = WAVE / 2; o > 0; o >>= 1) maxd = max(maxd, (u32)__shfl_xor((int)maxd, o, WAVE));
-                if (maxd <= (u32)TR_SMALL) {
-                    // lane per probe: the matches of both probes gathered at once, then inserted
-                    u32 zs[2][TR_SMALL];
-#pragma unroll
-                    for (int h = 0; h < 2; h++)
-#pragma unroll
-                        for (int i = 0; i < TR_SMALL; i++) zs[h][i] = (u32)i < d[h] ? csr_dst[st[h] + i] : 0u;
+                for (int h = 0; h < 2; h++) yn[h] = (u32)(h * WAVE + lane) < len ? dy[beg + h * WAVE + lane] : 0u;
+                for (u32 pc = 0; pc < len; pc += 2 * WAVE) {
+                    // two probes per lane and their adjacency ranges
+                    u64 st[2] = {0, 0};
+                    u32 d[2] = {0, 0};
 #pragma unroll
                     for (int h = 0; h < 2; h++) {
-#pragma unroll
-                        for (int i0 = 0; i0 < TR_SMALL; i0 += C::STEP) {
-                            if ((u32)i0 < maxd) {   // uniform
-                                if (s.nused > (u32)C::FLUSH_SMALL)   // uniform: every lane reads the same LDS word
-                                    cnt += tr_flush(s, F, rowbase, clo, ny + out0 + cnt);
-#pragma unroll
-                                for (int i = i0; i < i0 + C::STEP; i++)
-                                    if ((u32)i < d[h]) tr_insert(s, zs[h][i] - clo);
-                                wave_sync();
-                            }
+                        if (pc + h * WAVE + lane < len) {
+                            u32 y = yn[h] - lo;
+                            st[h] = csr_off[y];
+                            d[h] = deg[y];
                         }
                     }
-                    continue;
-                }
-                // load-balanced rows, one half-chunk of 64 probes at a time
-#pragma unroll 1
-                for (int h = 0; h < 2; h++) {
-                    u64 R;
-                    u64 ex = wave_excl_scan(d[h], &R);
-                    wave_sync();
-                    s.p_off[lane] = ex;
-                    s.p_start[lane] = st[h];
-                    if (lane == 0) s.p_off[WAVE] = R;
-                    wave_sync();
-                    for (u64 rb = 0; rb < R; rb += TR_BATCH) {
-                        if (s.nused > (u32)C::FLUSH)   // uniform: every lane reads the same LDS word
-                            cnt += tr_flush(s, F, rowbase, clo, ny + out0 + cnt);
-                        const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
-                        u64 ci[TR_U];
 #pragma unroll
-                        for (int q = 0; q < TR_U; q++) {   // locate TR_U rows per lane ...
-                            u64 r = rb + (u64)q * WAVE + lane;
-                            ci[q] = ~0ull;
-                            if (r < re) {
-                                u32 a = 0, b = WAVE;   // last probe with p_off <= r
-                                while (b - a > 1) {
-                                    u32 m = (a + b) >> 1;
-                                    if (s.p_off[m] <= r) a = m; else b = m;
+                    for (int h = 0; h < 2; h++) {
+                        u32 i = pc + 2 * WAVE + h * WAVE + lane;
+                        yn[h] = i < len ? dy[beg + i] : 0u;
+                    }
+                    wave_rows += d[0] + d[1];
+                    u32 maxd = max(d[0], d[1]);
+#pragma unroll
+                    for (int o = WAVE / 2; o > 0; o >>= 1) maxd = max(maxd, (u32)__shfl_xor((int)maxd, o, WAVE));
+                    if (maxd <= (u32)TR_SMALL) {
+                        // lane per probe: the matches of both probes gathered at once, then inserted
+                        u32 zs[2][TR_SMALL];
+#pragma unroll
+                        for (int h = 0; h < 2; h++)
+#pragma unroll
+                            for (int i = 0; i < TR_SMALL; i++) zs[h][i] = (u32)i < d[h] ? csr_dst[st[h] + i] : 0u;
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+#pragma unroll
+                            for (int g0 = 0; g0 < TR_SMALL; g0 += C::NI) {
+                                if ((u32)g0 < maxd) {   // uniform
+                                    u32 w[C::NI], mk[C::NI];
+#pragma unroll
+                                    for (int i = 0; i < C::NI; i++) {
+                                        const u32 z = zs[h][g0 + i] - clo;
+                                        w[i] = z >> 5;
+                                        mk[i] = (u32)(g0 + i) < d[h] ? 1u << (z & 31) : 0u;
+                                    }
+                                    // a probe's matches are sorted: runs in one word merge into one entry
+#pragma unroll
+                                    for (int i = C::NI - 1; i > 0; i--)
+                                        if (mk[i] && mk[i - 1] && w[i] == w[i - 1]) {
+                                            mk[i - 1] |= mk[i];
+                                            mk[i] = 0u;
+                                        }
+                                    cnt += tr_step(s, nused, w, mk, F, rowbase, clo, ny + out0 + cnt);
                                 }
-                                ci[q] = s.p_start[a] + (r - s.p_off[a]);
                             }
                         }
-                        u32 zq[TR_U];
-#pragma unroll
-                        for (int q = 0; q < TR_U; q++) zq[q] = ci[q] != ~0ull ? csr_dst[ci[q]] : 0u;   // ... gather
-#pragma unroll
-                        for (int q = 0; q < TR_U; q++)   // ... insert
-                            if (ci[q] != ~0ull) tr_insert(s, zq[q] - clo);
+                        continue;
+                    }
+                    // load-balanced rows, one half-chunk of 64 probes at a time
+#pragma unroll 1
+                    for (int h = 0; h < 2; h++) {
+                        u64 R;
+                        u64 ex = wave_excl_scan(d[h], &R);
                         wave_sync();
+                        s.p_off[lane] = ex;
+                        s.p_start[lane] = st[h];
+                        if (lane == 0) s.p_off[WAVE] = R;
+                        wave_sync();
+                        for (u64 rb = 0; rb < R; rb += TR_BATCH) {
+                            const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
+                            u64 ci[TR_U];
+#pragma unroll
+                            for (int q = 0; q < TR_U; q++) {   // locate TR_U rows per lane ...
+                                u64 r = rb + (u64)q * WAVE + lane;
+                                ci[q] = ~0ull;
+                                if (r < re) {
+                                    u32 a = 0, b = WAVE;   // last probe with p_off <= r
+                                    while (b - a > 1) {
+                                        u32 m = (a + b) >> 1;
+                                        if (s.p_off[m] <= r) a = m; else b = m;
+                                    }
+                                    ci[q] = s.p_start[a] + (r - s.p_off[a]);
+                                }
+                            }
+                            u32 w[TR_U], mk[TR_U];
+#pragma unroll
+                            for (int q = 0; q < TR_U; q++) {   // ... gather ...
+                                const u32 z = (ci[q] != ~0ull ? csr_dst[ci[q]] : clo) - clo;
+                                w[q] = z >> 5;
+                                mk[q] = ci[q] != ~0ull ? 1u << (z & 31) : 0u;
+                            }
+                            cnt += tr_step(s, nused, w, mk, F, rowbase, clo, ny + out0 + cnt);   // ... insert
+                        }
+                    }
+                }
+                cnt += tr_flush(s, nused, F, rowbase, clo, ny + out0 + cnt);
+                if (dmax) {
+                    rows_next = lane == 0 ? min(cnt * dmax, wcap) : 0;
+                } else {
+                    // the next round's join rows of this segment: Σ deg over its new ends
+                    wave_sync();
+                    for (u64 i0 = 0; i0 < cnt; i0 += TR_U * WAVE) {
+                        u32 zz[TR_U];
+#pragma unroll
+                        for (int q = 0; q < TR_U; q++) {
+                            const u64 i = i0 + (u64)q * WAVE + lane;
+                            zz[q] = i < cnt ? ny[out0 + i] : lo;
+                        }
+#pragma unroll
+                        for (int q = 0; q < TR_U; q++)   // every id is an end, the range's last one included
+                            if (i0 + (u64)q * WAVE + lane < cnt) rows_next += deg[zz[q] - lo];
                     }
                 }
             }
-            cnt += tr_flush(s, F, rowbase, clo, ny + out0 + cnt);
-            if (dmax) {
-                rows_next = lane == 0 ? min(cnt * dmax, wcap) : 0;
-            } else {
-                // the next round's join rows of this segment: Σ deg over its new ends
-                wave_sync();
-                for (u64 i0 = 0; i0 < cnt; i0 += TR_U * WAVE) {
-                    u32 zz[TR_U];
 #pragma unroll
-                    for (int q = 0; q < TR_U; q++) {
-                        u64 i = i0 + (u64)q * WAVE + lane;
-                        zz[q] = i < cnt ? ny[out0 + i] : TR_EMPTY;
-                    }
-#pragma unroll
-                    for (int q = 0; q < TR_U; q++)
-                        if (zz[q] != TR_EMPTY) rows_next += deg[zz[q] - lo];
-                }
+            for (int o = WAVE / 2; o > 0; o >>= 1) rows_next += __shfl_xor(rows_next, o, WAVE);
+            if (lane == 0) {
+                rcnt[ai] = (u32)cnt;
+                rbound[ai] = rows_next;
             }
+            wave_new += cnt;
         }
-#pragma unroll
-        for (int o = WAVE / 2; o > 0; o >>= 1) rows_next += __shfl_xor(rows_next, o, WAVE);
-        if (lane == 0) {
-            nseg_len[sg] = (u32)cnt;
-            next_rows[sg] = rows_next;
-        }
-        wave_new += cnt;
     }
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) wave_rows += __shfl_xor(wave_rows, o, WAVE);
@@ -308,17 +361,92 @@ __global__ __launch_bounds__(TR_T) void tc_rows_k(const u32 *dy, const u32 *seg_
     if (lane == 0 && wave_rows) atomicAdd(counters + 2, (unsigned long long)wave_rows);
 }
 
-// Σ deg over each segment's ends (the first round's join rows per segment).
-__global__ void tc_rows_segrows_k(const u32 *dy, const u64 *seg_beg, const u32 *seg_len, u64 nseg,
-                                  const u64 *csr_off, u32 lo, u64 *rows) {
+// Σ deg over each entry's ends (the first round's join rows per source).
+__global__ void tc_rows_segrows_k(const u32 *dy, const u64 *abeg, const u32 *alen, u64 nact, const u64 *csr_off,
+                                  u32 lo, u64 *rows) {
     u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nseg; j += stride) {
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nact; j += stride) {
         u64 s = 0;
-        for (u32 i = 0; i < seg_len[j]; i++) {
-            u32 y = dy[seg_beg[j] + i] - lo;
+        for (u32 i = 0; i < alen[j]; i++) {
+            u32 y = dy[abeg[j] + i] - lo;
             s += csr_off[y + 1] - csr_off[y];
         }
         rows[j] = s;
+    }
+}
+
+// The next active list: the entries with new ends, in order, each with its
+// ends' place (its output offset this round), their count and its next
+// output offset (the exclusive scan of the bounds).  Chunks of TA_CHUNK
+// entries: tc_rows_part_k sums each chunk, tc_rows_compact_k adds the sums
+// of the chunks before its own and writes; its last block writes the totals
+// (counters[1] = Σ bounds, counters[3] = the list's length).
+constexpr int TA_T = 256, TA_I = 8;
+constexpr u64 TA_CHUNK = (u64)TA_T * TA_I;
+
+__global__ __launch_bounds__(TA_T) void tc_rows_part_k(const u32 *rcnt, const u64 *rbound, u64 nact, u64 *part) {
+    __shared__ u64 scr[TA_T / WAVE + 1];
+    const u64 b0 = (u64)blockIdx.x * TA_CHUNK;
+    u64 c = 0, s = 0;
+#pragma unroll
+    for (int k = 0; k < TA_I; k++) {
+        const u64 i = b0 + (u64)k * TA_T + threadIdx.x;
+        if (i < nact && rcnt[i]) {
+            c++;
+            s += rbound[i];
+        }
+    }
+    u64 tc, ts;
+    block_exclusive_scan(c, scr, &tc);
+    block_exclusive_scan(s, scr, &ts);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = tc;
+        part[2 * blockIdx.x + 1] = ts;
+    }
+}
+
+__global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u64 *aoff, const u32 *rcnt,
+                                                          const u64 *rbound, u64 nact, const u64 *part, u32 *nx,
+                                                          u64 *nbeg, u32 *nlen, u64 *noff,
+                                                          unsigned long long *counters) {
+    __shared__ u64 scr[TA_T / WAVE + 1];
+    u64 pc = 0, ps = 0;   // the chunks before this one
+    for (u32 j = threadIdx.x; j < blockIdx.x; j += TA_T) {
+        pc += part[2 * j];
+        ps += part[2 * j + 1];
+    }
+    u64 bc, bs;
+    block_exclusive_scan(pc, scr, &bc);
+    block_exclusive_scan(ps, scr, &bs);
+    const u64 b0 = (u64)blockIdx.x * TA_CHUNK + (u64)threadIdx.x * TA_I;   // TA_I consecutive entries
+    u32 c = 0;
+    u64 s = 0;
+#pragma unroll
+    for (int k = 0; k < TA_I; k++) {
+        const u64 i = b0 + k;
+        if (i < nact && rcnt[i]) {
+            c++;
+            s += rbound[i];
+        }
+    }
+    u64 tc, ts;
+    u64 ec = block_exclusive_scan((u64)c, scr, &tc) + bc;
+    u64 es = block_exclusive_scan(s, scr, &ts) + bs;
+#pragma unroll
+    for (int k = 0; k < TA_I; k++) {
+        const u64 i = b0 + k;
+        if (i < nact && rcnt[i]) {
+            nx[ec] = ax[i];
+            nbeg[ec] = aoff[i];
+            nlen[ec] = rcnt[i];
+            noff[ec] = es;
+            ec++;
+            es += rbound[i];
+        }
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        counters[1] = bs + ts;
+        counters[3] = bc + tc;
     }
 }
 
@@ -332,11 +460,11 @@ __global__ void csr_dst_k(const u64 *rows, u64 n, u32 *dst) {
     u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = tval(rows[i]);
 }
-// Δ0 from the unique sorted edges (src, dst): ends = dst, one segment per src
-// run (heads = run starts), and the edges' bits (row src, column dst) in F.
-__global__ void tc_rows_init_k(const u64 *uniq, u64 n, const u32 *heads, u64 nh, u32 *dy, u32 *seg_x,
-                               u64 *seg_beg, u32 *seg_len, u32 *F, const u64 *row_base, const u32 *col_lo,
-                               u32 lo) {
+// Δ0 from the unique sorted edges (src, dst): ends = dst, one active entry
+// per src run (heads = run starts), and the edges' bits (row src, column
+// dst) in F.
+__global__ void tc_rows_init_k(const u64 *uniq, u64 n, const u32 *heads, u64 nh, u32 *dy, u32 *ax, u64 *abeg,
+                               u32 *alen, u32 *F, const u64 *row_base, const u32 *col_lo, u32 lo) {
     u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         u64 t = uniq[i];
@@ -346,22 +474,17 @@ __global__ void tc_rows_init_k(const u64 *uniq, u64 n, const u32 *heads, u64 nh,
     }
     for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += stride) {
         u64 b = heads[j], e = j + 1 < nh ? heads[j + 1] : n;
-        seg_x[j] = tkey(uniq[b]);
-        seg_beg[j] = b;
-        seg_len[j] = (u32)(e - b);
+        ax[j] = tkey(uniq[b]);
+        abeg[j] = b;
+        alen[j] = (u32)(e - b);
     }
 }
-// Segments back to (z, x) tuples, compacted (the ends have gaps between
-// segments), for a round hook.
-__global__ void tc_rows_tuples_k(const u32 *dy, const u32 *seg_x, const u64 *seg_beg, const u32 *seg_len,
-                                 const u64 *pos, u64 nseg, u64 *out) {
-    for (u64 j = blockIdx.x; j < nseg; j += gridDim.x)
-        for (u32 i = threadIdx.x; i < seg_len[j]; i += blockDim.x)
-            out[pos[j] + i] = tmake(dy[seg_beg[j] + i], seg_x[j]);
-}
-__global__ void u32_to_u64_k(const u32 *in, u64 n, u64 *out) {
-    u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = in[i];
+// The active list back to (z, x) tuples, compacted (the ends have gaps
+// between entries), for a round hook.
+__global__ void tc_rows_tuples_k(const u32 *dy, const u32 *ax, const u64 *abeg, const u32 *alen, const u64 *pos,
+                                 u64 nact, u64 *out) {
+    for (u64 j = blockIdx.x; j < nact; j += gridDim.x)
+        for (u32 i = threadIdx.x; i < alen[j]; i += blockDim.x) out[pos[j] + i] = tmake(dy[abeg[j] + i], ax[j]);
 }
 
 void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
@@ -394,26 +517,24 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
     DBuf<u32> heads(ctx, nu ? nu : 1);
     u64 nh = nu ? select(ctx, SEL_KEY_HEAD, uniq.p, nu, nullptr, 0, (u64 *)heads.p) : 0;
     tr.dy.ensure(nu ? nu : 1);
-    tr.seg_x.ensure(nh ? nh : 1);
-    tr.seg_beg.ensure(nh ? nh : 1);
-    tr.seg_len.ensure(nh ? nh : 1);
-    tr.nseg_len.ensure(nh ? nh : 1);
-    tr.seg_out.ensure(nh ? nh : 1);
-    tr.next_rows.ensure(nh ? nh : 1);
+    tr.ax.ensure(nh ? nh : 1);
+    tr.abeg.ensure(nh ? nh : 1);
+    tr.alen.ensure(nh ? nh : 1);
+    tr.aoff.ensure(nh ? nh : 1);
     tr.counters.ensure(4);
     tr.nd = nu;
-    tr.nseg = nh;
+    tr.nact = nh;
     tr.rows = 0;
     tr.wcap = 0;
     for (const auto &b : F.blocks) tr.wcap = std::max<u64>(tr.wcap, b.W);
     if (!nu) return;
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 20.0 * nu, tc_rows_init_k, sgrid(ctx, nu), ctx->block_size, 0,
-               (const u64 *)uniq.p, nu, (const u32 *)heads.p, nh, tr.dy.p, tr.seg_x.p, tr.seg_beg.p,
-               tr.seg_len.p, F.words.p, (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo);
+               (const u64 *)uniq.p, nu, (const u32 *)heads.p, nh, tr.dy.p, tr.ax.p, tr.abeg.p, tr.alen.p,
+               F.words.p, (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo);
     DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * nu, tc_rows_segrows_k, sgrid(ctx, nh), ctx->block_size, 0,
-               (const u32 *)tr.dy.p, (const u64 *)tr.seg_beg.p, (const u32 *)tr.seg_len.p, nh,
-               (const u64 *)tr.csr_off.p, F.lo, tr.seg_out.p);
-    scan_u64_async(ctx, tr.seg_out.p, nh, tr.seg_out.p, tr.counters.p + 1);
+               (const u32 *)tr.dy.p, (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, nh, (const u64 *)tr.csr_off.p,
+               F.lo, tr.aoff.p);
+    scan_u64_async(ctx, tr.aoff.p, nh, tr.aoff.p, tr.counters.p + 1);
     u64 h[2];
     ctx->read_scalars(tr.counters.p, 2, h);
     tr.rows = h[1];
@@ -423,12 +544,12 @@ constexpr u64 TR_LONG_SEGMENTS = 64;   // average probes per segment above which
 
 template <int LOG>
 static void launch_rows(dlg_ctx *ctx, RowMap &F, TcRows &tr) {
-    unsigned g = (unsigned)std::min<u64>((tr.nseg + TR_W - 1) / TR_W, (u64)ctx->num_cus * TrCfg<LOG>::BLOCKS_PER_CU);
-    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, tc_rows_k<LOG>, g, TR_T, 0, (const u32 *)tr.dy.p, (const u32 *)tr.seg_x.p,
-               (const u64 *)tr.seg_beg.p, (const u32 *)tr.seg_len.p, (const u64 *)tr.seg_out.p, tr.nseg,
+    unsigned g = (unsigned)std::min<u64>((tr.nact + TR_W - 1) / TR_W, (u64)ctx->num_cus * TrCfg<LOG>::BLOCKS_PER_CU);
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, tc_rows_k<LOG>, g, TR_T, 0, (const u32 *)tr.dy.p, (const u32 *)tr.ax.p,
+               (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, (const u64 *)tr.aoff.p, tr.nact, tr.rows,
                (const u64 *)tr.csr_off.p, (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, tr.dmax, F.words.p,
-               (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo, tr.wcap, tr.ny.p, tr.nseg_len.p,
-               tr.next_rows.p, (unsigned long long *)tr.counters.p);
+               (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo, tr.wcap, tr.ny.p, tr.rcnt.p, tr.rbound.p,
+               (unsigned long long *)tr.counters.p);
 }
 
 u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
@@ -436,22 +557,39 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
         tr.nd = 0;
         return 0;
     }
+    const u64 na = tr.nact;
     tr.ny.ensure(tr.rows);
-    fill_u64(ctx, tr.counters.p, 3, 0ull);
+    tr.rcnt.ensure(na);
+    tr.rbound.ensure(na);
+    fill_u64(ctx, tr.counters.p, 4, 0ull);   // new ends, next bound total, join rows, next list length
     // algorithmic bytes, added once the round's join rows are known: 4 B per
     // join row (its matched end) + 20 B per probe (its end, CSR offset, degree)
     const u64 nd0 = tr.nd;
-    if (tr.nd >= TR_LONG_SEGMENTS * tr.nseg) launch_rows<10>(ctx, F, tr);   // long segments: big tables
-    else launch_rows<9>(ctx, F, tr);                                        // short ones: more waves
-    // the new ends sit at the old output offsets; the next offsets scan the next rows
-    tr.seg_beg.swap(tr.seg_out);
-    scan_u64_async(ctx, tr.next_rows.p, tr.nseg, tr.seg_out.p, tr.counters.p + 1);
-    u64 h[3];
-    ctx->read_scalars(tr.counters.p, 3, h);
+    if (tr.nd >= TR_LONG_SEGMENTS * na) launch_rows<10>(ctx, F, tr);   // long segments: big tables
+    else launch_rows<9>(ctx, F, tr);                                   // short ones: more waves
+    // the next active list; its ends sit at this round's output offsets
+    const u64 nb = (na + TA_CHUNK - 1) / TA_CHUNK;
+    tr.part.ensure(2 * nb);
+    tr.nx.ensure(na);
+    tr.nbeg.ensure(na);
+    tr.nlen.ensure(na);
+    tr.noff.ensure(na);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * na, tc_rows_part_k, nb, TA_T, 0, (const u32 *)tr.rcnt.p,
+               (const u64 *)tr.rbound.p, na, tr.part.p);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 48.0 * na, tc_rows_compact_k, nb, TA_T, 0, (const u32 *)tr.ax.p,
+               (const u64 *)tr.aoff.p, (const u32 *)tr.rcnt.p, (const u64 *)tr.rbound.p, na,
+               (const u64 *)tr.part.p, tr.nx.p, tr.nbeg.p, tr.nlen.p, tr.noff.p,
+               (unsigned long long *)tr.counters.p);
+    u64 h[4];
+    ctx->read_scalars(tr.counters.p, 4, h);
     tr.dy.swap(tr.ny);
-    tr.seg_len.swap(tr.nseg_len);
+    tr.ax.swap(tr.nx);
+    tr.abeg.swap(tr.nbeg);
+    tr.alen.swap(tr.nlen);
+    tr.aoff.swap(tr.noff);
     tr.nd = h[0];
     tr.rows = h[1];
+    tr.nact = h[3];
     if (raw) *raw += h[2];
     ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[2] + 20.0 * nd0);
     return h[0];
@@ -517,16 +655,13 @@ u64 rowmap_emit(dlg_ctx *ctx, RowMap &m, DBuf<u64> &out, u64 expect) {
 void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out) {
     out.ensure(tr.nd ? tr.nd : 1);
     if (!tr.nd) return;
-    DBuf<u64> pos(ctx, tr.nseg);
+    DBuf<u64> pos(ctx, tr.nact);
     DBuf<u64> tot(ctx, 1);
-    DBuf<u64> lens(ctx, tr.nseg);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * tr.nseg, u32_to_u64_k, sgrid(ctx, tr.nseg), ctx->block_size, 0,
-               (const u32 *)tr.seg_len.p, tr.nseg, lens.p);
-    scan_u64_async(ctx, lens.p, tr.nseg, pos.p, tot.p);
-    unsigned g = (unsigned)std::min<u64>(tr.nseg, (u64)ctx->num_cus * 8);
+    scan_u32_u64_async(ctx, tr.alen.p, tr.nact, pos.p, tot.p);
+    unsigned g = (unsigned)std::min<u64>(tr.nact, (u64)ctx->num_cus * 8);
     DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * tr.nd, tc_rows_tuples_k, g, 256, 0, (const u32 *)tr.dy.p,
-               (const u32 *)tr.seg_x.p, (const u64 *)tr.seg_beg.p, (const u32 *)tr.seg_len.p,
-               (const u64 *)pos.p, tr.nseg, out.p);
+               (const u32 *)tr.ax.p, (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, (const u64 *)pos.p, tr.nact,
+               out.p);
 }
 
 }  // namespace dlg
