@@ -374,7 +374,7 @@ u64 rowmap_emit(dlg_ctx *ctx, RowMap &m, DBuf<u64> &out, u64 expect);
 // output bounds = where their new ends go.
 struct TcRows {
     DBuf<u64> csr_off;
-    DBuf<u32> csr_dst, deg;
+    DBuf<u32> csr_dst, deg, ell;   // ell: ELL rows when every degree is small (else unallocated)
     // Δ: ends dy grouped by source, and the active list (the sources with
     // ends): source ax, its ends at dy[abeg ..][0, alen), its output offset
     // aoff (exclusive scan of the bounds); n*: the next round's.  rcnt,
@@ -385,7 +385,7 @@ struct TcRows {
     u32 dmax = 0;                     // nonzero: every degree <= dmax, bounds = new ends * dmax
     u64 wcap = 0;                     // the widest block: no source has more new ends
     explicit TcRows(dlg_ctx *c)
-        : csr_off(c), csr_dst(c), deg(c), dy(c), ny(c), ax(c), nx(c), alen(c), nlen(c), rcnt(c), abeg(c), nbeg(c),
+        : csr_off(c), csr_dst(c), deg(c), ell(c), dy(c), ny(c), ax(c), nx(c), alen(c), nlen(c), rcnt(c), abeg(c), nbeg(c),
           aoff(c), noff(c), rbound(c), part(c), counters(c) {}
 };
 // CSR of the index's rows; Δ0 = unique(E reversed) grouped by source, its

@@ -30,6 +30,7 @@ constexpr int TR_T = 256;                    // threads per workgroup: four wave
 constexpr int TR_W = TR_T / WAVE;
 constexpr u32 TR_EMPTY = 0xffffffffu;
 constexpr int TR_SMALL = 8;   // chunks whose probes all have at most this many matches: lane per probe
+constexpr int TR_ELL_K = 8;   // degrees up to this: the join reads an ELL adjacency (one 32-byte row per id)
 constexpr int TR_U = 4;       // words / rows per lane with their loads in flight together
 constexpr int TR_BATCH = TR_U * WAVE;        // rows per batch of the load-balanced path
 constexpr u32 TR_DMAX_BOUND = 16;   // degrees up to this: output regions bounded by new ends * max degree
@@ -46,7 +47,8 @@ struct TrCfg {
     static constexpr int NI = LOG >= 10 ? 8 : 4;
     static constexpr u32 LIMIT = H * 3 / 4;
     static constexpr int BLOCKS_PER_CU = LOG >= 10 ? 3 : 4;   // what LDS (big table) or registers allow
-    static_assert(NI * WAVE <= (int)LIMIT && TR_BATCH <= (int)LIMIT && TR_SMALL % NI == 0, "table load");
+    static_assert(NI * WAVE <= (int)LIMIT && TR_BATCH <= (int)LIMIT && TR_SMALL % NI == 0 && TR_ELL_K % NI == 0,
+                  "table load");
 };
 
 template <int LOG>
@@ -177,6 +179,97 @@ __device__ __forceinline__ u64 tr_step(TrWave<LOG> &s, u32 &nused, const u32 (&w
     return c;
 }
 
+// ELL row of id y (relative to lo); a lane without a probe (ok false, y 0)
+// loads row 0 and drops it, so the load needs no select of addresses.
+__device__ __forceinline__ void tr_ell_load(const uint4 *ell, u32 y, bool ok, uint4 (&e)[2]) {
+    e[0] = ell[2 * (u64)y];
+    e[1] = ell[2 * (u64)y + 1];
+    if (!ok) {
+        e[0] = make_uint4(TR_EMPTY, TR_EMPTY, TR_EMPTY, TR_EMPTY);
+        e[1] = e[0];
+    }
+}
+
+// The lane-per-probe join over an ELL adjacency (every degree <= TR_ELL_K):
+// a probe's matches are one 32-byte row of ids relative to lo (TR_EMPTY past
+// its degree).  The rows are loaded one chunk ahead and the chunks' ends two
+// chunks ahead, so the global loads of a chunk overlap the previous chunk's
+// inserts.  Returns the ends flushed to dst[0 ..); rows += the join rows.
+template <int LOG>
+__device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32 *dy, u64 beg, u32 len,
+                                           const uint4 *ell, u32 lo, u32 *F, u64 rowbase, u32 clo, u32 *dst,
+                                           u64 &rows) {
+    using C = TrCfg<LOG>;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const u32 shift = lo - clo;   // ELL ids are relative to lo, columns to the row's block
+    u64 cnt = 0;
+    u32 yn[2];
+    bool yv[2];
+    uint4 ea[2][2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const u32 i = h * WAVE + lane;
+        yv[h] = i < len;
+        yn[h] = yv[h] ? dy[beg + i] - lo : 0u;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) tr_ell_load(ell, yn[h], yv[h], ea[h]);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const u32 i = 2 * WAVE + h * WAVE + lane;
+        yv[h] = i < len;
+        yn[h] = yv[h] ? dy[beg + i] - lo : 0u;
+    }
+    for (u32 pc = 0; pc < len; pc += 2 * WAVE) {
+        uint4 eb[2][2];   // the next chunk's rows, in flight while this chunk inserts
+#pragma unroll
+        for (int h = 0; h < 2; h++) tr_ell_load(ell, yn[h], yv[h], eb[h]);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const u32 i = pc + 4 * WAVE + h * WAVE + lane;
+            yv[h] = i < len;
+            yn[h] = yv[h] ? dy[beg + i] - lo : 0u;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const u32 z[TR_ELL_K] = {ea[h][0].x, ea[h][0].y, ea[h][0].z, ea[h][0].w,
+                                     ea[h][1].x, ea[h][1].y, ea[h][1].z, ea[h][1].w};
+            u32 w[TR_ELL_K], mk[TR_ELL_K];
+#pragma unroll
+            for (int i = 0; i < TR_ELL_K; i++) {
+                const bool ok = z[i] != TR_EMPTY;
+                const u32 c = z[i] + shift;
+                w[i] = c >> 5;
+                mk[i] = ok ? 1u << (c & 31) : 0u;
+                rows += ok;
+            }
+            // a probe's matches are sorted: runs in one word merge into one entry
+#pragma unroll
+            for (int i = TR_ELL_K - 1; i > 0; i--)
+                if (mk[i] && mk[i - 1] && w[i] == w[i - 1]) {
+                    mk[i - 1] |= mk[i];
+                    mk[i] = 0u;
+                }
+#pragma unroll
+            for (int g0 = 0; g0 < TR_ELL_K; g0 += C::NI) {
+                u32 gw[C::NI], gm[C::NI];
+#pragma unroll
+                for (int i = 0; i < C::NI; i++) {
+                    gw[i] = w[g0 + i];
+                    gm[i] = mk[g0 + i];
+                }
+                cnt += tr_step(s, nused, gw, gm, F, rowbase, clo, dst + cnt);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            ea[h][0] = eb[h][0];
+            ea[h][1] = eb[h][1];
+        }
+    }
+    return cnt;
+}
+
 // One round over the active list (the sources with ends): entry i (source
 // ax[i], ends dy[abeg[i] ..][0, alen[i])) writes its new ends at
 // ny[aoff[i] ..] -- aoff is the exclusive scan of the entries' output
@@ -190,10 +283,10 @@ __device__ __forceinline__ u64 tr_step(TrWave<LOG> &s, u32 &nused, const u32 (&w
 // workgroup b takes the entries with c(i) in [b, b + 1) * c(nact) / grid (a
 // binary search; no global counter), and its four waves share them through
 // an LDS counter.
-template <int LOG>
+template <int LOG, bool ELL>
 __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
     const u32 *dy, const u32 *ax, const u64 *abeg, const u32 *alen, const u64 *aoff, u64 nact, u64 span,
-    const u64 *csr_off, const u32 *csr_dst, const u32 *deg, u32 dmax, u32 *F, const u64 *row_base,
+    const u64 *csr_off, const u32 *csr_dst, const u32 *deg, const uint4 *ell, u32 dmax, u32 *F, const u64 *row_base,
     const u32 *col_lo, u32 lo, u64 wcap, u32 *ny, u32 *rcnt, u64 *rbound, unsigned long long *counters) {
     using C = TrCfg<LOG>;
     __shared__ TrWave<LOG> sw[TR_W];
@@ -233,97 +326,101 @@ __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
                 const u64 beg = abeg[ai], out0 = aoff[ai];
                 const u64 rowbase = row_base[x - lo];   // x's row and its block's first id
                 const u32 clo = col_lo[x - lo];
-                // the chunks' ends are loaded one chunk ahead (software pipelining)
-                u32 yn[2];
+                if constexpr (ELL) {
+                    cnt += tr_ell_rows(s, nused, dy, beg, len, ell, lo, F, rowbase, clo, ny + out0, wave_rows);
+                } else {
+                    // the chunks' ends are loaded one chunk ahead (software pipelining)
+                    u32 yn[2];
 #pragma unroll
-                for (int h = 0; h < 2; h++) yn[h] = (u32)(h * WAVE + lane) < len ? dy[beg + h * WAVE + lane] : 0u;
-                for (u32 pc = 0; pc < len; pc += 2 * WAVE) {
-                    // two probes per lane and their adjacency ranges
-                    u64 st[2] = {0, 0};
-                    u32 d[2] = {0, 0};
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        if (pc + h * WAVE + lane < len) {
-                            u32 y = yn[h] - lo;
-                            st[h] = csr_off[y];
-                            d[h] = deg[y];
-                        }
-                    }
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        u32 i = pc + 2 * WAVE + h * WAVE + lane;
-                        yn[h] = i < len ? dy[beg + i] : 0u;
-                    }
-                    wave_rows += d[0] + d[1];
-                    u32 maxd = max(d[0], d[1]);
-#pragma unroll
-                    for (int o = WAVE / 2; o > 0; o >>= 1) maxd = max(maxd, (u32)__shfl_xor((int)maxd, o, WAVE));
-                    if (maxd <= (u32)TR_SMALL) {
-                        // lane per probe: the matches of both probes gathered at once, then inserted
-                        u32 zs[2][TR_SMALL];
-#pragma unroll
-                        for (int h = 0; h < 2; h++)
-#pragma unroll
-                            for (int i = 0; i < TR_SMALL; i++) zs[h][i] = (u32)i < d[h] ? csr_dst[st[h] + i] : 0u;
+                    for (int h = 0; h < 2; h++) yn[h] = (u32)(h * WAVE + lane) < len ? dy[beg + h * WAVE + lane] : 0u;
+                    for (u32 pc = 0; pc < len; pc += 2 * WAVE) {
+                        // two probes per lane and their adjacency ranges
+                        u64 st[2] = {0, 0};
+                        u32 d[2] = {0, 0};
 #pragma unroll
                         for (int h = 0; h < 2; h++) {
-#pragma unroll
-                            for (int g0 = 0; g0 < TR_SMALL; g0 += C::NI) {
-                                if ((u32)g0 < maxd) {   // uniform
-                                    u32 w[C::NI], mk[C::NI];
-#pragma unroll
-                                    for (int i = 0; i < C::NI; i++) {
-                                        const u32 z = zs[h][g0 + i] - clo;
-                                        w[i] = z >> 5;
-                                        mk[i] = (u32)(g0 + i) < d[h] ? 1u << (z & 31) : 0u;
-                                    }
-                                    // a probe's matches are sorted: runs in one word merge into one entry
-#pragma unroll
-                                    for (int i = C::NI - 1; i > 0; i--)
-                                        if (mk[i] && mk[i - 1] && w[i] == w[i - 1]) {
-                                            mk[i - 1] |= mk[i];
-                                            mk[i] = 0u;
-                                        }
-                                    cnt += tr_step(s, nused, w, mk, F, rowbase, clo, ny + out0 + cnt);
-                                }
+                            if (pc + h * WAVE + lane < len) {
+                                u32 y = yn[h] - lo;
+                                st[h] = csr_off[y];
+                                d[h] = deg[y];
                             }
                         }
-                        continue;
-                    }
-                    // load-balanced rows, one half-chunk of 64 probes at a time
-#pragma unroll 1
-                    for (int h = 0; h < 2; h++) {
-                        u64 R;
-                        u64 ex = wave_excl_scan(d[h], &R);
-                        wave_sync();
-                        s.p_off[lane] = ex;
-                        s.p_start[lane] = st[h];
-                        if (lane == 0) s.p_off[WAVE] = R;
-                        wave_sync();
-                        for (u64 rb = 0; rb < R; rb += TR_BATCH) {
-                            const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
-                            u64 ci[TR_U];
 #pragma unroll
-                            for (int q = 0; q < TR_U; q++) {   // locate TR_U rows per lane ...
-                                u64 r = rb + (u64)q * WAVE + lane;
-                                ci[q] = ~0ull;
-                                if (r < re) {
-                                    u32 a = 0, b = WAVE;   // last probe with p_off <= r
-                                    while (b - a > 1) {
-                                        u32 m = (a + b) >> 1;
-                                        if (s.p_off[m] <= r) a = m; else b = m;
+                        for (int h = 0; h < 2; h++) {
+                            u32 i = pc + 2 * WAVE + h * WAVE + lane;
+                            yn[h] = i < len ? dy[beg + i] : 0u;
+                        }
+                        wave_rows += d[0] + d[1];
+                        u32 maxd = max(d[0], d[1]);
+#pragma unroll
+                        for (int o = WAVE / 2; o > 0; o >>= 1) maxd = max(maxd, (u32)__shfl_xor((int)maxd, o, WAVE));
+                        if (maxd <= (u32)TR_SMALL) {
+                            // lane per probe: the matches of both probes gathered at once, then inserted
+                            u32 zs[2][TR_SMALL];
+#pragma unroll
+                            for (int h = 0; h < 2; h++)
+#pragma unroll
+                                for (int i = 0; i < TR_SMALL; i++) zs[h][i] = (u32)i < d[h] ? csr_dst[st[h] + i] : 0u;
+#pragma unroll
+                            for (int h = 0; h < 2; h++) {
+#pragma unroll
+                                for (int g0 = 0; g0 < TR_SMALL; g0 += C::NI) {
+                                    if ((u32)g0 < maxd) {   // uniform
+                                        u32 w[C::NI], mk[C::NI];
+#pragma unroll
+                                        for (int i = 0; i < C::NI; i++) {
+                                            const u32 z = zs[h][g0 + i] - clo;
+                                            w[i] = z >> 5;
+                                            mk[i] = (u32)(g0 + i) < d[h] ? 1u << (z & 31) : 0u;
+                                        }
+                                        // a probe's matches are sorted: runs in one word merge into one entry
+#pragma unroll
+                                        for (int i = C::NI - 1; i > 0; i--)
+                                            if (mk[i] && mk[i - 1] && w[i] == w[i - 1]) {
+                                                mk[i - 1] |= mk[i];
+                                                mk[i] = 0u;
+                                            }
+                                        cnt += tr_step(s, nused, w, mk, F, rowbase, clo, ny + out0 + cnt);
                                     }
-                                    ci[q] = s.p_start[a] + (r - s.p_off[a]);
                                 }
                             }
-                            u32 w[TR_U], mk[TR_U];
+                            continue;
+                        }
+                        // load-balanced rows, one half-chunk of 64 probes at a time
+#pragma unroll 1
+                        for (int h = 0; h < 2; h++) {
+                            u64 R;
+                            u64 ex = wave_excl_scan(d[h], &R);
+                            wave_sync();
+                            s.p_off[lane] = ex;
+                            s.p_start[lane] = st[h];
+                            if (lane == 0) s.p_off[WAVE] = R;
+                            wave_sync();
+                            for (u64 rb = 0; rb < R; rb += TR_BATCH) {
+                                const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
+                                u64 ci[TR_U];
 #pragma unroll
-                            for (int q = 0; q < TR_U; q++) {   // ... gather ...
-                                const u32 z = (ci[q] != ~0ull ? csr_dst[ci[q]] : clo) - clo;
-                                w[q] = z >> 5;
-                                mk[q] = ci[q] != ~0ull ? 1u << (z & 31) : 0u;
+                                for (int q = 0; q < TR_U; q++) {   // locate TR_U rows per lane ...
+                                    u64 r = rb + (u64)q * WAVE + lane;
+                                    ci[q] = ~0ull;
+                                    if (r < re) {
+                                        u32 a = 0, b = WAVE;   // last probe with p_off <= r
+                                        while (b - a > 1) {
+                                            u32 m = (a + b) >> 1;
+                                            if (s.p_off[m] <= r) a = m; else b = m;
+                                        }
+                                        ci[q] = s.p_start[a] + (r - s.p_off[a]);
+                                    }
+                                }
+                                u32 w[TR_U], mk[TR_U];
+#pragma unroll
+                                for (int q = 0; q < TR_U; q++) {   // ... gather ...
+                                    const u32 z = (ci[q] != ~0ull ? csr_dst[ci[q]] : clo) - clo;
+                                    w[q] = z >> 5;
+                                    mk[q] = ci[q] != ~0ull ? 1u << (z & 31) : 0u;
+                                }
+                                cnt += tr_step(s, nused, w, mk, F, rowbase, clo, ny + out0 + cnt);   // ... insert
                             }
-                            cnt += tr_step(s, nused, w, mk, F, rowbase, clo, ny + out0 + cnt);   // ... insert
                         }
                     }
                 }
@@ -456,6 +553,14 @@ __global__ void csr_hist_k(const u64 *rows, u64 n, u32 lo, u32 *cnt) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         atomicAdd(cnt + (tkey(rows[i]) - lo), 1u);
 }
+// ELL rows: the matches of id s at ell[s * TR_ELL_K ..], ids relative to lo.
+__global__ void ell_fill_k(const u64 *rows, u64 n, const u64 *csr_off, u32 lo, u32 *ell) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32 s = tkey(rows[i]) - lo;
+        ell[(u64)s * TR_ELL_K + (i - csr_off[s])] = tval(rows[i]) - lo;
+    }
+}
 __global__ void csr_dst_k(const u64 *rows, u64 n, u32 *dst) {
     u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = tval(rows[i]);
@@ -511,6 +616,14 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
     if (n)
         DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, csr_dst_k, sgrid(ctx, n), ctx->block_size, 0,
                    (const u64 *)ix.rows.p, n, tr.csr_dst.p);
+    // every degree <= TR_ELL_K: the join reads ELL rows instead (one load level less)
+    tr.ell.release();
+    if (n && tr.dmax && tr.dmax <= (u32)TR_ELL_K) {
+        tr.ell.ensure(W * TR_ELL_K);
+        fill_u32(ctx, tr.ell.p, W * TR_ELL_K, TR_EMPTY);
+        DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * n, ell_fill_k, sgrid(ctx, n), ctx->block_size, 0, (const u64 *)ix.rows.p,
+                   n, (const u64 *)tr.csr_off.p, F.lo, tr.ell.p);
+    }
     // Δ0 = unique(E reversed), grouped by source; one segment per source, for good
     DBuf<u64> uniq(ctx, n ? n : 1);
     u64 nu = n ? select(ctx, SEL_UNIQUE, ix.rows.p, n, nullptr, 0, uniq.p) : 0;
@@ -542,12 +655,13 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
 
 constexpr u64 TR_LONG_SEGMENTS = 64;   // average probes per segment above which the big table wins
 
-template <int LOG>
+template <int LOG, bool ELL>
 static void launch_rows(dlg_ctx *ctx, RowMap &F, TcRows &tr) {
     unsigned g = (unsigned)std::min<u64>((tr.nact + TR_W - 1) / TR_W, (u64)ctx->num_cus * TrCfg<LOG>::BLOCKS_PER_CU);
-    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, tc_rows_k<LOG>, g, TR_T, 0, (const u32 *)tr.dy.p, (const u32 *)tr.ax.p,
-               (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, (const u64 *)tr.aoff.p, tr.nact, tr.rows,
-               (const u64 *)tr.csr_off.p, (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p, tr.dmax, F.words.p,
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, (tc_rows_k<LOG, ELL>), g, TR_T, 0, (const u32 *)tr.dy.p,
+               (const u32 *)tr.ax.p, (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, (const u64 *)tr.aoff.p, tr.nact,
+               tr.rows, (const u64 *)tr.csr_off.p, (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p,
+               (const uint4 *)tr.ell.p, tr.dmax, F.words.p,
                (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo, tr.wcap, tr.ny.p, tr.rcnt.p, tr.rbound.p,
                (unsigned long long *)tr.counters.p);
 }
@@ -565,8 +679,14 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     // algorithmic bytes, added once the round's join rows are known: 4 B per
     // join row (its matched end) + 20 B per probe (its end, CSR offset, degree)
     const u64 nd0 = tr.nd;
-    if (tr.nd >= TR_LONG_SEGMENTS * na) launch_rows<10>(ctx, F, tr);   // long segments: big tables
-    else launch_rows<9>(ctx, F, tr);                                   // short ones: more waves
+    const bool big = tr.nd >= TR_LONG_SEGMENTS * na;   // long segments: big tables; short ones: more waves
+    if (tr.ell.p) {
+        if (big) launch_rows<10, true>(ctx, F, tr);
+        else launch_rows<9, true>(ctx, F, tr);
+    } else {
+        if (big) launch_rows<10, false>(ctx, F, tr);
+        else launch_rows<9, false>(ctx, F, tr);
+    }
     // the next active list; its ends sit at this round's output offsets
     const u64 nb = (na + TA_CHUNK - 1) / TA_CHUNK;
     tr.part.ensure(2 * nb);

@@ -147,6 +147,33 @@ def test_config_grid_wcc(ctx, orc):
     assert_same(ctx.run("wcc", rows), orc.wcc(rows), secondary=True)
 
 
+def capped_degree_rows(seed, n, dmax, dup):
+    """A chain 0 -> 1 -> ... -> n-1 (n - 1 rounds) with a few chords, and one
+    hub of out-degree exactly dmax (repeated targets when dup: a duplicate
+    edge counts towards the degree and is joined twice)."""
+    rng = np.random.default_rng(seed)
+    e = [(u, u + 1) for u in range(n - 1)]
+    hub = n // 2
+    tg = rng.integers(0, n, size=dmax - 1)
+    if dup:
+        tg[1] = tg[0]
+    e += [(hub, int(t)) for t in tg]
+    for u in rng.choice(np.arange(n - 1), size=n // 10, replace=False):
+        if u != hub:
+            e.append((int(u), int(rng.integers(0, n))))
+    return np.array(e, dtype=np.int32)
+
+
+@pytest.mark.parametrize("dmax,dup", [(4, False), (8, True), (9, False), (16, True), (17, False), (40, True)])
+def test_tc_rows_join_paths(ctx, orc, dmax, dup):
+    # max degree <= 8: ELL rows; 9..16: CSR with new-end bounds; above: CSR with exact bounds
+    rows = capped_degree_rows(dmax, 1500, dmax, dup)
+    assert np.unique(rows[:, 0], return_counts=True)[1].max() == dmax
+    want = orc.tc(rows)
+    for strategy in (0, 4):
+        assert_same(ctx.run("tc", rows, strategy=strategy), want)
+
+
 # ------------------------------------------------------------------ published counts, full size
 @pytest.mark.parametrize("name,its,size", [
     ("data_223001.bin", 287, 80_498_014),   # logs/small-merged-job-output.txt:108 (SF.cedge)
