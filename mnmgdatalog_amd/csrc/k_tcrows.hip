@@ -179,22 +179,14 @@ __device__ __forceinline__ u64 tr_step(TrWave<LOG> &s, u32 &nused, const u32 (&w
     return c;
 }
 
-// ELL row of id y (relative to lo); a lane without a probe (ok false, y 0)
-// loads row 0 and drops it, so the load needs no select of addresses.
-__device__ __forceinline__ void tr_ell_load(const uint4 *ell, u32 y, bool ok, uint4 (&e)[2]) {
-    e[0] = ell[2 * (u64)y];
-    e[1] = ell[2 * (u64)y + 1];
-    if (!ok) {
-        e[0] = make_uint4(TR_EMPTY, TR_EMPTY, TR_EMPTY, TR_EMPTY);
-        e[1] = e[0];
-    }
-}
-
 // The lane-per-probe join over an ELL adjacency (every degree <= TR_ELL_K):
 // a probe's matches are one 32-byte row of ids relative to lo (TR_EMPTY past
 // its degree).  The rows are loaded one chunk ahead and the chunks' ends two
 // chunks ahead, so the global loads of a chunk overlap the previous chunk's
-// inserts.  Returns the ends flushed to dst[0 ..); rows += the join rows.
+// inserts.  Loads are never predicated (a lane past the end reads the last
+// end again) and their results are not touched before the next chunk, so no
+// wait for them is placed in this chunk; validity travels as flags.
+// Returns the ends flushed to dst[0 ..); rows += the join rows.
 template <int LOG>
 __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32 *dy, u64 beg, u32 len,
                                            const uint4 *ell, u32 lo, u32 *F, u64 rowbase, u32 clo, u32 *dst,
@@ -202,33 +194,42 @@ __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32
     using C = TrCfg<LOG>;
     const int lane = threadIdx.x & (WAVE - 1);
     const u32 shift = lo - clo;   // ELL ids are relative to lo, columns to the row's block
+    const u64 last = beg + len - 1;
     u64 cnt = 0;
-    u32 yn[2];
-    bool yv[2];
-    uint4 ea[2][2];
+    u32 yr[2];        // ends of the next chunk (raw ids)
+    bool yv[2], ev[2];
+    uint4 ea[2][2];   // rows of this chunk
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        const u32 i = h * WAVE + lane;
-        yv[h] = i < len;
-        yn[h] = yv[h] ? dy[beg + i] - lo : 0u;
+        const u64 i = beg + h * WAVE + lane;
+        ev[h] = i <= last;
+        yr[h] = dy[i <= last ? i : last];
     }
 #pragma unroll
-    for (int h = 0; h < 2; h++) tr_ell_load(ell, yn[h], yv[h], ea[h]);
+    for (int h = 0; h < 2; h++) {
+        ea[h][0] = ell[2 * (u64)(yr[h] - lo)];
+        ea[h][1] = ell[2 * (u64)(yr[h] - lo) + 1];
+    }
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        const u32 i = 2 * WAVE + h * WAVE + lane;
-        yv[h] = i < len;
-        yn[h] = yv[h] ? dy[beg + i] - lo : 0u;
+        const u64 i = beg + 2 * WAVE + h * WAVE + lane;
+        yv[h] = i <= last;
+        yr[h] = dy[i <= last ? i : last];
     }
     for (u32 pc = 0; pc < len; pc += 2 * WAVE) {
         uint4 eb[2][2];   // the next chunk's rows, in flight while this chunk inserts
-#pragma unroll
-        for (int h = 0; h < 2; h++) tr_ell_load(ell, yn[h], yv[h], eb[h]);
+        bool evn[2];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const u32 i = pc + 4 * WAVE + h * WAVE + lane;
-            yv[h] = i < len;
-            yn[h] = yv[h] ? dy[beg + i] - lo : 0u;
+            eb[h][0] = ell[2 * (u64)(yr[h] - lo)];
+            eb[h][1] = ell[2 * (u64)(yr[h] - lo) + 1];
+            evn[h] = yv[h];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const u64 i = beg + pc + 4 * WAVE + h * WAVE + lane;
+            yv[h] = i <= last;
+            yr[h] = dy[i <= last ? i : last];
         }
 #pragma unroll
         for (int h = 0; h < 2; h++) {
@@ -237,7 +238,7 @@ __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32
             u32 w[TR_ELL_K], mk[TR_ELL_K];
 #pragma unroll
             for (int i = 0; i < TR_ELL_K; i++) {
-                const bool ok = z[i] != TR_EMPTY;
+                const bool ok = ev[h] && z[i] != TR_EMPTY;
                 const u32 c = z[i] + shift;
                 w[i] = c >> 5;
                 mk[i] = ok ? 1u << (c & 31) : 0u;
@@ -265,6 +266,7 @@ __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32
         for (int h = 0; h < 2; h++) {
             ea[h][0] = eb[h][0];
             ea[h][1] = eb[h][1];
+            ev[h] = evn[h];
         }
     }
     return cnt;
@@ -329,10 +331,12 @@ __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
                 if constexpr (ELL) {
                     cnt += tr_ell_rows(s, nused, dy, beg, len, ell, lo, F, rowbase, clo, ny + out0, wave_rows);
                 } else {
-                    // the chunks' ends are loaded one chunk ahead (software pipelining)
+                    // the chunks' ends are loaded one chunk ahead (software pipelining;
+                    // unpredicated: a lane past the end reads the last end again)
+                    const u64 last = beg + len - 1;
                     u32 yn[2];
 #pragma unroll
-                    for (int h = 0; h < 2; h++) yn[h] = (u32)(h * WAVE + lane) < len ? dy[beg + h * WAVE + lane] : 0u;
+                    for (int h = 0; h < 2; h++) yn[h] = dy[min(beg + h * WAVE + lane, last)];
                     for (u32 pc = 0; pc < len; pc += 2 * WAVE) {
                         // two probes per lane and their adjacency ranges
                         u64 st[2] = {0, 0};
@@ -346,10 +350,7 @@ __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
                             }
                         }
 #pragma unroll
-                        for (int h = 0; h < 2; h++) {
-                            u32 i = pc + 2 * WAVE + h * WAVE + lane;
-                            yn[h] = i < len ? dy[beg + i] : 0u;
-                        }
+                        for (int h = 0; h < 2; h++) yn[h] = dy[min(beg + pc + 2 * WAVE + h * WAVE + lane, last)];
                         wave_rows += d[0] + d[1];
                         u32 maxd = max(d[0], d[1]);
 #pragma unroll
