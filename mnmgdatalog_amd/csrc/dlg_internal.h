@@ -396,6 +396,18 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw);
 // The current Δ as (z, x) tuples.
 void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out);
 
+// ------------------------------------------------------------- SG row ORs (k_sgrows.hip)
+struct SgRows {
+    DBuf<u64> poff, counters;   // predecessor list offsets (W + 1); raw rows, R words, F words
+    DBuf<u32> pred, rflag;      // predecessors (relative to lo); nonempty rows of R
+    explicit SgRows(dlg_ctx *c) : poff(c), counters(c), pred(c), rflag(c) {}
+};
+// Predecessor lists of the index's edges over F's id range.
+void sg_rows_setup(dlg_ctx *ctx, const Index &ix, const Bitmap &F, SgRows &sr);
+// N = (T ⋈ E) \ F with T in the row-major map R: new bits into F and D (both
+// transposed); R is left empty.  Returns the join's raw rows.
+u64 sg_rows_step(dlg_ctx *ctx, SgRows &sr, Bitmap &R, Bitmap &F, Bitmap &D);
+
 // ------------------------------------------------------------- hash set (k_hashset.hip)
 struct HashSet {
     DBuf<u64> slots;   // open addressing, EMPTY_SLOT = free

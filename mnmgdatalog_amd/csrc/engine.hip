@@ -363,18 +363,18 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     JoinScratch js(ctx);
     int it = 0;
     if (use_bitmap(o, ne, ib)) {
-        // F, the round's T and the round's N as bitmaps: join rows go straight
-        // into them (k_bitmap.hip), and T and N come out of their maps in bit
-        // order, which keeps each join's writes local:
-        //   Δ (a, b) in b order   -> T rows (b, x) of one b share R's row b;
-        //   T (b, x) in (b, x) order -> N rows (x, y) of one b share F's rows y
-        //   (F and D transposed), at neighbouring columns x.
-        // SG is symmetric (both rules are), so F equals its transpose and is
-        // emitted without a sort.
+        // F, the round's T and the round's N as bitmaps.  The first join's
+        // rows go straight into T's map R (k_bitmap.hip; Δ (a, b) in b order
+        // puts the T rows (b, x) of one b into R's row b); the second join is
+        // row ORs of R into F and D (k_sgrows.hip); N comes out of D in bit
+        // order.  SG is symmetric (both rules are), so F equals its transpose
+        // and is emitted without a sort.
         Bitmap F(ctx), R(ctx), D(ctx);
         bitmap_init(ctx, F, ib[0], ib[1], true);
         bitmap_init(ctx, R, ib[0], ib[1], false);
         bitmap_init(ctx, D, ib[0], ib[1], true);
+        SgRows sr(ctx);
+        sg_rows_setup(ctx, ix, F, sr);
         u64 raw = 0;
         pc.mark(DLG_PH_JOIN);
         bitmap_join_mark(ctx, ix, edge.p, ne, js, DLG_PROJ_HV_PV, true, F, &D, &raw);   // base (y, x)
@@ -383,9 +383,8 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         u64 total = nd;
         while (true) {
             pc.mark(DLG_PH_JOIN);
-            bitmap_join_mark(ctx, ix, delta.p, nd, js, DLG_PROJ_PV_HV, false, R, nullptr, &raw);  // (b, x)
-            u64 nt = bitmap_drain(ctx, R, T, false);
-            bitmap_join_mark(ctx, ix, T.p, nt, js, DLG_PROJ_PV_HV, false, F, &D, &raw);     // (x, y)
+            bitmap_join_mark(ctx, ix, delta.p, nd, js, DLG_PROJ_PV_HV, false, R, nullptr, &raw);  // T (b, x)
+            raw += sg_rows_step(ctx, sr, R, F, D);                                               // N (x, y)
             u64 nn = bitmap_drain(ctx, D, N, true);
             total += nn;
             it++;

@@ -1,0 +1,173 @@
+// k_sgrows.hip -- SG's second join as whole-row ORs over the bitmaps.
+//
+// A round of SG (sg.cu:300-375) derives
+//     T(b, x) = Δ(a, b) ⋈ edge(a, x)            (the round's T map R, row b)
+//     N(x, y) = T(b, x) ⋈ edge(b, y) \ F        (new tuples; F and D transposed: row y)
+// The second join needs no per-row work: with R row-major (row b = the x of
+// T(b, x)) and F, D transposed (row y = the x of N(x, y)),
+//     N row y = (OR over b with edge(b, y) of R row b) & ~(F row y)
+// so one workgroup owns row y: it ORs its predecessors' R rows word by word
+// (coalesced, no hashing, no atomics), and writes the new bits into F and D
+// with plain stores.  The join's raw rows, Σ over T(b, x) of outdeg(b), are
+// the popcounts of the R words it reads (each R row b is read once per y
+// with edge(b, y)).  R rows without bits are skipped (a flag per row, set by
+// a scan after the first join) and the flagged rows are cleared afterwards,
+// so R is empty again for the next round.
+#include "dlg_internal.h"
+
+#include <algorithm>
+
+namespace dlg {
+
+constexpr int SR_T = 256;   // threads per workgroup of the row OR
+
+// predecessor lists: pred[poff[y] ..] = the b of every edge(b, y), relative to lo
+__global__ void sg_pred_hist_k(const u64 *rows, u64 n, u32 lo, u32 *cnt) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        atomicAdd(cnt + (tval(rows[i]) - lo), 1u);
+}
+__global__ void sg_pred_fill_k(const u64 *rows, u64 n, u32 lo, const u64 *poff, u32 *cur, u32 *pred) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32 y = tval(rows[i]) - lo;
+        pred[poff[y] + atomicAdd(cur + y, 1u)] = tkey(rows[i]) - lo;
+    }
+}
+
+// flag[r] = row r of a row-major map has a bit (one wave per row)
+__global__ void bm_row_flags_k(const u32 *words, u64 W, u64 wpr, u32 *flag) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const u64 nw = (u64)gridDim.x * blockDim.x / WAVE;
+    for (u64 r = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; r < W; r += nw) {
+        u32 acc = 0;
+        for (u64 j = lane; j < wpr; j += WAVE) acc |= words[r * wpr + j];
+        const bool any = __ballot(acc != 0) != 0;
+        if (lane == 0) flag[r] = any ? 1u : 0u;
+    }
+}
+
+// clear the flagged rows and their flags (one wave per row)
+__global__ void bm_clear_rows_k(u32 *words, u64 W, u64 wpr, u32 *flag) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const u64 nw = (u64)gridDim.x * blockDim.x / WAVE;
+    for (u64 r = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; r < W; r += nw) {
+        if (!flag[r]) continue;   // uniform per wave
+        for (u64 j = lane; j < wpr; j += WAVE) words[r * wpr + j] = 0u;
+        if (lane == 0) flag[r] = 0u;
+    }
+}
+
+// N row y = OR of the flagged R rows of y's predecessors, minus F row y;
+// new bits go into F and D.  counters[0] += the raw join rows (popcounts of
+// the R words read), counters[1] += R words read, counters[2] += F words read.
+__global__ __launch_bounds__(SR_T) void sg_rowor_k(const u32 *R, const u32 *rflag, const u64 *poff,
+                                                   const u32 *pred, u64 W, u64 wpr, u32 *F, u32 *D,
+                                                   unsigned long long *counters) {
+    __shared__ u32 s_act[SR_T];
+    __shared__ u32 s_n;
+    __shared__ u64 s_red[3][SR_T / WAVE];
+    u64 raw = 0, rwords = 0, fwords = 0;
+    for (u64 y = blockIdx.x; y < W; y += gridDim.x) {
+        const u64 p0 = poff[y], p1 = poff[y + 1];
+        for (u64 pb = p0; pb < p1; pb += SR_T) {   // predecessors in batches of SR_T
+            if (threadIdx.x == 0) s_n = 0;
+            __syncthreads();
+            const u64 k = pb + threadIdx.x;
+            if (k < p1) {
+                const u32 b = pred[k];
+                if (rflag[b]) s_act[atomicAdd(&s_n, 1u)] = b;
+            }
+            __syncthreads();
+            const u32 na = s_n;
+            if (na) {   // uniform
+                u32 *fy = F + y * wpr, *dw = D + y * wpr;
+                if (threadIdx.x < wpr) rwords += (u64)na * ((wpr - threadIdx.x + SR_T - 1) / SR_T);
+                for (u64 j = threadIdx.x; j < wpr; j += SR_T) {
+                    u32 acc = 0;
+                    u32 i = 0;
+                    for (; i + 4 <= na; i += 4) {   // four rows' loads in flight
+                        const u32 v0 = R[(u64)s_act[i] * wpr + j], v1 = R[(u64)s_act[i + 1] * wpr + j];
+                        const u32 v2 = R[(u64)s_act[i + 2] * wpr + j], v3 = R[(u64)s_act[i + 3] * wpr + j];
+                        acc |= v0 | v1 | v2 | v3;
+                        raw += (u64)(__popc(v0) + __popc(v1) + __popc(v2) + __popc(v3));
+                    }
+                    for (; i < na; i++) {
+                        const u32 v = R[(u64)s_act[i] * wpr + j];
+                        acc |= v;
+                        raw += (u64)__popc(v);
+                    }
+                    if (acc) {
+                        fwords++;
+                        const u32 f = fy[j], nb = acc & ~f;   // the row's owner: plain read-modify-write
+                        if (nb) {
+                            fy[j] = f | nb;
+                            dw[j] |= nb;
+                        }
+                    }
+                }
+            }
+            __syncthreads();   // s_act and s_n are refilled by the next batch
+        }
+    }
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        raw += __shfl_xor(raw, o, WAVE);
+        rwords += __shfl_xor(rwords, o, WAVE);
+        fwords += __shfl_xor(fwords, o, WAVE);
+    }
+    if (lane == 0) {
+        s_red[0][wv] = raw;
+        s_red[1][wv] = rwords;
+        s_red[2][wv] = fwords;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        u64 t = 0;
+        for (int w = 0; w < SR_T / WAVE; w++) t += s_red[threadIdx.x][w];
+        if (t) atomicAdd(counters + threadIdx.x, (unsigned long long)t);
+    }
+}
+
+void sg_rows_setup(dlg_ctx *ctx, const Index &ix, const Bitmap &F, SgRows &sr) {
+    const u64 W = F.W, n = ix.n;
+    DBuf<u32> cnt(ctx, W + 1);
+    fill_u32(ctx, cnt.p, W + 1, 0u);
+    if (n)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, sg_pred_hist_k, sgrid(ctx, n), ctx->block_size, 0,
+                   (const u64 *)ix.rows.p, n, F.lo, cnt.p);
+    sr.poff.ensure(W + 1);
+    scan_u32_u64(ctx, cnt.p, W + 1, sr.poff.p);
+    sr.pred.ensure(n ? n : 1);
+    fill_u32(ctx, cnt.p, W + 1, 0u);
+    if (n)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * n, sg_pred_fill_k, sgrid(ctx, n), ctx->block_size, 0,
+                   (const u64 *)ix.rows.p, n, F.lo, (const u64 *)sr.poff.p, cnt.p, sr.pred.p);
+    sr.rflag.ensure(W);
+    fill_u32(ctx, sr.rflag.p, W, 0u);
+    sr.counters.ensure(3);
+    ctx->sync();   // cnt goes out of scope
+}
+
+u64 sg_rows_step(dlg_ctx *ctx, SgRows &sr, Bitmap &R, Bitmap &F, Bitmap &D) {
+    if (R.tr || !F.tr || !D.tr || R.W != F.W || R.W != D.W || R.lo != F.lo || R.lo != D.lo)
+        fail(DLG_INVALID_INPUT, "sg_rows_step: R row-major, F and D transposed, one geometry");
+    const u64 W = R.W, wpr = R.wpr;
+    const unsigned gw = (unsigned)std::min<u64>((W + 3) / 4, (u64)ctx->num_cus * 16);   // a wave per row
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * R.nwords, bm_row_flags_k, gw, 256, 0, (const u32 *)R.words.p, W, wpr,
+               sr.rflag.p);
+    fill_u64(ctx, sr.counters.p, 3, 0ull);
+    const unsigned g = (unsigned)std::min<u64>(W, (u64)ctx->num_cus * 8);
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, sg_rowor_k, g, SR_T, 0, (const u32 *)R.words.p,
+               (const u32 *)sr.rflag.p, (const u64 *)sr.poff.p, (const u32 *)sr.pred.p, W, wpr, F.words.p,
+               D.words.p, (unsigned long long *)sr.counters.p);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * W, bm_clear_rows_k, gw, 256, 0, R.words.p, W, wpr, sr.rflag.p);
+    u64 h[3];
+    ctx->read_scalars(sr.counters.p, 3, h);
+    // algorithmic bytes: every R word read, every touched F word read (and written when new)
+    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[1] + 8.0 * h[2]);
+    return h[0];
+}
+
+}  // namespace dlg
