@@ -547,27 +547,49 @@ u64 bitmap_drain(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, bool swap) {
     return h;
 }
 
-// Bit-matrix transpose of a W x S map, 32 x 32-bit blocks: 32 lanes load the
-// 32 rows of a block (one word each), five exchange steps swap the
-// off-diagonal halves of 16-, 8-, 4-, 2- and 1-bit sub-blocks, and lane i
-// stores the block's column i as a row of the destination.
-__global__ void bm_transpose_k(const u32 *src, u32 *dst, u64 W, u64 wpr) {
-    const int lane = threadIdx.x & 31;
-    u64 blk = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 5;   // one block per half wave
-    u64 nblk = wpr * wpr;
-    if (blk >= nblk) return;   // whole half waves leave together
-    u64 R = blk / wpr, C = blk - R * wpr;   // source block: rows 32R.., word column C
-    u64 r = R * 32 + lane;
-    u32 x = r < W ? src[r * wpr + C] : 0u;
-    const u32 m[5] = {0x0000ffffu, 0x00ff00ffu, 0x0f0f0f0fu, 0x33333333u, 0x55555555u};
-#pragma unroll
-    for (int s = 0; s < 5; s++) {
-        const int j = 16 >> s;
-        u32 y = __shfl_xor(x, j, 32);
-        x = (lane & j) ? ((x & ~m[s]) | ((y & ~m[s]) >> j)) : ((x & m[s]) | ((y & m[s]) << j));
+// Bit-matrix transpose of a W x S map in 32 x 32-bit blocks.  A workgroup
+// takes a tile of BT x BT blocks (32 BT source rows x BT words): the tile
+// comes in with coalesced loads (BT consecutive words per row) into LDS,
+// each half wave transposes blocks there -- 32 lanes hold a block's 32 rows,
+// five exchange steps swap the off-diagonal halves of 16-, 8-, 4-, 2- and
+// 1-bit sub-blocks, and lane i holds the block's column i as a destination
+// word -- into a second LDS tile, which goes out with coalesced stores (BT
+// consecutive words per destination row).  Rows are padded to BT + 1 words,
+// so a block's 32 rows sit in 32 different banks.
+constexpr int BT = 16;
+__global__ __launch_bounds__(256) void bm_transpose_k(const u32 *src, u32 *dst, u64 W, u64 wpr) {
+    __shared__ u32 tin[32 * BT][BT + 1];
+    __shared__ u32 tout[32 * BT][BT + 1];
+    const u64 nt = (wpr + BT - 1) / BT;
+    const u64 R0 = (u64)(blockIdx.x / nt) * BT, C0 = (u64)(blockIdx.x % nt) * BT;   // tile: block row, word column
+    const int t = threadIdx.x;
+#pragma unroll 4
+    for (int k = 0; k < 32 * BT * BT / 256; k++) {
+        const int rr = t / BT + (256 / BT) * k, cc = t % BT;
+        const u64 r = R0 * 32 + rr, c = C0 + cc;
+        tin[rr][cc] = r < W && c < wpr ? src[r * wpr + c] : 0u;
     }
-    u64 c = C * 32 + lane;   // destination row = source column
-    if (c < W) dst[c * wpr + R] = x;
+    __syncthreads();
+    const int lane = t & 31;
+    const u32 m[5] = {0x0000ffffu, 0x00ff00ffu, 0x0f0f0f0fu, 0x33333333u, 0x55555555u};
+    for (int bk = t >> 5; bk < BT * BT; bk += 256 / 32) {
+        const int Rl = bk / BT, Cl = bk % BT;
+        u32 x = tin[32 * Rl + lane][Cl];
+#pragma unroll
+        for (int st = 0; st < 5; st++) {
+            const int j = 16 >> st;
+            const u32 y = __shfl_xor(x, j, 32);
+            x = (lane & j) ? ((x & ~m[st]) | ((y & ~m[st]) >> j)) : ((x & m[st]) | ((y & m[st]) << j));
+        }
+        tout[32 * Cl + lane][Rl] = x;   // destination row = source column
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < 32 * BT * BT / 256; k++) {
+        const int rr = t / BT + (256 / BT) * k, cc = t % BT;
+        const u64 r = C0 * 32 + rr, c = R0 + cc;
+        if (r < W && c < wpr) dst[r * wpr + c] = tout[rr][cc];
+    }
 }
 
 // A W-row block of wpr words per row, transposed (or copied) into a padded
@@ -580,9 +602,9 @@ u64 bitmap_emit_block(dlg_ctx *ctx, const u32 *words, u32 lo, u64 W, u64 wpr, bo
     scratch.ensure(padded);
     fill_u32(ctx, scratch.p + nwords, padded - nwords, 0u);
     if (transposed) {
-        u64 threads = wpr * wpr * 32;
-        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, bm_transpose_k, grid_for(threads, 256), 256, 0, words,
-                   scratch.p, W, wpr);
+        const u64 nt = (wpr + BT - 1) / BT;
+        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, bm_transpose_k, (unsigned)(nt * nt), 256, 0, words, scratch.p, W,
+                   wpr);
     } else {
         HIPCHK(hipMemcpyAsync(scratch.p, words, 4 * nwords, hipMemcpyDeviceToDevice, ctx->stream));
     }
