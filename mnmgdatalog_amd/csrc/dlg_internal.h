@@ -435,14 +435,20 @@ struct Index {
     DBuf<u64> rows;      // relation sorted by (key, value); duplicates kept
     DBuf<u32> gstart;    // start row of each key group, plus gstart[G] = n
     DBuf<u64> slots;     // open-addressed (key << 32 | group) slots, EMPTY_SLOT = free
-    u64 n = 0, groups = 0, nslots = 0;
+    DBuf<u64> dense;     // optional: first row of each key of [dense_lo, dense_lo + dense_w), plus n
+    u64 n = 0, groups = 0, nslots = 0, dense_w = 0;
+    u32 dense_lo = 0;
     bool sorted = false; // probes binary-search `rows` instead of the slots (no slots built)
-    explicit Index(dlg_ctx *c) : rows(c), gstart(c), slots(c) {}
+    explicit Index(dlg_ctx *c) : rows(c), gstart(c), slots(c), dense(c) {}
 };
 // Build from a device relation (copied); `presorted` skips the sort;
 // `sorted_lookup` builds no hash slots (probes binary-search the rows).
 void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double load_factor,
                  bool presorted, bool sorted_lookup = false);
+// Dense row offsets by key over [lo, lo + W) (index_build drops them); the
+// hash-index count phase then reads an offset pair per probe instead of
+// probing the slots.
+void index_dense(dlg_ctx *ctx, Index &ix, u32 lo, u64 W);
 // Count phase: per probe, the matching group's row start and count.  Returns |J|.
 struct JoinScratch {
     DBuf<u32> cnt, pstart;

@@ -375,6 +375,7 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         bitmap_init(ctx, D, ib[0], ib[1], true);
         SgRows sr(ctx);
         sg_rows_setup(ctx, ix, F, sr);
+        if (!ix.sorted) index_dense(ctx, ix, F.lo, F.W);   // the joins' count pass reads offsets
         u64 raw = 0;
         pc.mark(DLG_PH_JOIN);
         bitmap_join_mark(ctx, ix, edge.p, ne, js, DLG_PROJ_HV_PV, true, F, &D, &raw);   // base (y, x)

@@ -65,6 +65,7 @@ void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double lf, bool
     ix.groups = 0;
     ix.nslots = 0;
     ix.sorted = sorted_lookup;
+    ix.dense_w = 0;
     if (n == 0) return;
     if (n >= 0xffffffffull) fail(DLG_OVERFLOW, "index: relation larger than 2^32-2 rows");
     if (!(lf > 0.0)) lf = ctx->load_factor;
@@ -108,6 +109,43 @@ __global__ __launch_bounds__(1024) void join_count_k(const u64 *probe, u64 n, co
     }
 }
 
+// Dense-offset variant: the key's rows are rows[dense[k - lo] .. dense[k - lo + 1]).
+__global__ __launch_bounds__(1024) void join_count_dense_k(const u64 *probe, u64 n, const u64 *dense, u32 lo,
+                                                          u64 W, u32 *cnt, u32 *pstart) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const u32 k = tkey(probe[i]) - lo;
+        u32 c = 0, st = 0;
+        if (k < W) {
+            st = (u32)dense[k];
+            c = (u32)(dense[k + 1] - st);
+        }
+        cnt[i] = c;
+        pstart[i] = st;
+    }
+}
+__global__ void dense_hist_k(const u64 *rows, u64 n, u32 lo, u64 W, u32 *cnt) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32 k = tkey(rows[i]) - lo;
+        if (k < W) atomicAdd(cnt + k, 1u);
+    }
+}
+
+void index_dense(dlg_ctx *ctx, Index &ix, u32 lo, u64 W) {
+    ix.dense_w = 0;
+    if (ix.n == 0 || W == 0) return;
+    DBuf<u32> cnt(ctx, W + 1);
+    fill_u32(ctx, cnt.p, W + 1, 0u);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * ix.n, dense_hist_k, sgrid(ctx, ix.n), ctx->block_size, 0,
+               (const u64 *)ix.rows.p, ix.n, lo, W, cnt.p);
+    ix.dense.ensure(W + 1);
+    scan_u32_u64(ctx, cnt.p, W + 1, ix.dense.p);   // synchronises: cnt may go
+    ix.dense_lo = lo;
+    ix.dense_w = W;
+}
+
 // Sorted-index variant (common/join.cu:89-142): the probe key's run of rows
 // by two binary searches over the sorted relation.
 __global__ __launch_bounds__(1024) void join_count_sorted_k(const u64 *probe, u64 n, const u64 *rows, u64 nr,
@@ -133,6 +171,13 @@ u64 join_count(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScrat
         double lg = std::log2((double)ix.n + 1.0);
         DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 16.0 * lg * n + 8.0 * n, join_count_sorted_k, sgrid(ctx, n),
                    ctx->block_size, 0, probe, n, (const u64 *)ix.rows.p, ix.n, js.cnt.p, js.pstart.p);
+        return scan_u32_u64(ctx, js.cnt.p, n, js.offs.p);
+    }
+    if (ix.dense_w) {
+        // probe read + an offset pair per probe + count/start writes
+        DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 16.0 * n + 8.0 * n, join_count_dense_k, sgrid(ctx, n),
+                   ctx->block_size, 0, probe, n, (const u64 *)ix.dense.p, ix.dense_lo, ix.dense_w, js.cnt.p,
+                   js.pstart.p);
         return scan_u32_u64(ctx, js.cnt.p, n, js.offs.p);
     }
     // probe read + one slot line per probe + gstart pair + count/start writes
