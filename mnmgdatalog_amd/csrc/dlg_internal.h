@@ -324,8 +324,10 @@ u64 bitmap_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, J
 void bitmap_join_mark(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, int proj,
                       bool nodiag, Bitmap &bm, Bitmap *mark, u64 *raw);
 // All set bits in bit order (as (column, row) tuples when `swap`), then the
-// map cleared.  Returns their number.
-u64 bitmap_drain(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, bool swap);
+// map cleared (unless `clear` is false).  Returns their number.
+u64 bitmap_drain(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, bool swap, bool clear = true);
+// The number of set bits (synchronises).
+u64 bitmap_count(dlg_ctx *ctx, const Bitmap &bm);
 // Insert a list; the tuples that were absent go to `out`.
 u64 bitmap_insert_list(dlg_ctx *ctx, const u64 *in, u64 n, bool nodiag, Bitmap &bm, u64 *out);
 // Clear the bits of a list whose bits are the only ones set in their words.
@@ -398,15 +400,16 @@ void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out);
 
 // ------------------------------------------------------------- SG row ORs (k_sgrows.hip)
 struct SgRows {
-    DBuf<u64> poff, counters;   // predecessor list offsets (W + 1); raw rows, R words, F words
-    DBuf<u32> pred, rflag;      // predecessors (relative to lo); nonempty rows of R
-    explicit SgRows(dlg_ctx *c) : poff(c), counters(c), pred(c), rflag(c) {}
+    DBuf<u64> poff, soff, counters;   // predecessor / successor list offsets (W + 1); step counters
+    DBuf<u32> pred, succ, rflag;      // predecessors, successors (relative to lo); nonempty rows of R
+    explicit SgRows(dlg_ctx *c) : poff(c), soff(c), counters(c), pred(c), succ(c), rflag(c) {}
 };
-// Predecessor lists of the index's edges over F's id range.
+// Predecessor and successor lists of the index's edges over F's id range.
 void sg_rows_setup(dlg_ctx *ctx, const Index &ix, const Bitmap &F, SgRows &sr);
-// N = (T ⋈ E) \ F with T in the row-major map R: new bits into F and D (both
-// transposed); R is left empty.  Returns the join's raw rows.
-u64 sg_rows_step(dlg_ctx *ctx, SgRows &sr, Bitmap &R, Bitmap &F, Bitmap &D);
+// One SG round on the maps: T = Δ ⋈ E from Δ in D (D emptied) into the
+// row-major map R, then N = (T ⋈ E) \ F: new bits into F and D (both
+// transposed); R is left empty.  Returns |N|; *raw += both joins' rows.
+u64 sg_rows_round(dlg_ctx *ctx, SgRows &sr, Bitmap &R, Bitmap &F, Bitmap &D, u64 *raw);
 
 // ------------------------------------------------------------- hash set (k_hashset.hip)
 struct HashSet {

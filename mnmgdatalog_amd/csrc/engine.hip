@@ -363,12 +363,11 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     JoinScratch js(ctx);
     int it = 0;
     if (use_bitmap(o, ne, ib)) {
-        // F, the round's T and the round's N as bitmaps.  The first join's
-        // rows go straight into T's map R (k_bitmap.hip; Δ (a, b) in b order
-        // puts the T rows (b, x) of one b into R's row b); the second join is
-        // row ORs of R into F and D (k_sgrows.hip); N comes out of D in bit
-        // order.  SG is symmetric (both rules are), so F equals its transpose
-        // and is emitted without a sort.
+        // F, the round's T and the round's N as bitmaps; both joins of a
+        // round are row operations on them (k_sgrows.hip), Δ never leaves its
+        // map D.  The base rule's rows go straight into F and D (k_bitmap.hip).
+        // SG is symmetric (both rules are), so F equals its transpose and is
+        // emitted without a sort.
         Bitmap F(ctx), R(ctx), D(ctx);
         bitmap_init(ctx, F, ib[0], ib[1], true);
         bitmap_init(ctx, R, ib[0], ib[1], false);
@@ -380,19 +379,17 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         pc.mark(DLG_PH_JOIN);
         bitmap_join_mark(ctx, ix, edge.p, ne, js, DLG_PROJ_HV_PV, true, F, &D, &raw);   // base (y, x)
         edge.release();
-        u64 nd = bitmap_drain(ctx, D, delta, true);
-        u64 total = nd;
+        u64 total = bitmap_count(ctx, D);
         while (true) {
             pc.mark(DLG_PH_JOIN);
-            bitmap_join_mark(ctx, ix, delta.p, nd, js, DLG_PROJ_PV_HV, false, R, nullptr, &raw);  // T (b, x)
-            raw += sg_rows_step(ctx, sr, R, F, D);                                               // N (x, y)
-            u64 nn = bitmap_drain(ctx, D, N, true);
+            const u64 nn = sg_rows_round(ctx, sr, R, F, D, &raw);   // Δ in D -> T in R -> N in D (and F)
             total += nn;
             it++;
-            if (hook) hook->on_round(ctx, N.p, nn, it, false);
+            if (hook) {
+                bitmap_drain(ctx, D, N, true, false);   // N as (x, y) tuples, D kept for the next round
+                hook->on_round(ctx, N.p, nn, it, false);
+            }
             if (nn == 0) break;                     // sg.cu:376-389
-            delta.swap(N);
-            nd = nn;
             if (o.max_it > 0 && it >= o.max_it) break;
         }
         pc.mark(DLG_PH_FINAL);

@@ -536,15 +536,26 @@ static void emit_pass(dlg_ctx *ctx, u32 lo, u64 wpr, u32 *words, const DBuf<u64>
                (unsigned)nb, BE_T, 0, words, (const u64 *)bsum.p, lo, wpr, out);
 }
 
-u64 bitmap_drain(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, bool swap) {
+u64 bitmap_drain(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, bool swap, bool clear) {
     DBuf<u64> bsum(ctx);
     u64 nb = 0;
     u64 h = words_count(ctx, bm.words.p, bm.nwords, bsum, nb);
     out.ensure(h ? h : 1);
     if (h == 0) return 0;
-    if (swap) emit_pass<true, true>(ctx, bm.lo, bm.wpr, bm.words.p, bsum, nb, h, out.p);
-    else emit_pass<false, true>(ctx, bm.lo, bm.wpr, bm.words.p, bsum, nb, h, out.p);
+    if (clear) {
+        if (swap) emit_pass<true, true>(ctx, bm.lo, bm.wpr, bm.words.p, bsum, nb, h, out.p);
+        else emit_pass<false, true>(ctx, bm.lo, bm.wpr, bm.words.p, bsum, nb, h, out.p);
+    } else {
+        if (swap) emit_pass<true, false>(ctx, bm.lo, bm.wpr, bm.words.p, bsum, nb, h, out.p);
+        else emit_pass<false, false>(ctx, bm.lo, bm.wpr, bm.words.p, bsum, nb, h, out.p);
+    }
     return h;
+}
+
+u64 bitmap_count(dlg_ctx *ctx, const Bitmap &bm) {
+    DBuf<u64> bsum(ctx);
+    u64 nb = 0;
+    return words_count(ctx, bm.words.p, bm.nwords, bsum, nb);
 }
 
 // Bit-matrix transpose of a W x S map in 32 x 32-bit blocks.  A workgroup

@@ -1,18 +1,22 @@
-// k_sgrows.hip -- SG's second join as whole-row ORs over the bitmaps.
+// k_sgrows.hip -- a round of SG as row operations on bitmaps.
 //
 // A round of SG (sg.cu:300-375) derives
-//     T(b, x) = Δ(a, b) ⋈ edge(a, x)            (the round's T map R, row b)
-//     N(x, y) = T(b, x) ⋈ edge(b, y) \ F        (new tuples; F and D transposed: row y)
-// The second join needs no per-row work: with R row-major (row b = the x of
-// T(b, x)) and F, D transposed (row y = the x of N(x, y)),
-//     N row y = (OR over b with edge(b, y) of R row b) & ~(F row y)
-// so one workgroup owns row y: it ORs its predecessors' R rows word by word
-// (coalesced, no hashing, no atomics), and writes the new bits into F and D
-// with plain stores.  The join's raw rows, Σ over T(b, x) of outdeg(b), are
-// the popcounts of the R words it reads (each R row b is read once per y
-// with edge(b, y)).  R rows without bits are skipped (a flag per row, set by
-// a scan after the first join) and the flagged rows are cleared afterwards,
-// so R is empty again for the next round.
+//     T(b, x) = Δ(a, b) ⋈ edge(a, x)            (the round's T map R, row-major: row b)
+//     N(x, y) = T(b, x) ⋈ edge(b, y) \ F        (new tuples)
+// with F and D transposed (row y = the x's of N(x, y); D holds the round's
+// new tuples, i.e. the next round's Δ, row b = the a's of Δ(a, b)).  Both
+// joins become work on whole rows, with one owner per row and no atomics in
+// device memory:
+//   T row b = OR over a in D row b of a's successors: one workgroup per row b
+//     ORs the successors' bits into an LDS copy of the row (LDS atomics with
+//     no result), clears D row b as it reads it, and writes R row b (empty
+//     before) and its flag;
+//   N row y = (OR over b with edge(b, y) of R row b) & ~(F row y): one
+//     workgroup per row y ORs its predecessors' flagged R rows word by word
+//     (coalesced) and sets the new bits in F and D with plain stores.
+// The raw join rows are Σ deg(a) over Δ and the popcounts of the R words the
+// second step reads (each R row b is read once per y with edge(b, y)).  The
+// flagged R rows are cleared afterwards, so R is empty again.
 #include "dlg_internal.h"
 
 #include <algorithm>
@@ -35,15 +39,81 @@ __global__ void sg_pred_fill_k(const u64 *rows, u64 n, u32 lo, const u64 *poff, 
     }
 }
 
-// flag[r] = row r of a row-major map has a bit (one wave per row)
-__global__ void bm_row_flags_k(const u32 *words, u64 W, u64 wpr, u32 *flag) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const u64 nw = (u64)gridDim.x * blockDim.x / WAVE;
-    for (u64 r = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; r < W; r += nw) {
-        u32 acc = 0;
-        for (u64 j = lane; j < wpr; j += WAVE) acc |= words[r * wpr + j];
-        const bool any = __ballot(acc != 0) != 0;
-        if (lane == 0) flag[r] = any ? 1u : 0u;
+// successor lists: succ[soff[a] ..] = the x of every edge(a, x), relative to lo
+__global__ void sg_succ_hist_k(const u64 *rows, u64 n, u32 lo, u32 *cnt) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        atomicAdd(cnt + (tkey(rows[i]) - lo), 1u);
+}
+__global__ void sg_succ_fill_k(const u64 *rows, u64 n, u32 lo, u32 *succ) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) succ[i] = tval(rows[i]) - lo;
+}
+
+// T row b = OR over a in D row b of a's successors (row b of D emptied).  The
+// row is built in LDS (wpr words, dynamic shared memory), then R row b (empty
+// before) is written and flagged.  counters[4] += Σ deg(a) (the join's raw
+// rows), counters[5] += the words of D read.
+__global__ __launch_bounds__(SR_T) void sg_tjoin_k(u32 *D, const u64 *soff, const u32 *succ, u64 W, u64 wpr,
+                                                   u32 *R, u32 *rflag, unsigned long long *counters) {
+    extern __shared__ u32 acc[];
+    __shared__ u32 s_any;
+    u64 raw = 0, dwords = 0;
+    for (u64 b = blockIdx.x; b < W; b += gridDim.x) {
+        u32 *drow = D + b * wpr;
+        if (threadIdx.x == 0) s_any = 0;
+        __syncthreads();
+        u32 any = 0;
+        for (u64 j = threadIdx.x; j < wpr; j += SR_T) {
+            acc[j] = 0u;
+            any |= drow[j];
+        }
+        if (any) s_any = 1u;
+        dwords += threadIdx.x < wpr ? (wpr - threadIdx.x + SR_T - 1) / SR_T : 0;
+        __syncthreads();
+        if (!s_any) continue;   // uniform: no Δ tuple with this b
+        for (u64 j = threadIdx.x; j < wpr; j += SR_T) {
+            u32 v = drow[j];
+            if (!v) continue;
+            drow[j] = 0u;   // the row's owner
+            for (; v; v &= v - 1) {
+                const u64 a = j * 32 + (u64)(__ffs(v) - 1);
+                const u64 e0 = soff[a], e1 = soff[a + 1];
+                raw += e1 - e0;
+                for (u64 e = e0; e < e1; e++) {
+                    const u32 x = succ[e];
+                    atomicOr(&acc[x >> 5], 1u << (x & 31));
+                }
+            }
+        }
+        __syncthreads();
+        u32 wrote = 0;
+        for (u64 j = threadIdx.x; j < wpr; j += SR_T) {
+            const u32 v = acc[j];
+            if (v) {
+                R[b * wpr + j] = v;
+                wrote = 1;
+            }
+        }
+        if (wrote) rflag[b] = 1u;
+        __syncthreads();   // acc and s_any are reused by the next row
+    }
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    __shared__ u64 s_red[2][SR_T / WAVE];
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        raw += __shfl_xor(raw, o, WAVE);
+        dwords += __shfl_xor(dwords, o, WAVE);
+    }
+    if (lane == 0) {
+        s_red[0][wv] = raw;
+        s_red[1][wv] = dwords;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        u64 t = 0;
+        for (int w = 0; w < SR_T / WAVE; w++) t += s_red[threadIdx.x][w];
+        if (t) atomicAdd(counters + 4 + threadIdx.x, (unsigned long long)t);
     }
 }
 
@@ -60,14 +130,15 @@ __global__ void bm_clear_rows_k(u32 *words, u64 W, u64 wpr, u32 *flag) {
 
 // N row y = OR of the flagged R rows of y's predecessors, minus F row y;
 // new bits go into F and D.  counters[0] += the raw join rows (popcounts of
-// the R words read), counters[1] += R words read, counters[2] += F words read.
+// the R words read), counters[1] += R words read, counters[2] += F words
+// read, counters[3] += new bits.
 __global__ __launch_bounds__(SR_T) void sg_rowor_k(const u32 *R, const u32 *rflag, const u64 *poff,
                                                    const u32 *pred, u64 W, u64 wpr, u32 *F, u32 *D,
                                                    unsigned long long *counters) {
     __shared__ u32 s_act[SR_T];
     __shared__ u32 s_n;
-    __shared__ u64 s_red[3][SR_T / WAVE];
-    u64 raw = 0, rwords = 0, fwords = 0;
+    __shared__ u64 s_red[4][SR_T / WAVE];
+    u64 raw = 0, rwords = 0, fwords = 0, nnew = 0;
     for (u64 y = blockIdx.x; y < W; y += gridDim.x) {
         const u64 p0 = poff[y], p1 = poff[y + 1];
         for (u64 pb = p0; pb < p1; pb += SR_T) {   // predecessors in batches of SR_T
@@ -103,6 +174,7 @@ __global__ __launch_bounds__(SR_T) void sg_rowor_k(const u32 *R, const u32 *rfla
                         if (nb) {
                             fy[j] = f | nb;
                             dw[j] |= nb;
+                            nnew += (u64)__popc(nb);
                         }
                     }
                 }
@@ -116,14 +188,16 @@ __global__ __launch_bounds__(SR_T) void sg_rowor_k(const u32 *R, const u32 *rfla
         raw += __shfl_xor(raw, o, WAVE);
         rwords += __shfl_xor(rwords, o, WAVE);
         fwords += __shfl_xor(fwords, o, WAVE);
+        nnew += __shfl_xor(nnew, o, WAVE);
     }
     if (lane == 0) {
         s_red[0][wv] = raw;
         s_red[1][wv] = rwords;
         s_red[2][wv] = fwords;
+        s_red[3][wv] = nnew;
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < 4) {
         u64 t = 0;
         for (int w = 0; w < SR_T / WAVE; w++) t += s_red[threadIdx.x][w];
         if (t) atomicAdd(counters + threadIdx.x, (unsigned long long)t);
@@ -144,30 +218,46 @@ void sg_rows_setup(dlg_ctx *ctx, const Index &ix, const Bitmap &F, SgRows &sr) {
     if (n)
         DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * n, sg_pred_fill_k, sgrid(ctx, n), ctx->block_size, 0,
                    (const u64 *)ix.rows.p, n, F.lo, (const u64 *)sr.poff.p, cnt.p, sr.pred.p);
+    // successors: the index's rows are sorted by (source, target), so the
+    // lists are the rows' targets at the rows' offsets by source
+    fill_u32(ctx, cnt.p, W + 1, 0u);
+    if (n)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, sg_succ_hist_k, sgrid(ctx, n), ctx->block_size, 0,
+                   (const u64 *)ix.rows.p, n, F.lo, cnt.p);
+    sr.soff.ensure(W + 1);
+    scan_u32_u64(ctx, cnt.p, W + 1, sr.soff.p);
+    sr.succ.ensure(n ? n : 1);
+    if (n)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, sg_succ_fill_k, sgrid(ctx, n), ctx->block_size, 0,
+                   (const u64 *)ix.rows.p, n, F.lo, sr.succ.p);
     sr.rflag.ensure(W);
     fill_u32(ctx, sr.rflag.p, W, 0u);
-    sr.counters.ensure(3);
+    sr.counters.ensure(6);
     ctx->sync();   // cnt goes out of scope
 }
 
-u64 sg_rows_step(dlg_ctx *ctx, SgRows &sr, Bitmap &R, Bitmap &F, Bitmap &D) {
+u64 sg_rows_round(dlg_ctx *ctx, SgRows &sr, Bitmap &R, Bitmap &F, Bitmap &D, u64 *raw) {
     if (R.tr || !F.tr || !D.tr || R.W != F.W || R.W != D.W || R.lo != F.lo || R.lo != D.lo)
-        fail(DLG_INVALID_INPUT, "sg_rows_step: R row-major, F and D transposed, one geometry");
+        fail(DLG_INVALID_INPUT, "sg_rows_round: R row-major, F and D transposed, one geometry");
     const u64 W = R.W, wpr = R.wpr;
-    const unsigned gw = (unsigned)std::min<u64>((W + 3) / 4, (u64)ctx->num_cus * 16);   // a wave per row
-    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * R.nwords, bm_row_flags_k, gw, 256, 0, (const u32 *)R.words.p, W, wpr,
-               sr.rflag.p);
-    fill_u64(ctx, sr.counters.p, 3, 0ull);
+    if ((u64)ctx->lds_per_block < 4 * wpr + 4096)
+        fail(DLG_INVALID_INPUT, "sg_rows_round: a row of the map does not fit the LDS of a workgroup");
+    fill_u64(ctx, sr.counters.p, 6, 0ull);
     const unsigned g = (unsigned)std::min<u64>(W, (u64)ctx->num_cus * 8);
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, sg_tjoin_k, g, SR_T, (unsigned)(4 * wpr), D.words.p,
+               (const u64 *)sr.soff.p, (const u32 *)sr.succ.p, W, wpr, R.words.p, sr.rflag.p,
+               (unsigned long long *)sr.counters.p);
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, sg_rowor_k, g, SR_T, 0, (const u32 *)R.words.p,
                (const u32 *)sr.rflag.p, (const u64 *)sr.poff.p, (const u32 *)sr.pred.p, W, wpr, F.words.p,
                D.words.p, (unsigned long long *)sr.counters.p);
+    const unsigned gw = (unsigned)std::min<u64>((W + 3) / 4, (u64)ctx->num_cus * 16);   // a wave per row
     DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * W, bm_clear_rows_k, gw, 256, 0, R.words.p, W, wpr, sr.rflag.p);
-    u64 h[3];
-    ctx->read_scalars(sr.counters.p, 3, h);
-    // algorithmic bytes: every R word read, every touched F word read (and written when new)
-    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[1] + 8.0 * h[2]);
-    return h[0];
+    u64 h[6];
+    ctx->read_scalars(sr.counters.p, 6, h);
+    // algorithmic bytes: D words read, 4 B per successor, R words read, touched F words read (+ written)
+    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[5] + 4.0 * h[4] + 4.0 * h[1] + 8.0 * h[2]);
+    if (raw) *raw += h[4] + h[0];
+    return h[3];
 }
 
 }  // namespace dlg
