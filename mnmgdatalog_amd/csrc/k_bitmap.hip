@@ -567,7 +567,28 @@ u64 bitmap_count(dlg_ctx *ctx, const Bitmap &bm) {
 // word -- into a second LDS tile, which goes out with coalesced stores (BT
 // consecutive words per destination row).  Rows are padded to BT + 1 words,
 // so a block's 32 rows sit in 32 different banks.
+// Small maps (a few tiles) go block by block instead: a half wave per block,
+// loading the block's 32 rows and storing its 32 columns directly.
+__global__ void bm_transpose_small_k(const u32 *src, u32 *dst, u64 W, u64 wpr) {
+    const int lane = threadIdx.x & 31;
+    const u64 blk = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 5;   // one block per half wave
+    if (blk >= wpr * wpr) return;   // whole half waves leave together
+    const u64 R = blk / wpr, C = blk - R * wpr;   // source block: rows 32R.., word column C
+    const u64 r = R * 32 + lane;
+    u32 x = r < W ? src[r * wpr + C] : 0u;
+    const u32 m[5] = {0x0000ffffu, 0x00ff00ffu, 0x0f0f0f0fu, 0x33333333u, 0x55555555u};
+#pragma unroll
+    for (int st = 0; st < 5; st++) {
+        const int j = 16 >> st;
+        const u32 y = __shfl_xor(x, j, 32);
+        x = (lane & j) ? ((x & ~m[st]) | ((y & ~m[st]) >> j)) : ((x & m[st]) | ((y & m[st]) << j));
+    }
+    const u64 c = C * 32 + lane;   // destination row = source column
+    if (c < W) dst[c * wpr + R] = x;
+}
+
 constexpr int BT = 16;
+constexpr u64 BT_SMALL_WPR = 256;   // maps up to this many words per row go block by block
 __global__ __launch_bounds__(256) void bm_transpose_k(const u32 *src, u32 *dst, u64 W, u64 wpr) {
     __shared__ u32 tin[32 * BT][BT + 1];
     __shared__ u32 tout[32 * BT][BT + 1];
@@ -614,8 +635,12 @@ u64 bitmap_emit_block(dlg_ctx *ctx, const u32 *words, u32 lo, u64 W, u64 wpr, bo
     fill_u32(ctx, scratch.p + nwords, padded - nwords, 0u);
     if (transposed) {
         const u64 nt = (wpr + BT - 1) / BT;
-        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, bm_transpose_k, (unsigned)(nt * nt), 256, 0, words, scratch.p, W,
-                   wpr);
+        if (wpr <= BT_SMALL_WPR)
+            DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, bm_transpose_small_k, grid_for(wpr * wpr * 32, 256), 256, 0,
+                       words, scratch.p, W, wpr);
+        else
+            DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, bm_transpose_k, (unsigned)(nt * nt), 256, 0, words, scratch.p,
+                       W, wpr);
     } else {
         HIPCHK(hipMemcpyAsync(scratch.p, words, 4 * nwords, hipMemcpyDeviceToDevice, ctx->stream));
     }
