@@ -402,7 +402,8 @@ void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out);
 struct SgRows {
     DBuf<u64> poff, soff, counters;   // predecessor / successor list offsets (W + 1); step counters
     DBuf<u32> pred, succ, rflag;      // predecessors, successors (relative to lo); nonempty rows of R
-    explicit SgRows(dlg_ctx *c) : poff(c), soff(c), counters(c), pred(c), succ(c), rflag(c) {}
+    DBuf<u32> ell;                    // successors as ELL rows when every out-degree is small
+    explicit SgRows(dlg_ctx *c) : poff(c), soff(c), counters(c), pred(c), succ(c), rflag(c), ell(c) {}
 };
 // Predecessor and successor lists of the index's edges over F's id range.
 void sg_rows_setup(dlg_ctx *ctx, const Index &ix, const Bitmap &F, SgRows &sr);

@@ -20,6 +20,7 @@
 #include "dlg_internal.h"
 
 #include <algorithm>
+#include <vector>
 
 namespace dlg {
 
@@ -51,69 +52,116 @@ __global__ void sg_succ_fill_k(const u64 *rows, u64 n, u32 lo, u32 *succ) {
 }
 
 // T row b = OR over a in D row b of a's successors (row b of D emptied).  The
-// row is built in LDS (wpr words, dynamic shared memory), then R row b (empty
-// before) is written and flagged.  counters[4] += Σ deg(a) (the join's raw
-// rows), counters[5] += the words of D read.
-__global__ __launch_bounds__(SR_T) void sg_tjoin_k(u32 *D, const u64 *soff, const u32 *succ, u64 W, u64 wpr,
-                                                   u32 *R, u32 *rflag, unsigned long long *counters) {
+// row is built in LDS (acc: wpr words of dynamic shared memory) and R row b
+// (empty before) is written and flagged.  D row b is taken 256 words at a
+// time; a block scan of the words' popcounts packs their a's into an LDS
+// list (SG_CAP at a time), and the threads take the list's a's in turn, two
+// at a time, so a word dense in a's is spread over the workgroup.  ELL: a's
+// successors are one row of SG_ELL_K ids (SG_EMPTY past the degree), else
+// a CSR list.  counters[4] += Σ deg(a) (the join's raw rows), counters[5] +=
+// the words of D read.
+constexpr int SG_ELL_K = 16;
+constexpr u32 SG_EMPTY = 0xffffffffu;   // ids relative to lo stay below it
+constexpr int SG_CAP = 2048;
+template <bool ELL>
+__global__ __launch_bounds__(SR_T) void sg_tjoin_k(u32 *D, const u64 *soff, const u32 *succ, const uint4 *ell,
+                                                   u64 W, u64 wpr, u32 *R, u32 *rflag,
+                                                   unsigned long long *counters) {
     extern __shared__ u32 acc[];
-    __shared__ u32 s_any;
+    __shared__ u32 alist[SG_CAP];
+    __shared__ u64 s_scan[SR_T / WAVE + 1];
+    const int t = threadIdx.x;
     u64 raw = 0, dwords = 0;
     for (u64 b = blockIdx.x; b < W; b += gridDim.x) {
         u32 *drow = D + b * wpr;
-        if (threadIdx.x == 0) s_any = 0;
-        __syncthreads();
-        u32 any = 0;
-        for (u64 j = threadIdx.x; j < wpr; j += SR_T) {
-            acc[j] = 0u;
-            any |= drow[j];
+        for (u64 j = t; j < wpr; j += SR_T) acc[j] = 0u;
+        u64 rowbits = 0;
+        for (u64 j0 = 0; j0 < wpr; j0 += SR_T) {   // uniform
+            const u64 j = j0 + t;
+            u32 v = 0;
+            if (j < wpr) {
+                v = drow[j];
+                dwords++;
+                if (v) drow[j] = 0u;   // the row's owner
+            }
+            u64 tot;
+            const u64 off = block_exclusive_scan((u64)__popc(v), s_scan, &tot);   // syncs: acc zeroed too
+            rowbits += tot;
+            for (u64 w0 = 0; w0 < tot; w0 += SG_CAP) {   // uniform
+                u64 k = off;
+                for (u32 m = v; m; m &= m - 1, k++)
+                    if (k >= w0 && k < w0 + SG_CAP) alist[k - w0] = (u32)(j * 32 + (u32)(__ffs(m) - 1));
+                __syncthreads();
+                const u32 n = (u32)(tot - w0 < (u64)SG_CAP ? tot - w0 : (u64)SG_CAP);
+                for (u32 i = t; i < n; i += 2 * SR_T) {
+                    const bool two = i + SR_T < n;
+                    const u32 a0 = alist[i], a1 = two ? alist[i + SR_T] : a0;
+                    if constexpr (ELL) {
+                        uint4 e[2][SG_ELL_K / 4];   // both rows' loads in flight
+#pragma unroll
+                        for (int q = 0; q < SG_ELL_K / 4; q++) {
+                            e[0][q] = ell[(u64)a0 * (SG_ELL_K / 4) + q];
+                            e[1][q] = ell[(u64)a1 * (SG_ELL_K / 4) + q];
+                        }
+#pragma unroll
+                        for (int r = 0; r < 2; r++) {
+                            if (r == 1 && !two) break;
+#pragma unroll
+                            for (int q = 0; q < SG_ELL_K / 4; q++) {
+                                const u32 z[4] = {e[r][q].x, e[r][q].y, e[r][q].z, e[r][q].w};
+#pragma unroll
+                                for (int c = 0; c < 4; c++)
+                                    if (z[c] != SG_EMPTY) {
+                                        atomicOr(&acc[z[c] >> 5], 1u << (z[c] & 31));
+                                        raw++;
+                                    }
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 2; r++) {
+                            if (r == 1 && !two) break;
+                            const u32 a = r ? a1 : a0;
+                            const u64 e0 = soff[a], e1 = soff[a + 1];
+                            raw += e1 - e0;
+                            for (u64 e = e0; e < e1; e++) {
+                                const u32 x = succ[e];
+                                atomicOr(&acc[x >> 5], 1u << (x & 31));
+                            }
+                        }
+                    }
+                }
+                __syncthreads();   // the list is refilled by the next window
+            }
         }
-        if (any) s_any = 1u;
-        dwords += threadIdx.x < wpr ? (wpr - threadIdx.x + SR_T - 1) / SR_T : 0;
-        __syncthreads();
-        if (!s_any) continue;   // uniform: no Δ tuple with this b
-        for (u64 j = threadIdx.x; j < wpr; j += SR_T) {
-            u32 v = drow[j];
-            if (!v) continue;
-            drow[j] = 0u;   // the row's owner
-            for (; v; v &= v - 1) {
-                const u64 a = j * 32 + (u64)(__ffs(v) - 1);
-                const u64 e0 = soff[a], e1 = soff[a + 1];
-                raw += e1 - e0;
-                for (u64 e = e0; e < e1; e++) {
-                    const u32 x = succ[e];
-                    atomicOr(&acc[x >> 5], 1u << (x & 31));
+        if (rowbits) {   // uniform: write R row b and its flag
+            u32 wrote = 0;
+            for (u64 j = t; j < wpr; j += SR_T) {
+                const u32 v = acc[j];
+                if (v) {
+                    R[b * wpr + j] = v;
+                    wrote = 1;
                 }
             }
+            if (wrote) rflag[b] = 1u;
         }
-        __syncthreads();
-        u32 wrote = 0;
-        for (u64 j = threadIdx.x; j < wpr; j += SR_T) {
-            const u32 v = acc[j];
-            if (v) {
-                R[b * wpr + j] = v;
-                wrote = 1;
-            }
-        }
-        if (wrote) rflag[b] = 1u;
-        __syncthreads();   // acc and s_any are reused by the next row
+        __syncthreads();   // acc is zeroed again for the next row
     }
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    __shared__ u64 s_red[2][SR_T / WAVE];
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) {
-        raw += __shfl_xor(raw, o, WAVE);
-        dwords += __shfl_xor(dwords, o, WAVE);
+    u64 rt, dt;
+    block_exclusive_scan(raw, s_scan, &rt);
+    block_exclusive_scan(dwords, s_scan, &dt);
+    if (t == 0) {
+        if (rt) atomicAdd(counters + 4, (unsigned long long)rt);
+        if (dt) atomicAdd(counters + 5, (unsigned long long)dt);
     }
-    if (lane == 0) {
-        s_red[0][wv] = raw;
-        s_red[1][wv] = dwords;
-    }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-        u64 t = 0;
-        for (int w = 0; w < SR_T / WAVE; w++) t += s_red[threadIdx.x][w];
-        if (t) atomicAdd(counters + 4 + threadIdx.x, (unsigned long long)t);
+}
+
+// SG's ELL rows: the successors of id a at ell[a * SG_ELL_K ..], relative to lo.
+__global__ void sg_ell_fill_k(const u64 *rows, u64 n, const u64 *soff, u32 lo, u32 *ell) {
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32 a = tkey(rows[i]) - lo;
+        ell[(u64)a * SG_ELL_K + (i - soff[a])] = tval(rows[i]) - lo;
     }
 }
 
@@ -230,6 +278,21 @@ void sg_rows_setup(dlg_ctx *ctx, const Index &ix, const Bitmap &F, SgRows &sr) {
     if (n)
         DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, sg_succ_fill_k, sgrid(ctx, n), ctx->block_size, 0,
                    (const u64 *)ix.rows.p, n, F.lo, sr.succ.p);
+    // every out-degree <= SG_ELL_K: successors as ELL rows (one load level, no offsets)
+    {
+        std::vector<u64> ho(W + 1);
+        HIPCHK(hipMemcpyAsync(ho.data(), sr.soff.p, 8 * (W + 1), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        u64 dm = 0;
+        for (u64 i = 0; i < W; i++) dm = std::max(dm, ho[i + 1] - ho[i]);
+        sr.ell.release();
+        if (n && dm <= (u64)SG_ELL_K) {
+            sr.ell.ensure(W * SG_ELL_K);
+            fill_u32(ctx, sr.ell.p, W * SG_ELL_K, SG_EMPTY);
+            DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * n, sg_ell_fill_k, sgrid(ctx, n), ctx->block_size, 0,
+                       (const u64 *)ix.rows.p, n, (const u64 *)sr.soff.p, F.lo, sr.ell.p);
+        }
+    }
     sr.rflag.ensure(W);
     fill_u32(ctx, sr.rflag.p, W, 0u);
     sr.counters.ensure(6);
@@ -240,13 +303,18 @@ u64 sg_rows_round(dlg_ctx *ctx, SgRows &sr, Bitmap &R, Bitmap &F, Bitmap &D, u64
     if (R.tr || !F.tr || !D.tr || R.W != F.W || R.W != D.W || R.lo != F.lo || R.lo != D.lo)
         fail(DLG_INVALID_INPUT, "sg_rows_round: R row-major, F and D transposed, one geometry");
     const u64 W = R.W, wpr = R.wpr;
-    if ((u64)ctx->lds_per_block < 4 * wpr + 4096)
+    if ((u64)ctx->lds_per_block < 4 * wpr + 4 * SG_CAP + 4096)
         fail(DLG_INVALID_INPUT, "sg_rows_round: a row of the map does not fit the LDS of a workgroup");
     fill_u64(ctx, sr.counters.p, 6, 0ull);
     const unsigned g = (unsigned)std::min<u64>(W, (u64)ctx->num_cus * 8);
-    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, sg_tjoin_k, g, SR_T, (unsigned)(4 * wpr), D.words.p,
-               (const u64 *)sr.soff.p, (const u32 *)sr.succ.p, W, wpr, R.words.p, sr.rflag.p,
-               (unsigned long long *)sr.counters.p);
+    if (sr.ell.p)
+        DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, sg_tjoin_k<true>, g, SR_T, (unsigned)(4 * wpr), D.words.p,
+                   (const u64 *)sr.soff.p, (const u32 *)sr.succ.p, (const uint4 *)sr.ell.p, W, wpr, R.words.p,
+                   sr.rflag.p, (unsigned long long *)sr.counters.p);
+    else
+        DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, sg_tjoin_k<false>, g, SR_T, (unsigned)(4 * wpr), D.words.p,
+                   (const u64 *)sr.soff.p, (const u32 *)sr.succ.p, (const uint4 *)sr.ell.p, W, wpr, R.words.p,
+                   sr.rflag.p, (unsigned long long *)sr.counters.p);
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, sg_rowor_k, g, SR_T, 0, (const u32 *)R.words.p,
                (const u32 *)sr.rflag.p, (const u64 *)sr.poff.p, (const u32 *)sr.pred.p, W, wpr, F.words.p,
                D.words.p, (unsigned long long *)sr.counters.p);

@@ -174,6 +174,17 @@ def test_tc_rows_join_paths(ctx, orc, dmax, dup):
         assert_same(ctx.run("tc", rows, strategy=strategy), want)
 
 
+@pytest.mark.parametrize("dmax,dup", [(9, False), (16, True), (17, False), (40, True)])
+def test_sg_rows_join_paths(ctx, orc, dmax, dup):
+    # SG's first join reads 16-wide ELL rows up to out-degree 16, CSR lists above
+    rows = capped_degree_rows(dmax, 400, dmax, dup)
+    want = orc.sg(rows)
+    for strategy in (0, 4):
+        got = ctx.run("sg", rows, strategy=strategy)
+        assert_same(got, want)
+        assert got["raw_join_rows"] == want["raw_join_rows"]
+
+
 # ------------------------------------------------------------------ published counts, full size
 @pytest.mark.parametrize("name,its,size", [
     ("data_223001.bin", 287, 80_498_014),   # logs/small-merged-job-output.txt:108 (SF.cedge)
