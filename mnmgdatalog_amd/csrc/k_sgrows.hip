@@ -53,8 +53,8 @@ __global__ void sg_succ_fill_k(const u64 *rows, u64 n, u32 lo, u32 *succ) {
 
 // T row b = OR over a in D row b of a's successors (row b of D emptied).  The
 // row is built in LDS (acc: wpr words of dynamic shared memory) and R row b
-// (empty before) is written and flagged.  D row b is taken 256 words at a
-// time; a block scan of the words' popcounts packs their a's into an LDS
+// (empty before) is written and flagged.  D row b is taken SG_KW x 256 words
+// at a time; a block scan of the words' popcounts packs their a's into an LDS
 // list (SG_CAP at a time), and the threads take the list's a's in turn, two
 // at a time, so a word dense in a's is spread over the workgroup.  ELL: a's
 // successors are one row of SG_ELL_K ids (SG_EMPTY past the degree), else
@@ -63,6 +63,7 @@ __global__ void sg_succ_fill_k(const u64 *rows, u64 n, u32 lo, u32 *succ) {
 constexpr int SG_ELL_K = 16;
 constexpr u32 SG_EMPTY = 0xffffffffu;   // ids relative to lo stay below it
 constexpr int SG_CAP = 2048;
+constexpr int SG_KW = 8;   // D words per thread per scan of a row
 template <bool ELL>
 __global__ __launch_bounds__(SR_T) void sg_tjoin_k(u32 *D, const u64 *soff, const u32 *succ, const uint4 *ell,
                                                    u64 W, u64 wpr, u32 *R, u32 *rflag,
@@ -76,21 +77,29 @@ __global__ __launch_bounds__(SR_T) void sg_tjoin_k(u32 *D, const u64 *soff, cons
         u32 *drow = D + b * wpr;
         for (u64 j = t; j < wpr; j += SR_T) acc[j] = 0u;
         u64 rowbits = 0;
-        for (u64 j0 = 0; j0 < wpr; j0 += SR_T) {   // uniform
-            const u64 j = j0 + t;
-            u32 v = 0;
-            if (j < wpr) {
-                v = drow[j];
-                dwords++;
-                if (v) drow[j] = 0u;   // the row's owner
+        for (u64 j0 = 0; j0 < wpr; j0 += SG_KW * SR_T) {   // uniform: SG_KW words per thread
+            u32 v[SG_KW], pc = 0;
+#pragma unroll
+            for (int q = 0; q < SG_KW; q++) {
+                const u64 j = j0 + (u64)q * SR_T + t;
+                v[q] = j < wpr ? drow[j] : 0u;
+                dwords += j < wpr;
+                pc += (u32)__popc(v[q]);
             }
+#pragma unroll
+            for (int q = 0; q < SG_KW; q++)
+                if (v[q]) drow[j0 + (u64)q * SR_T + t] = 0u;   // the row's owner
             u64 tot;
-            const u64 off = block_exclusive_scan((u64)__popc(v), s_scan, &tot);   // syncs: acc zeroed too
+            const u64 off = block_exclusive_scan((u64)pc, s_scan, &tot);   // syncs: acc zeroed too
             rowbits += tot;
             for (u64 w0 = 0; w0 < tot; w0 += SG_CAP) {   // uniform
                 u64 k = off;
-                for (u32 m = v; m; m &= m - 1, k++)
-                    if (k >= w0 && k < w0 + SG_CAP) alist[k - w0] = (u32)(j * 32 + (u32)(__ffs(m) - 1));
+#pragma unroll
+                for (int q = 0; q < SG_KW; q++) {
+                    const u64 j = j0 + (u64)q * SR_T + t;
+                    for (u32 m = v[q]; m; m &= m - 1, k++)
+                        if (k >= w0 && k < w0 + SG_CAP) alist[k - w0] = (u32)(j * 32 + (u32)(__ffs(m) - 1));
+                }
                 __syncthreads();
                 const u32 n = (u32)(tot - w0 < (u64)SG_CAP ? tot - w0 : (u64)SG_CAP);
                 for (u32 i = t; i < n; i += 2 * SR_T) {
