@@ -678,7 +678,8 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     tr.rbound.ensure(na);
     fill_u64(ctx, tr.counters.p, 4, 0ull);   // new ends, next bound total, join rows, next list length
     // algorithmic bytes, added once the round's join rows are known: 4 B per
-    // join row (its matched end) + 20 B per probe (its end, CSR offset, degree)
+    // join row (its matched end) + 20 B per probe (its end, CSR offset,
+    // degree) + 4 B per new end (written as the next round's Δ)
     const u64 nd0 = tr.nd;
     const bool big = tr.nd >= TR_LONG_SEGMENTS * na;   // long segments: big tables; short ones: more waves
     if (tr.ell.p) {
@@ -712,7 +713,7 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     tr.rows = h[1];
     tr.nact = h[3];
     if (raw) *raw += h[2];
-    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[2] + 20.0 * nd0);
+    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[2] + 20.0 * nd0 + 4.0 * h[0]);
     return h[0];
 }
 
