@@ -97,6 +97,9 @@ struct dlg_ctx {
     hipEvent_t cur_a = nullptr;
     // pinned host scalar area for size read-backs
     uint64_t *h_scalars = nullptr;
+    // pinned host words that kernels write directly (mapped): a fixpoint
+    // round's totals, read after the stream synchronises, with no copy
+    uint64_t *h_mapped = nullptr, *d_mapped = nullptr;
 
     void prof_begin();
     void prof_end(int kid, double bytes);
@@ -382,13 +385,13 @@ struct TcRows {
     // aoff (exclusive scan of the bounds); n*: the next round's.  rcnt,
     // rbound: each entry's new ends and next bound; part: chunk sums.
     DBuf<u32> dy, ny, ax, nx, alen, nlen, rcnt;
-    DBuf<u64> abeg, nbeg, aoff, noff, rbound, part, counters;
+    DBuf<u64> abeg, nbeg, aoff, noff, rbound, part, bpart, counters;
     u64 nd = 0, nact = 0, rows = 0;   // rows: the round's output bound (Σ of the entries')
     u32 dmax = 0;                     // nonzero: every degree <= dmax, bounds = new ends * dmax
     u64 wcap = 0;                     // the widest block: no source has more new ends
     explicit TcRows(dlg_ctx *c)
         : csr_off(c), csr_dst(c), deg(c), ell(c), dy(c), ny(c), ax(c), nx(c), alen(c), nlen(c), rcnt(c), abeg(c), nbeg(c),
-          aoff(c), noff(c), rbound(c), part(c), counters(c) {}
+          aoff(c), noff(c), rbound(c), part(c), bpart(c), counters(c) {}
 };
 // CSR of the index's rows; Δ0 = unique(E reversed) grouped by source, its
 // bits set in F (F empty).

@@ -279,9 +279,9 @@ __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32
 // rbound[i].  The bound is the source's next join rows, Σ deg over its new
 // ends (exact; `dmax` = 0), or, when every degree is at most dmax,
 // min(new ends * dmax, W) (no degree lookups; the output has gaps).  The
-// round's new ends and join rows are counted on the way (counters[0],
-// counters[2]).  An entry costs about its bound (join rows) plus a fixed
-// TR_ENTRY_COST, so with cost prefix c(i) = aoff[i] + TR_ENTRY_COST * i,
+// round's new ends and join rows are counted on the way, per workgroup b
+// at bpart[2 b] and bpart[2 b + 1].  An entry costs about its bound (join
+// rows) plus a fixed TR_ENTRY_COST, so with cost prefix c(i) = aoff[i] + TR_ENTRY_COST * i,
 // workgroup b takes the entries with c(i) in [b, b + 1) * c(nact) / grid (a
 // binary search; no global counter), and its four waves share them through
 // an LDS counter.
@@ -289,7 +289,7 @@ template <int LOG, bool ELL>
 __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
     const u32 *dy, const u32 *ax, const u64 *abeg, const u32 *alen, const u64 *aoff, u64 nact, u64 span,
     const u64 *csr_off, const u32 *csr_dst, const u32 *deg, const uint4 *ell, u32 dmax, u32 *F, const u64 *row_base,
-    const u32 *col_lo, u32 lo, u64 wcap, u32 *ny, u32 *rcnt, u64 *rbound, unsigned long long *counters) {
+    const u32 *col_lo, u32 lo, u64 wcap, u32 *ny, u32 *rcnt, u64 *rbound, u64 *bpart) {
     using C = TrCfg<LOG>;
     __shared__ TrWave<LOG> sw[TR_W];
     __shared__ u64 s_rng[2];
@@ -455,8 +455,17 @@ __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
     }
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) wave_rows += __shfl_xor(wave_rows, o, WAVE);
-    if (lane == 0 && wave_new) atomicAdd(counters, (unsigned long long)wave_new);
-    if (lane == 0 && wave_rows) atomicAdd(counters + 2, (unsigned long long)wave_rows);
+    __shared__ u64 s_tot[2][TR_W];
+    if (lane == 0) {
+        s_tot[0][wv] = wave_new;
+        s_tot[1][wv] = wave_rows;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {   // the workgroup's new ends and join rows (summed by the compaction)
+        u64 t = 0;
+        for (int w = 0; w < TR_W; w++) t += s_tot[threadIdx.x][w];
+        bpart[2 * blockIdx.x + threadIdx.x] = t;
+    }
 }
 
 // Σ deg over each entry's ends (the first round's join rows per source).
@@ -477,8 +486,9 @@ __global__ void tc_rows_segrows_k(const u32 *dy, const u64 *abeg, const u32 *ale
 // ends' place (its output offset this round), their count and its next
 // output offset (the exclusive scan of the bounds).  Chunks of TA_CHUNK
 // entries: tc_rows_part_k sums each chunk, tc_rows_compact_k adds the sums
-// of the chunks before its own and writes; its last block writes the totals
-// (counters[1] = Σ bounds, counters[3] = the list's length).
+// of the chunks before its own and writes; its last block writes the round's
+// totals (new ends, Σ bounds, join rows, the list's length) into counters and
+// into the context's mapped host words.
 constexpr int TA_T = 256, TA_I = 8;
 constexpr u64 TA_CHUNK = (u64)TA_T * TA_I;
 
@@ -505,8 +515,8 @@ __global__ __launch_bounds__(TA_T) void tc_rows_part_k(const u32 *rcnt, const u6
 
 __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u64 *aoff, const u32 *rcnt,
                                                           const u64 *rbound, u64 nact, const u64 *part, u32 *nx,
-                                                          u64 *nbeg, u32 *nlen, u64 *noff,
-                                                          unsigned long long *counters) {
+                                                          u64 *nbeg, u32 *nlen, u64 *noff, const u64 *bpart,
+                                                          u32 nbp, u64 *counters, u64 *hcounters) {
     __shared__ u64 scr[TA_T / WAVE + 1];
     u64 pc = 0, ps = 0;   // the chunks before this one
     for (u32 j = threadIdx.x; j < blockIdx.x; j += TA_T) {
@@ -542,9 +552,23 @@ __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u
             es += rbound[i];
         }
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-        counters[1] = bs + ts;
-        counters[3] = bc + tc;
+    if (blockIdx.x == gridDim.x - 1) {   // the round's totals, also into mapped host memory
+        u64 pn = 0, pr = 0;
+        for (u32 j = threadIdx.x; j < nbp; j += TA_T) {
+            pn += bpart[2 * j];
+            pr += bpart[2 * j + 1];
+        }
+        u64 tn, trw;
+        block_exclusive_scan(pn, scr, &tn);
+        block_exclusive_scan(pr, scr, &trw);
+        if (threadIdx.x == 0) {
+            const u64 v[4] = {tn, bs + ts, trw, bc + tc};
+            for (int q = 0; q < 4; q++) {
+                counters[q] = v[q];
+                hcounters[q] = v[q];
+            }
+            __threadfence_system();
+        }
     }
 }
 
@@ -657,14 +681,16 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
 constexpr u64 TR_LONG_SEGMENTS = 64;   // average probes per segment above which the big table wins
 
 template <int LOG, bool ELL>
-static void launch_rows(dlg_ctx *ctx, RowMap &F, TcRows &tr) {
+static unsigned launch_rows(dlg_ctx *ctx, RowMap &F, TcRows &tr) {
     unsigned g = (unsigned)std::min<u64>((tr.nact + TR_W - 1) / TR_W, (u64)ctx->num_cus * TrCfg<LOG>::BLOCKS_PER_CU);
+    tr.bpart.ensure(2 * (u64)g);
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, (tc_rows_k<LOG, ELL>), g, TR_T, 0, (const u32 *)tr.dy.p,
                (const u32 *)tr.ax.p, (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, (const u64 *)tr.aoff.p, tr.nact,
                tr.rows, (const u64 *)tr.csr_off.p, (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p,
                (const uint4 *)tr.ell.p, tr.dmax, F.words.p,
                (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo, tr.wcap, tr.ny.p, tr.rcnt.p, tr.rbound.p,
-               (unsigned long long *)tr.counters.p);
+               tr.bpart.p);
+    return g;
 }
 
 u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
@@ -676,19 +702,14 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     tr.ny.ensure(tr.rows);
     tr.rcnt.ensure(na);
     tr.rbound.ensure(na);
-    fill_u64(ctx, tr.counters.p, 4, 0ull);   // new ends, next bound total, join rows, next list length
     // algorithmic bytes, added once the round's join rows are known: 4 B per
     // join row (its matched end) + 20 B per probe (its end, CSR offset,
     // degree) + 4 B per new end (written as the next round's Δ)
     const u64 nd0 = tr.nd;
     const bool big = tr.nd >= TR_LONG_SEGMENTS * na;   // long segments: big tables; short ones: more waves
-    if (tr.ell.p) {
-        if (big) launch_rows<10, true>(ctx, F, tr);
-        else launch_rows<9, true>(ctx, F, tr);
-    } else {
-        if (big) launch_rows<10, false>(ctx, F, tr);
-        else launch_rows<9, false>(ctx, F, tr);
-    }
+    unsigned g;
+    if (tr.ell.p) g = big ? launch_rows<10, true>(ctx, F, tr) : launch_rows<9, true>(ctx, F, tr);
+    else g = big ? launch_rows<10, false>(ctx, F, tr) : launch_rows<9, false>(ctx, F, tr);
     // the next active list; its ends sit at this round's output offsets
     const u64 nb = (na + TA_CHUNK - 1) / TA_CHUNK;
     tr.part.ensure(2 * nb);
@@ -700,10 +721,13 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
                (const u64 *)tr.rbound.p, na, tr.part.p);
     DLG_LAUNCH(ctx, DLG_K_MISC, 48.0 * na, tc_rows_compact_k, nb, TA_T, 0, (const u32 *)tr.ax.p,
                (const u64 *)tr.aoff.p, (const u32 *)tr.rcnt.p, (const u64 *)tr.rbound.p, na,
-               (const u64 *)tr.part.p, tr.nx.p, tr.nbeg.p, tr.nlen.p, tr.noff.p,
-               (unsigned long long *)tr.counters.p);
+               (const u64 *)tr.part.p, tr.nx.p, tr.nbeg.p, tr.nlen.p, tr.noff.p, (const u64 *)tr.bpart.p, g,
+               tr.counters.p, ctx->d_mapped);
+    // new ends, next bound total, join rows, next list length: written by the
+    // compaction into mapped host memory, read once the stream is done
+    ctx->sync();
     u64 h[4];
-    ctx->read_scalars(tr.counters.p, 4, h);
+    std::memcpy(h, ctx->h_mapped, sizeof(h));
     tr.dy.swap(tr.ny);
     tr.ax.swap(tr.nx);
     tr.abeg.swap(tr.nbeg);

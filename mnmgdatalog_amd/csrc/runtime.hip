@@ -178,6 +178,8 @@ dlg_status dlg_ctx_create(int device, dlg_ctx **out) {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->pool = new Pool(device);
         HIPCHK(hipHostMalloc((void **)&c->h_scalars, 64 * sizeof(uint64_t), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&c->h_mapped, 8 * sizeof(uint64_t), hipHostMallocMapped));
+        HIPCHK(hipHostGetDevicePointer((void **)&c->d_mapped, c->h_mapped, 0));
     } catch (const Error &e) {
         fprintf(stderr, "dlg_ctx_create: %s\n", e.msg.c_str());
         dlg_ctx_destroy(c);
@@ -198,6 +200,7 @@ void dlg_ctx_destroy(dlg_ctx *c) {
     for (auto e : c->event_free) (void)hipEventDestroy(e);
     delete c->pool;
     if (c->h_scalars) (void)hipHostFree(c->h_scalars);
+    if (c->h_mapped) (void)hipHostFree(c->h_mapped);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
