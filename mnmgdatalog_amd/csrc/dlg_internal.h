@@ -383,15 +383,16 @@ struct TcRows {
     // Δ: ends dy grouped by source, and the active list (the sources with
     // ends): source ax, its ends at dy[abeg ..][0, alen), its output offset
     // aoff (exclusive scan of the bounds); n*: the next round's.  rcnt,
-    // rbound: each entry's new ends and next bound; part: chunk sums.
+    // rbound: each entry's new ends and next bound; bpart, brange: the row
+    // kernel's totals and entry range per workgroup.
     DBuf<u32> dy, ny, ax, nx, alen, nlen, rcnt;
-    DBuf<u64> abeg, nbeg, aoff, noff, rbound, part, bpart, counters;
+    DBuf<u64> abeg, nbeg, aoff, noff, rbound, bpart, brange, counters;
     u64 nd = 0, nact = 0, rows = 0;   // rows: the round's output bound (Σ of the entries')
     u32 dmax = 0;                     // nonzero: every degree <= dmax, bounds = new ends * dmax
     u64 wcap = 0;                     // the widest block: no source has more new ends
     explicit TcRows(dlg_ctx *c)
         : csr_off(c), csr_dst(c), deg(c), ell(c), dy(c), ny(c), ax(c), nx(c), alen(c), nlen(c), rcnt(c), abeg(c), nbeg(c),
-          aoff(c), noff(c), rbound(c), part(c), bpart(c), counters(c) {}
+          aoff(c), noff(c), rbound(c), bpart(c), brange(c), counters(c) {}
 };
 // CSR of the index's rows; Δ0 = unique(E reversed) grouped by source, its
 // bits set in F (F empty).

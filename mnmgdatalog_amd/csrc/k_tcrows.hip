@@ -278,18 +278,18 @@ __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32
 // bounds, span its total -- and its new end count and next bound at rcnt[i],
 // rbound[i].  The bound is the source's next join rows, Σ deg over its new
 // ends (exact; `dmax` = 0), or, when every degree is at most dmax,
-// min(new ends * dmax, W) (no degree lookups; the output has gaps).  The
-// round's new ends and join rows are counted on the way, per workgroup b
-// at bpart[2 b] and bpart[2 b + 1].  An entry costs about its bound (join
-// rows) plus a fixed TR_ENTRY_COST, so with cost prefix c(i) = aoff[i] + TR_ENTRY_COST * i,
-// workgroup b takes the entries with c(i) in [b, b + 1) * c(nact) / grid (a
-// binary search; no global counter), and its four waves share them through
-// an LDS counter.
+// min(new ends * dmax, W) (no degree lookups; the output has gaps).
+// Workgroup b leaves its totals at bpart[4 b ..] (entries with new ends,
+// their bounds, new ends, join rows) and its first entry at brange[b].  An
+// entry costs about its bound (join rows) plus a fixed TR_ENTRY_COST, so
+// with cost prefix c(i) = aoff[i] + TR_ENTRY_COST * i, workgroup b takes the
+// entries with c(i) in [b, b + 1) * c(nact) / grid (a binary search; no
+// global counter), and its four waves share them through an LDS counter.
 template <int LOG, bool ELL>
 __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
     const u32 *dy, const u32 *ax, const u64 *abeg, const u32 *alen, const u64 *aoff, u64 nact, u64 span,
     const u64 *csr_off, const u32 *csr_dst, const u32 *deg, const uint4 *ell, u32 dmax, u32 *F, const u64 *row_base,
-    const u32 *col_lo, u32 lo, u64 wcap, u32 *ny, u32 *rcnt, u64 *rbound, u64 *bpart) {
+    const u32 *col_lo, u32 lo, u64 wcap, u32 *ny, u32 *rcnt, u64 *rbound, u64 *bpart, u64 *brange) {
     using C = TrCfg<LOG>;
     __shared__ TrWave<LOG> sw[TR_W];
     __shared__ u64 s_rng[2];
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
     __syncthreads();
     const u64 r0 = s_rng[0], r1 = s_rng[1];
     u32 nused = 0;   // the wave's occupied slots (uniform)
-    u64 wave_new = 0, wave_rows = 0;
+    u64 wave_new = 0, wave_rows = 0, wave_act = 0, wave_bound = 0;
     while (true) {
         u32 k0 = 0;
         if (lane == 0) k0 = atomicAdd(&s_next, 1u);
@@ -451,20 +451,28 @@ __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
                 rbound[ai] = rows_next;
             }
             wave_new += cnt;
+            wave_act += cnt != 0;
+            wave_bound += rows_next;   // zero without new ends
         }
     }
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) wave_rows += __shfl_xor(wave_rows, o, WAVE);
-    __shared__ u64 s_tot[2][TR_W];
+    __shared__ u64 s_tot[4][TR_W];
     if (lane == 0) {
-        s_tot[0][wv] = wave_new;
-        s_tot[1][wv] = wave_rows;
+        s_tot[0][wv] = wave_act;
+        s_tot[1][wv] = wave_bound;
+        s_tot[2][wv] = wave_new;
+        s_tot[3][wv] = wave_rows;
     }
     __syncthreads();
-    if (threadIdx.x < 2) {   // the workgroup's new ends and join rows (summed by the compaction)
+    if (threadIdx.x < 4) {   // the workgroup's totals, for the compaction
         u64 t = 0;
         for (int w = 0; w < TR_W; w++) t += s_tot[threadIdx.x][w];
-        bpart[2 * blockIdx.x + threadIdx.x] = t;
+        bpart[4 * blockIdx.x + threadIdx.x] = t;
+    }
+    if (threadIdx.x == 0) {   // and its range of entries
+        brange[blockIdx.x] = r0;
+        if (blockIdx.x == gridDim.x - 1) brange[gridDim.x] = nact;
     }
 }
 
@@ -484,91 +492,76 @@ __global__ void tc_rows_segrows_k(const u32 *dy, const u64 *abeg, const u32 *ale
 
 // The next active list: the entries with new ends, in order, each with its
 // ends' place (its output offset this round), their count and its next
-// output offset (the exclusive scan of the bounds).  Chunks of TA_CHUNK
-// entries: tc_rows_part_k sums each chunk, tc_rows_compact_k adds the sums
-// of the chunks before its own and writes; its last block writes the round's
-// totals (new ends, Σ bounds, join rows, the list's length) into counters and
-// into the context's mapped host words.
+// output offset (the exclusive scan of the bounds).  One block per block of
+// the round's row kernel, over the same entries: it adds the totals of the
+// blocks before its own and compacts its entries TA_CHUNK at a time.  The
+// last block writes the round's totals (new ends, Σ bounds, join rows, the
+// list's length) into counters and into the context's mapped host words.
 constexpr int TA_T = 256, TA_I = 8;
 constexpr u64 TA_CHUNK = (u64)TA_T * TA_I;
 
-__global__ __launch_bounds__(TA_T) void tc_rows_part_k(const u32 *rcnt, const u64 *rbound, u64 nact, u64 *part) {
-    __shared__ u64 scr[TA_T / WAVE + 1];
-    const u64 b0 = (u64)blockIdx.x * TA_CHUNK;
-    u64 c = 0, s = 0;
-#pragma unroll
-    for (int k = 0; k < TA_I; k++) {
-        const u64 i = b0 + (u64)k * TA_T + threadIdx.x;
-        if (i < nact && rcnt[i]) {
-            c++;
-            s += rbound[i];
-        }
-    }
-    u64 tc, ts;
-    block_exclusive_scan(c, scr, &tc);
-    block_exclusive_scan(s, scr, &ts);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = tc;
-        part[2 * blockIdx.x + 1] = ts;
-    }
-}
-
 __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u64 *aoff, const u32 *rcnt,
-                                                          const u64 *rbound, u64 nact, const u64 *part, u32 *nx,
-                                                          u64 *nbeg, u32 *nlen, u64 *noff, const u64 *bpart,
-                                                          u32 nbp, u64 *counters, u64 *hcounters) {
+                                                          const u64 *rbound, const u64 *bpart, const u64 *brange,
+                                                          u32 *nx, u64 *nbeg, u32 *nlen, u64 *noff, u64 *counters,
+                                                          u64 *hcounters) {
     __shared__ u64 scr[TA_T / WAVE + 1];
-    u64 pc = 0, ps = 0;   // the chunks before this one
-    for (u32 j = threadIdx.x; j < blockIdx.x; j += TA_T) {
-        pc += part[2 * j];
-        ps += part[2 * j + 1];
-    }
-    u64 bc, bs;
-    block_exclusive_scan(pc, scr, &bc);
-    block_exclusive_scan(ps, scr, &bs);
-    const u64 b0 = (u64)blockIdx.x * TA_CHUNK + (u64)threadIdx.x * TA_I;   // TA_I consecutive entries
-    u32 c = 0;
-    u64 s = 0;
-#pragma unroll
-    for (int k = 0; k < TA_I; k++) {
-        const u64 i = b0 + k;
-        if (i < nact && rcnt[i]) {
-            c++;
-            s += rbound[i];
+    const bool last = blockIdx.x == gridDim.x - 1;
+    // the blocks before this one (all of them, for the last block's totals)
+    u64 pc = 0, ps = 0, pn = 0, pr = 0;
+    for (u32 j = threadIdx.x; j < (last ? gridDim.x : blockIdx.x); j += TA_T) {
+        const bool before = j < blockIdx.x;
+        if (before) {
+            pc += bpart[4 * j];
+            ps += bpart[4 * j + 1];
         }
+        pn += bpart[4 * j + 2];
+        pr += bpart[4 * j + 3];
     }
-    u64 tc, ts;
-    u64 ec = block_exclusive_scan((u64)c, scr, &tc) + bc;
-    u64 es = block_exclusive_scan(s, scr, &ts) + bs;
-#pragma unroll
-    for (int k = 0; k < TA_I; k++) {
-        const u64 i = b0 + k;
-        if (i < nact && rcnt[i]) {
-            nx[ec] = ax[i];
-            nbeg[ec] = aoff[i];
-            nlen[ec] = rcnt[i];
-            noff[ec] = es;
-            ec++;
-            es += rbound[i];
-        }
-    }
-    if (blockIdx.x == gridDim.x - 1) {   // the round's totals, also into mapped host memory
-        u64 pn = 0, pr = 0;
-        for (u32 j = threadIdx.x; j < nbp; j += TA_T) {
-            pn += bpart[2 * j];
-            pr += bpart[2 * j + 1];
-        }
-        u64 tn, trw;
+    u64 ec0, es0, tn, trw;
+    block_exclusive_scan(pc, scr, &ec0);
+    block_exclusive_scan(ps, scr, &es0);
+    if (last) {   // uniform
         block_exclusive_scan(pn, scr, &tn);
         block_exclusive_scan(pr, scr, &trw);
-        if (threadIdx.x == 0) {
-            const u64 v[4] = {tn, bs + ts, trw, bc + tc};
-            for (int q = 0; q < 4; q++) {
-                counters[q] = v[q];
-                hcounters[q] = v[q];
+    }
+    const u64 e0 = brange[blockIdx.x], e1 = brange[blockIdx.x + 1];
+    for (u64 t0 = e0; t0 < e1; t0 += TA_CHUNK) {   // uniform
+        const u64 b0 = t0 + (u64)threadIdx.x * TA_I;   // TA_I consecutive entries
+        u32 c = 0;
+        u64 sb = 0;
+#pragma unroll
+        for (int k = 0; k < TA_I; k++) {
+            const u64 i = b0 + k;
+            if (i < e1 && rcnt[i]) {
+                c++;
+                sb += rbound[i];
             }
-            __threadfence_system();
         }
+        u64 tc, ts;
+        u64 ec = block_exclusive_scan((u64)c, scr, &tc) + ec0;
+        u64 es = block_exclusive_scan(sb, scr, &ts) + es0;
+#pragma unroll
+        for (int k = 0; k < TA_I; k++) {
+            const u64 i = b0 + k;
+            if (i < e1 && rcnt[i]) {
+                nx[ec] = ax[i];
+                nbeg[ec] = aoff[i];
+                nlen[ec] = rcnt[i];
+                noff[ec] = es;
+                ec++;
+                es += rbound[i];
+            }
+        }
+        ec0 += tc;
+        es0 += ts;
+    }
+    if (last && threadIdx.x == 0) {   // the round's totals, also into mapped host memory
+        const u64 v[4] = {tn, es0, trw, ec0};
+        for (int q = 0; q < 4; q++) {
+            counters[q] = v[q];
+            hcounters[q] = v[q];
+        }
+        __threadfence_system();
     }
 }
 
@@ -683,13 +676,14 @@ constexpr u64 TR_LONG_SEGMENTS = 64;   // average probes per segment above which
 template <int LOG, bool ELL>
 static unsigned launch_rows(dlg_ctx *ctx, RowMap &F, TcRows &tr) {
     unsigned g = (unsigned)std::min<u64>((tr.nact + TR_W - 1) / TR_W, (u64)ctx->num_cus * TrCfg<LOG>::BLOCKS_PER_CU);
-    tr.bpart.ensure(2 * (u64)g);
+    tr.bpart.ensure(4 * (u64)g);
+    tr.brange.ensure((u64)g + 1);
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, (tc_rows_k<LOG, ELL>), g, TR_T, 0, (const u32 *)tr.dy.p,
                (const u32 *)tr.ax.p, (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, (const u64 *)tr.aoff.p, tr.nact,
                tr.rows, (const u64 *)tr.csr_off.p, (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p,
                (const uint4 *)tr.ell.p, tr.dmax, F.words.p,
                (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo, tr.wcap, tr.ny.p, tr.rcnt.p, tr.rbound.p,
-               tr.bpart.p);
+               tr.bpart.p, tr.brange.p);
     return g;
 }
 
@@ -711,18 +705,13 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     if (tr.ell.p) g = big ? launch_rows<10, true>(ctx, F, tr) : launch_rows<9, true>(ctx, F, tr);
     else g = big ? launch_rows<10, false>(ctx, F, tr) : launch_rows<9, false>(ctx, F, tr);
     // the next active list; its ends sit at this round's output offsets
-    const u64 nb = (na + TA_CHUNK - 1) / TA_CHUNK;
-    tr.part.ensure(2 * nb);
     tr.nx.ensure(na);
     tr.nbeg.ensure(na);
     tr.nlen.ensure(na);
     tr.noff.ensure(na);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * na, tc_rows_part_k, nb, TA_T, 0, (const u32 *)tr.rcnt.p,
-               (const u64 *)tr.rbound.p, na, tr.part.p);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 48.0 * na, tc_rows_compact_k, nb, TA_T, 0, (const u32 *)tr.ax.p,
-               (const u64 *)tr.aoff.p, (const u32 *)tr.rcnt.p, (const u64 *)tr.rbound.p, na,
-               (const u64 *)tr.part.p, tr.nx.p, tr.nbeg.p, tr.nlen.p, tr.noff.p, (const u64 *)tr.bpart.p, g,
-               tr.counters.p, ctx->d_mapped);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 48.0 * na, tc_rows_compact_k, g, TA_T, 0, (const u32 *)tr.ax.p,
+               (const u64 *)tr.aoff.p, (const u32 *)tr.rcnt.p, (const u64 *)tr.rbound.p, (const u64 *)tr.bpart.p,
+               (const u64 *)tr.brange.p, tr.nx.p, tr.nbeg.p, tr.nlen.p, tr.noff.p, tr.counters.p, ctx->d_mapped);
     // new ends, next bound total, join rows, next list length: written by the
     // compaction into mapped host memory, read once the stream is done
     ctx->sync();
