@@ -355,7 +355,8 @@ template <bool SWAP, bool CLEAR>
 __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u32 lo, u64 wpr, u64 *out) {
     __shared__ u64 sc[BE_T / WAVE + 1];
     __shared__ u32 s_w[BE_TILE];
-    __shared__ u64 s_pos[BE_TILE];
+    __shared__ u32 s_pos[BE_TILE];   // each word's first output, from the tile's first (at most 2^16 in a tile)
+    __shared__ u32 s_row[BE_TILE];   // each word's row
     const u64 tile = (u64)blockIdx.x * BE_TILE;
     const u64 base = tile + (u64)threadIdx.x * BE_W;
     u32 w[BE_W];
@@ -384,13 +385,23 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
         }
         return;
     }
+    {
+        const u64 out0 = bpre[blockIdx.x];
+        u64 k = base / wpr, cw = base - k * wpr;   // one division per thread; the words are consecutive
 #pragma unroll
-    for (int i = 0; i < BE_W; i++) {
-        s_w[threadIdx.x * BE_W + i] = w[i];
-        s_pos[threadIdx.x * BE_W + i] = pos;
-        pos += __popc(w[i]);
+        for (int i = 0; i < BE_W; i++) {
+            s_w[threadIdx.x * BE_W + i] = w[i];
+            s_pos[threadIdx.x * BE_W + i] = (u32)(pos - out0);
+            s_row[threadIdx.x * BE_W + i] = (u32)k;
+            pos += __popc(w[i]);
+            if (++cw == wpr) {
+                cw = 0;
+                k++;
+            }
+        }
     }
     __syncthreads();
+    out += bpre[blockIdx.x];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const u32 half = lane >> 5, bit = lane & 31;
     const u32 below = (1u << bit) - 1u;
@@ -399,10 +410,9 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
         if ((x0 | x1) == 0u) continue;   // uniform across the wave
         const u32 x = half ? x1 : x0;
         if (!((x >> bit) & 1u)) continue;
-        const u64 wi = tile + j + half;
-        const u64 k = wi / wpr;                                   // row
-        const u32 v = (u32)((wi - k * wpr) * 32) + bit;           // column
-        out[s_pos[j + half] + __popc(x & below)] = SWAP ? tmake(lo + v, lo + (u32)k) : tmake(lo + (u32)k, lo + v);
+        const u32 k = s_row[j + half];                                       // row
+        const u32 v = (u32)(tile + j + half - (u64)k * wpr) * 32u + bit;      // column
+        out[s_pos[j + half] + __popc(x & below)] = SWAP ? tmake(lo + v, lo + k) : tmake(lo + k, lo + v);
     }
 }
 
