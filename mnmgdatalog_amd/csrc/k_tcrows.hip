@@ -22,6 +22,7 @@
 #include "dlg_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace dlg {
@@ -46,7 +47,6 @@ struct TrCfg {
     static constexpr int H = 1 << LOG;
     static constexpr int NI = LOG >= 10 ? 8 : 4;
     static constexpr u32 LIMIT = H * 3 / 4;
-    static constexpr int BLOCKS_PER_CU = LOG >= 10 ? 3 : 4;   // what LDS (big table) or registers allow
     static_assert(NI * WAVE <= (int)LIMIT && TR_BATCH <= (int)LIMIT && TR_SMALL % NI == 0 && TR_ELL_K % NI == 0,
                   "table load");
 };
@@ -79,10 +79,21 @@ template <int LOG>
 struct TrWave {
     u32 key[TrCfg<LOG>::H];
     u32 bits[TrCfg<LOG>::H];
-    u16 used[TrCfg<LOG>::H];   // occupied slots, in insertion order
-    u64 p_off[WAVE + 1];       // row offsets of the chunk's probes (p_off[WAVE] = total)
-    u64 p_start[WAVE];         // CSR start of each probe's adjacency
+    u16 used[TrCfg<LOG>::LIMIT];   // occupied slots, in insertion order (a flush keeps them <= LIMIT)
 };
+// The CSR join's load-balanced path also keeps the chunk's probes.
+template <int LOG>
+struct TrWaveLB : TrWave<LOG> {
+    u64 p_off[WAVE + 1];   // row offsets of the chunk's probes (p_off[WAVE] = total)
+    u64 p_start[WAVE];     // CSR start of each probe's adjacency
+};
+
+// Workgroups per CU.  With 1024-slot tables, three: the CSR variant's LDS
+// (four 10.5 KB waves) allows no more, and the ELL variant (four 9.5 KB
+// waves) needs 137 registers; at four workgroups it spills 40 bytes per lane
+// and runs no faster.
+template <int LOG, bool ELL>
+constexpr int tr_blocks_per_cu() { return LOG >= 10 ? 3 : 4; }
 
 // OR the lane's entries (word w[i], bits mk[i]; mk[i] = 0: no entry) into
 // the wave's table, one entry at a time (a CAS claims or finds the word's
@@ -286,16 +297,18 @@ __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32
 // entries with c(i) in [b, b + 1) * c(nact) / grid (a binary search; no
 // global counter), and its four waves share them through an LDS counter.
 template <int LOG, bool ELL>
-__global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
+__global__ __launch_bounds__(TR_T, (tr_blocks_per_cu<LOG, ELL>())) void tc_rows_k(
     const u32 *dy, const u32 *ax, const u64 *abeg, const u32 *alen, const u64 *aoff, u64 nact, u64 span,
     const u64 *csr_off, const u32 *csr_dst, const u32 *deg, const uint4 *ell, u32 dmax, u32 *F, const u64 *row_base,
     const u32 *col_lo, u32 lo, u64 wcap, u32 *ny, u32 *rcnt, u64 *rbound, u64 *bpart, u64 *brange) {
     using C = TrCfg<LOG>;
-    __shared__ TrWave<LOG> sw[TR_W];
+    using SW = std::conditional_t<ELL, TrWave<LOG>, TrWaveLB<LOG>>;
+    __shared__ SW sw[TR_W];
     __shared__ u64 s_rng[2];
     __shared__ u32 s_next;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    TrWave<LOG> &s = sw[wv];
+    SW &sl = sw[wv];
+    TrWave<LOG> &s = sl;
     for (int i = lane; i < C::H; i += WAVE) {
         s.key[i] = TR_EMPTY;
         s.bits[i] = 0u;
@@ -393,9 +406,9 @@ __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
                             u64 R;
                             u64 ex = wave_excl_scan(d[h], &R);
                             wave_sync();
-                            s.p_off[lane] = ex;
-                            s.p_start[lane] = st[h];
-                            if (lane == 0) s.p_off[WAVE] = R;
+                            sl.p_off[lane] = ex;
+                            sl.p_start[lane] = st[h];
+                            if (lane == 0) sl.p_off[WAVE] = R;
                             wave_sync();
                             for (u64 rb = 0; rb < R; rb += TR_BATCH) {
                                 const u64 re = R < rb + TR_BATCH ? R : rb + TR_BATCH;
@@ -408,9 +421,9 @@ __global__ __launch_bounds__(TR_T, TrCfg<LOG>::BLOCKS_PER_CU) void tc_rows_k(
                                         u32 a = 0, b = WAVE;   // last probe with p_off <= r
                                         while (b - a > 1) {
                                             u32 m = (a + b) >> 1;
-                                            if (s.p_off[m] <= r) a = m; else b = m;
+                                            if (sl.p_off[m] <= r) a = m; else b = m;
                                         }
-                                        ci[q] = s.p_start[a] + (r - s.p_off[a]);
+                                        ci[q] = sl.p_start[a] + (r - sl.p_off[a]);
                                     }
                                 }
                                 u32 w[TR_U], mk[TR_U];
@@ -675,7 +688,7 @@ constexpr u64 TR_LONG_SEGMENTS = 64;   // average probes per segment above which
 
 template <int LOG, bool ELL>
 static unsigned launch_rows(dlg_ctx *ctx, RowMap &F, TcRows &tr) {
-    unsigned g = (unsigned)std::min<u64>((tr.nact + TR_W - 1) / TR_W, (u64)ctx->num_cus * TrCfg<LOG>::BLOCKS_PER_CU);
+    unsigned g = (unsigned)std::min<u64>((tr.nact + TR_W - 1) / TR_W, (u64)ctx->num_cus * tr_blocks_per_cu<LOG, ELL>());
     tr.bpart.ensure(4 * (u64)g);
     tr.brange.ensure((u64)g + 1);
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, (tc_rows_k<LOG, ELL>), g, TR_T, 0, (const u32 *)tr.dy.p,
