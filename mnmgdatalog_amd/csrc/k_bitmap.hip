@@ -600,37 +600,50 @@ __global__ void bm_transpose_small_k(const u32 *src, u32 *dst, u64 W, u64 wpr) {
 constexpr int BT = 16;
 constexpr u64 BT_SMALL_WPR = 256;   // maps up to this many words per row go block by block
 __global__ __launch_bounds__(256) void bm_transpose_k(const u32 *src, u32 *dst, u64 W, u64 wpr) {
-    __shared__ u32 tin[32 * BT][BT + 1];
-    __shared__ u32 tout[32 * BT][BT + 1];
+    __shared__ u32 tb[32 * BT][BT + 1];   // the tile, transposed in place
     const u64 nt = (wpr + BT - 1) / BT;
     const u64 R0 = (u64)(blockIdx.x / nt) * BT, C0 = (u64)(blockIdx.x % nt) * BT;   // tile: block row, word column
     const int t = threadIdx.x;
-#pragma unroll 4
-    for (int k = 0; k < 32 * BT * BT / 256; k++) {
+    constexpr int NK = 32 * BT * BT / 256;   // words per thread, all loaded before any is stored
+    u32 v[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
         const int rr = t / BT + (256 / BT) * k, cc = t % BT;
         const u64 r = R0 * 32 + rr, c = C0 + cc;
-        tin[rr][cc] = r < W && c < wpr ? src[r * wpr + c] : 0u;
+        v[k] = r < W && c < wpr ? src[r * wpr + c] : 0u;
     }
+#pragma unroll
+    for (int k = 0; k < NK; k++) tb[t / BT + (256 / BT) * k][t % BT] = v[k];
     __syncthreads();
+    // each half wave transposes BT * BT / 8 blocks into registers, then (after
+    // every half wave has read its blocks) writes them back transposed
     const int lane = t & 31;
     const u32 m[5] = {0x0000ffffu, 0x00ff00ffu, 0x0f0f0f0fu, 0x33333333u, 0x55555555u};
-    for (int bk = t >> 5; bk < BT * BT; bk += 256 / 32) {
-        const int Rl = bk / BT, Cl = bk % BT;
-        u32 x = tin[32 * Rl + lane][Cl];
+    constexpr int NB = BT * BT / (256 / 32);
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int bk = (t >> 5) + (256 / 32) * i, Rl = bk / BT, Cl = bk % BT;
+        u32 x = tb[32 * Rl + lane][Cl];
 #pragma unroll
         for (int st = 0; st < 5; st++) {
             const int j = 16 >> st;
             const u32 y = __shfl_xor(x, j, 32);
             x = (lane & j) ? ((x & ~m[st]) | ((y & ~m[st]) >> j)) : ((x & m[st]) | ((y & m[st]) << j));
         }
-        tout[32 * Cl + lane][Rl] = x;   // destination row = source column
+        v[i] = x;
     }
     __syncthreads();
-#pragma unroll 4
-    for (int k = 0; k < 32 * BT * BT / 256; k++) {
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int bk = (t >> 5) + (256 / 32) * i, Rl = bk / BT, Cl = bk % BT;
+        tb[32 * Cl + lane][Rl] = v[i];   // destination row = source column
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
         const int rr = t / BT + (256 / BT) * k, cc = t % BT;
         const u64 r = C0 * 32 + rr, c = R0 + cc;
-        if (r < W && c < wpr) dst[r * wpr + c] = tout[rr][cc];
+        if (r < W && c < wpr) dst[r * wpr + c] = tb[rr][cc];
     }
 }
 

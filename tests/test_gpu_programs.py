@@ -389,6 +389,19 @@ def test_tc_grid_all_layouts(ctx, orc, L):
         assert_same(ctx.run("tc", rows, strategy=s), want)
 
 
+@pytest.mark.parametrize("W", [8_200, 20_017])
+def test_tc_bitmap_tiled_transpose(ctx, orc, W):
+    """Id ranges of more than 256 words per row: the read-out transposes the
+    map by tiles of 16 x 16 blocks, with partial tiles at the edges."""
+    rng = np.random.default_rng(W)
+    ids = np.sort(rng.choice(W, size=1500, replace=False)).astype(np.int32)
+    ids[0], ids[-1] = 0, W - 1                                   # the whole id range
+    src = rng.integers(0, len(ids) - 1, size=2500)
+    dst = np.minimum(src + 1 + rng.integers(0, 40, size=2500), len(ids) - 1)   # forward edges: a DAG
+    rows = np.stack([ids[src], ids[dst]], 1).astype(np.int32)
+    assert_same(ctx.run("tc", rows, strategy=4), orc.tc(rows))
+
+
 def test_output_digest_matches_rows(ctx, orc):
     from grid_closure import rows_digest
     rows = read_bin("data_7035.bin")
