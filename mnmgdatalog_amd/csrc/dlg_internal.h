@@ -33,6 +33,7 @@ struct Error {
 
 [[noreturn]] void fail(dlg_status s, const std::string &msg);
 void check_hip(hipError_t e, const char *what, const char *file, int line);
+hipEvent_t timing_event();   // an event for timing only (no system-scope fence)
 #define HIPCHK(call) ::dlg::check_hip((call), #call, __FILE__, __LINE__)
 
 // ------------------------------------------------------------------ pool

@@ -25,11 +25,7 @@ PhaseClock::~PhaseClock() {
 }
 
 void PhaseClock::mark(int ph) {
-    if (ctx->event_free.empty()) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        ctx->event_free.push_back(e);
-    }
+    if (ctx->event_free.empty()) ctx->event_free.push_back(timing_event());
     hipEvent_t e = ctx->event_free.back();
     ctx->event_free.pop_back();
     HIPCHK(hipEventRecord(e, ctx->stream));

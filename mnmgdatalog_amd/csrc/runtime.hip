@@ -105,14 +105,22 @@ void Pool::trim_to(size_t keep) {
     }
 }
 
+// Timing events only (read by hipEventElapsedTime, never used to order
+// memory): no system-scope fence, so recording one does not write back and
+// invalidate the caches between two kernels.
+hipEvent_t timing_event() {
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    return e;
+}
+
 }  // namespace dlg
 
 using namespace dlg;
 
 void dlg_ctx::prof_begin() {
     if (event_free.empty()) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
+        hipEvent_t e = timing_event();
         event_free.push_back(e);
     }
     cur_a = event_free.back();
@@ -122,8 +130,7 @@ void dlg_ctx::prof_begin() {
 
 void dlg_ctx::prof_end(int kid, double bytes) {
     if (event_free.empty()) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
+        hipEvent_t e = timing_event();
         event_free.push_back(e);
     }
     hipEvent_t b = event_free.back();
