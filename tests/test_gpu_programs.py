@@ -377,6 +377,20 @@ def test_bitmap_strategy_rejects_wide_id_range(ctx):
     assert e.value.status == dlg.DLG_INVALID_INPUT
 
 
+@pytest.mark.parametrize("prog,lo", [("tc", -2**31), ("sg", 2**31 - 370_720)])
+def test_bitmap_widest_id_range(ctx, orc, prog, lo):
+    """The widest id range the bitmap layout takes, W = 370,720 (370,720 x
+    11,585 words, just under 2^32), at either end of the int32 ids."""
+    W = 370_720
+    rng = np.random.default_rng(5)
+    off = np.sort(rng.choice(W, size=2000, replace=False)).astype(np.int64)
+    off[0], off[-1] = 0, W - 1                                   # the whole id range
+    src = rng.integers(0, len(off) - 1, size=3000)
+    dst = np.minimum(src + 1 + rng.integers(0, 30, size=3000), len(off) - 1)   # forward edges: a DAG
+    rows = (np.stack([off[src], off[dst]], 1) + lo).astype(np.int32)
+    assert_same(ctx.run(prog, rows, strategy=4), getattr(orc, prog)(rows))
+
+
 # ------------------------------------------------------------------ the road-shaped grid
 GRID_SEED = 0x4D4E4D47
 
