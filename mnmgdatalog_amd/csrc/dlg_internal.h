@@ -99,8 +99,11 @@ struct dlg_ctx {
     // pinned host scalar area for size read-backs
     uint64_t *h_scalars = nullptr;
     // pinned host words that kernels write directly (mapped): a fixpoint
-    // round's totals, read after the stream synchronises, with no copy
+    // round's totals, then (after a system-scope fence) the round's sequence
+    // number at MAPPED_SEQ; the host waits for that number, with no copy
     uint64_t *h_mapped = nullptr, *d_mapped = nullptr;
+    uint64_t mapped_seq = 0;
+    static constexpr int MAPPED_SEQ = 4;
 
     void prof_begin();
     void prof_end(int kid, double bytes);
@@ -110,6 +113,10 @@ struct dlg_ctx {
         if (prof) kstat[kid].bytes += bytes;
     }
     void sync();
+    // Wait until a kernel has written `seq` to h_mapped[MAPPED_SEQ] (spinning:
+    // no sleep between the kernel's write and the host's next launch); fails
+    // if the stream runs dry without it.
+    void wait_mapped(uint64_t seq);
     // read `count` u64 scalars from device memory into host (synchronises)
     void read_scalars(const uint64_t *d, int count, uint64_t *h);
 };

@@ -516,7 +516,7 @@ constexpr u64 TA_CHUNK = (u64)TA_T * TA_I;
 __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u64 *aoff, const u32 *rcnt,
                                                           const u64 *rbound, const u64 *bpart, const u64 *brange,
                                                           u32 *nx, u64 *nbeg, u32 *nlen, u64 *noff, u64 *counters,
-                                                          u64 *hcounters) {
+                                                          u64 *hcounters, u64 seq) {
     __shared__ u64 scr[TA_T / WAVE + 1];
     const bool last = blockIdx.x == gridDim.x - 1;
     // the blocks before this one (all of them, for the last block's totals)
@@ -568,12 +568,14 @@ __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u
         ec0 += tc;
         es0 += ts;
     }
-    if (last && threadIdx.x == 0) {   // the round's totals, also into mapped host memory
+    if (last && threadIdx.x == 0) {   // the round's totals, also into mapped host memory, then its number
         const u64 v[4] = {tn, es0, trw, ec0};
         for (int q = 0; q < 4; q++) {
             counters[q] = v[q];
             hcounters[q] = v[q];
         }
+        __threadfence_system();
+        *(volatile u64 *)(hcounters + dlg_ctx::MAPPED_SEQ) = seq;
         __threadfence_system();
     }
 }
@@ -722,14 +724,19 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     tr.nbeg.ensure(na);
     tr.nlen.ensure(na);
     tr.noff.ensure(na);
+    const u64 seq = ++ctx->mapped_seq;
     DLG_LAUNCH(ctx, DLG_K_MISC, 48.0 * na, tc_rows_compact_k, g, TA_T, 0, (const u32 *)tr.ax.p,
                (const u64 *)tr.aoff.p, (const u32 *)tr.rcnt.p, (const u64 *)tr.rbound.p, (const u64 *)tr.bpart.p,
-               (const u64 *)tr.brange.p, tr.nx.p, tr.nbeg.p, tr.nlen.p, tr.noff.p, tr.counters.p, ctx->d_mapped);
+               (const u64 *)tr.brange.p, tr.nx.p, tr.nbeg.p, tr.nlen.p, tr.noff.p, tr.counters.p, ctx->d_mapped,
+               seq);
     // new ends, next bound total, join rows, next list length: written by the
-    // compaction into mapped host memory, read once the stream is done
-    ctx->sync();
+    // compaction into mapped host memory before the round's number.  The
+    // host goes on while the compaction's other blocks may still run: none
+    // of the buffers they use can grow (the list never lengthens), and the
+    // next launches queue behind them on the stream.
+    ctx->wait_mapped(seq);
     u64 h[4];
-    std::memcpy(h, ctx->h_mapped, sizeof(h));
+    for (int q = 0; q < 4; q++) h[q] = ((volatile u64 *)ctx->h_mapped)[q];
     tr.dy.swap(tr.ny);
     tr.ax.swap(tr.nx);
     tr.abeg.swap(tr.nbeg);

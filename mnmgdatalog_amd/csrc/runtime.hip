@@ -2,6 +2,7 @@
 #include "dlg_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 
 namespace dlg {
@@ -156,6 +157,24 @@ void dlg_ctx::prof_flush() {
 }
 
 void dlg_ctx::sync() { HIPCHK(hipStreamSynchronize(stream)); }
+
+void dlg_ctx::wait_mapped(uint64_t seq) {
+    volatile uint64_t *w = h_mapped + MAPPED_SEQ;
+    for (unsigned spin = 1; *w != seq; spin++) {
+        if (spin % 1024 == 0) {   // now and then: has the stream finished without writing it?
+            const hipError_t e = hipStreamQuery(stream);
+            if (e == hipSuccess) {
+                if (*w == seq) break;
+                fail(DLG_DEVICE_ERROR, "a round's totals never reached the host");
+            }
+            if (e != hipErrorNotReady) HIPCHK(e);
+        }
+#if defined(__x86_64__)
+        __builtin_ia32_pause();
+#endif
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);   // the totals were written before the number
+}
 
 void dlg_ctx::read_scalars(const uint64_t *d, int count, uint64_t *h) {
     HIPCHK(hipMemcpyAsync(h_scalars, d, sizeof(uint64_t) * count, hipMemcpyDeviceToHost, stream));
