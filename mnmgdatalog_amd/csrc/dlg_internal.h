@@ -103,7 +103,7 @@ struct dlg_ctx {
     // number at MAPPED_SEQ; the host waits for that number, with no copy
     uint64_t *h_mapped = nullptr, *d_mapped = nullptr;
     uint64_t mapped_seq = 0;
-    static constexpr int MAPPED_SEQ = 4;
+    static constexpr int MAPPED_SEQ = 7;
 
     void prof_begin();
     void prof_end(int kid, double bytes);

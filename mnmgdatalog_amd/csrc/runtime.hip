@@ -204,7 +204,9 @@ dlg_status dlg_ctx_create(int device, dlg_ctx **out) {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->pool = new Pool(device);
         HIPCHK(hipHostMalloc((void **)&c->h_scalars, 64 * sizeof(uint64_t), hipHostMallocDefault));
-        HIPCHK(hipHostMalloc((void **)&c->h_mapped, 8 * sizeof(uint64_t), hipHostMallocMapped));
+        HIPCHK(hipHostMalloc((void **)&c->h_mapped, (dlg_ctx::MAPPED_SEQ + 1) * sizeof(uint64_t),
+                             hipHostMallocMapped));
+        std::memset(c->h_mapped, 0, (dlg_ctx::MAPPED_SEQ + 1) * sizeof(uint64_t));   // no sequence number yet
         HIPCHK(hipHostGetDevicePointer((void **)&c->d_mapped, c->h_mapped, 0));
     } catch (const Error &e) {
         fprintf(stderr, "dlg_ctx_create: %s\n", e.msg.c_str());
