@@ -35,7 +35,7 @@ constexpr int TR_ELL_K = 8;   // degrees up to this: the join reads an ELL adjac
 constexpr int TR_U = 4;       // words / rows per lane with their loads in flight together
 constexpr int TR_BATCH = TR_U * WAVE;        // rows per batch of the load-balanced path
 constexpr u32 TR_DMAX_BOUND = 16;   // degrees up to this: output regions bounded by new ends * max degree
-constexpr u64 TR_ENTRY_COST = 256;  // an entry's fixed cost (loads, final flush) in join rows
+constexpr u64 TR_ENTRY_COST = 512;  // an entry's fixed cost (loads, final flush) in join rows
 
 // The word table of a wave: 2^LOG slots, linear probing.  A larger table
 // flushes a long segment fewer times; a smaller one leaves LDS for more waves
