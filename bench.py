@@ -37,7 +37,7 @@ SEED = 0x4D4E4D47
 # FETCH_SIZE and WRITE_SIZE runs of `bench.py --workload W --steps 1
 # --warmup 0 --no-cpu`, scaled by the copy-kernel calibration (FETCH x 2.0,
 # WRITE x 1.0): bytes per launch, keyed by (workload, kernel id).
-MEASURED_TRAFFIC = {("tc_grid", "bitmap_insert"): 3.722e9}
+MEASURED_TRAFFIC = {("tc_grid", "bitmap_insert"): 4.045e9}
 
 
 def workload_rows(name):

@@ -45,7 +45,7 @@ constexpr u64 TR_ENTRY_COST = 512;  // an entry's fixed cost (loads, final flush
 template <int LOG>
 struct TrCfg {
     static constexpr int H = 1 << LOG;
-    static constexpr int NI = LOG >= 10 ? 8 : 4;
+    static constexpr int NI = 4;
     static constexpr u32 LIMIT = H * 3 / 4;
     static_assert(NI * WAVE <= (int)LIMIT && TR_BATCH <= (int)LIMIT && TR_SMALL % NI == 0 && TR_ELL_K % NI == 0,
                   "table load");
@@ -88,12 +88,13 @@ struct TrWaveLB : TrWave<LOG> {
     u64 p_start[WAVE];     // CSR start of each probe's adjacency
 };
 
-// Workgroups per CU.  With 1024-slot tables, three: the CSR variant's LDS
-// (four 10.5 KB waves) allows no more, and the ELL variant (four 9.5 KB
-// waves) needs 137 registers; at four workgroups it spills 40 bytes per lane
-// and runs no faster.
+// Workgroups per CU: four, except for the CSR variant with 1024-slot tables,
+// whose LDS (four 10.5 KB waves) allows three.  At four workgroups the ELL
+// variants are held to 128 registers and spill 24 bytes per lane; with
+// 1024-slot tables that is still 4.5 % faster on the 400 x 400 grid than
+// three workgroups without spills.
 template <int LOG, bool ELL>
-constexpr int tr_blocks_per_cu() { return LOG >= 10 ? 3 : 4; }
+constexpr int tr_blocks_per_cu() { return LOG >= 10 && !ELL ? 3 : 4; }
 
 // OR the lane's entries (word w[i], bits mk[i]; mk[i] = 0: no entry) into
 // the wave's table, one entry at a time (a CAS claims or finds the word's
