@@ -117,15 +117,37 @@ void dlg_hash_free(dlg_ctx *ctx, dlg_hash *h);
 /* ------------------------------------------------------------------- join */
 typedef enum dlg_proj {
     DLG_PROJ_HV_PV = 0,  /* (ht.value, probe.value): get_join_result_entity (common/join.cu:38-41) */
-    DLG_PROJ_PV_HV = 1   /* (probe.value, ht.value): the same join followed by reverse_entity_ar
+    DLG_PROJ_PV_HV = 1,  /* (probe.value, ht.value): the same join followed by reverse_entity_ar
                             (sg.cu:252-253, 300-301) */
+    DLG_PROJ_HV_PV_DROP_EQ = 2,  /* DLG_PROJ_HV_PV, then the rows with key == value removed:
+                                    SG's base rule, get_join + remove_if(is_key_equal_value)
+                                    (sg.cu:166-182; common/utils.cu:90-95) */
+    DLG_PROJ_PV_HV_DROP_EQ = 3   /* DLG_PROJ_PV_HV with the same removal */
 } dlg_proj;
 /* Replaces get_join(grid, block, ht, ht_rows, probe, n, &out_n, &secs)
  * (common/join.cu:51-87): count pass, wavefront/block prefix scan, then a
  * load-balanced materialize pass.  *d_out is allocated by the library (NULL
- * when *out_n == 0) and released with dlg_dev_free. */
+ * when *out_n == 0) and released with dlg_dev_free.  The DROP_EQ projections
+ * keep the remaining rows in join order, as the stable remove_if does. */
 dlg_status dlg_join(dlg_ctx *ctx, const dlg_hash *h, const uint64_t *d_probe, int64_t n,
                     int proj, uint64_t **d_out, int64_t *out_n);
+
+/* --------------------------------------------------------- WCC min-label round */
+/* One round of WCC's loop body (wcc.cu:255-339) on tuples:
+ *   J   = {(m, l) : (n, l) in delta, (n, m) in E}   get_join over the edge index
+ *                                                    (wcc.cu:256-259)
+ *   cc' = per node, the least label of cc u J        sort/unique of J, merge with cc,
+ *                                                    unique(is_equal_key) (wcc.cu:285-323)
+ *   Δ'  = cc' minus cc                               set_difference (wcc.cu:333-336)
+ * `edges` is dlg_hash_build of the symmetric edge relation E u E^-1
+ * (wcc.cu:140-157, 247).  d_cc holds cc: one (node, label) tuple per distinct
+ * key of the index, in key order (the reference's cc, wcc.cu:160-175); it is
+ * updated in place to cc'.  Δ' goes to d_next (room for n_cc tuples), sorted;
+ * *n_next = |Δ'|.  d_delta need not be sorted.  A join row whose node is not a
+ * key of the index (E not symmetric), or n_cc different from the index's key
+ * count, gives DLG_INVALID_INPUT (d_cc is then unspecified). */
+dlg_status dlg_wcc_relax(dlg_ctx *ctx, const dlg_hash *edges, const uint64_t *d_delta, int64_t n_delta,
+                         uint64_t *d_cc, int64_t n_cc, uint64_t *d_next, int64_t *n_next);
 
 /* ------------------------------------------------- relational operators */
 /* thrust::sort(set_cmp) + thrust::unique(is_equal) (tc.cu:250-253 and every

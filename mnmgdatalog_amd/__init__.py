@@ -28,7 +28,7 @@ KERNELS = ["pack", "hash_build", "join_count", "scan", "join_expand", "sort_minm
            "sort_downsweep", "select", "merge", "wcc_relax", "wcc_final", "hashset_insert",
            "bitmap_insert", "misc"]
 
-PROJ_HV_PV, PROJ_PV_HV = 0, 1
+PROJ_HV_PV, PROJ_PV_HV, PROJ_HV_PV_DROP_EQ, PROJ_PV_HV_DROP_EQ = 0, 1, 2, 3
 ORIENT_KV, ORIENT_VK = 0, 1
 
 
@@ -107,6 +107,7 @@ def lib() -> ctypes.CDLL:
         "dlg_hash_info": (c_int, [vp, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
         "dlg_hash_free": (None, [vp, vp]),
         "dlg_join": (c_int, [vp, vp, vp, i64, c_int, POINTER(c_void_p), POINTER(c_int64)]),
+        "dlg_wcc_relax": (c_int, [vp, vp, vp, i64, vp, i64, vp, POINTER(c_int64)]),
         "dlg_sort_unique": (c_int, [vp, vp, i64, POINTER(c_int64)]),
         "dlg_sort": (c_int, [vp, vp, i64]),
         "dlg_set_difference": (c_int, [vp, vp, i64, vp, i64, vp, POINTER(c_int64)]),
@@ -365,6 +366,10 @@ class Ctx:
             self.free(pr)
             self.free(pp)
 
+    def hash_index(self, rel: np.ndarray, load_factor: float = 0.0) -> "HashIndex":
+        """get_hash_table(rel): an index kept on the device for several joins / WCC rounds."""
+        return HashIndex(self, rel, load_factor)
+
     # ---- profiling
     def profiling(self, on: bool):
         self.check(self.L.dlg_profiling(self.h, 1 if on else 0), "profiling")
@@ -446,6 +451,64 @@ class Ctx:
             return out
         finally:
             self.L.dlg_result_free(byref(r))
+
+
+class HashIndex:
+    """A dlg_hash (dlg_hash_build over a copy of `rel`) bound to one context."""
+
+    def __init__(self, ctx: Ctx, rel: np.ndarray, load_factor: float = 0.0):
+        self.ctx = ctx
+        self.h = c_void_p()
+        pr = ctx.upload(np.asarray(rel, np.uint64))
+        try:
+            ctx.check(ctx.L.dlg_hash_build(ctx.h, c_void_p(pr), len(rel), load_factor, byref(self.h)), "hash_build")
+        finally:
+            ctx.free(pr)
+
+    def info(self) -> dict:
+        slots, keys, rows = c_int64(), c_int64(), c_int64()
+        self.ctx.L.dlg_hash_info(self.h, byref(slots), byref(keys), byref(rows))
+        return {"slots": slots.value, "keys": keys.value, "rows": rows.value}
+
+    def join(self, probe: np.ndarray, proj: int = PROJ_HV_PV) -> np.ndarray:
+        """get_join(ht, probe) with the given projection, rows in the library's output order."""
+        c = self.ctx
+        pp = c.upload(np.asarray(probe, np.uint64))
+        out, n = c_void_p(), c_int64()
+        try:
+            c.check(c.L.dlg_join(c.h, self.h, c_void_p(pp), len(probe), proj, byref(out), byref(n)), "join")
+            return c.download(out.value, n.value, np.uint64) if n.value else np.zeros(0, np.uint64)
+        finally:
+            if out.value:
+                c.free(out.value)
+            c.free(pp)
+
+    def wcc_relax(self, delta: np.ndarray, cc: np.ndarray):
+        """One reference WCC round (dlg_wcc_relax): returns (cc', delta')."""
+        c = self.ctx
+        cc = np.ascontiguousarray(cc, np.uint64)
+        pd = c.upload(np.asarray(delta, np.uint64))
+        pc = c.upload(cc)
+        pn = c.alloc(8 * max(len(cc), 1))
+        k = c_int64()
+        try:
+            c.check(c.L.dlg_wcc_relax(c.h, self.h, c_void_p(pd), len(delta), c_void_p(pc), len(cc), c_void_p(pn),
+                                      byref(k)), "wcc_relax")
+            return c.download(pc, len(cc), np.uint64), c.download(pn, k.value, np.uint64)
+        finally:
+            for p in (pd, pc, pn):
+                c.free(p)
+
+    def close(self):
+        if self.h:
+            self.ctx.L.dlg_hash_free(self.ctx.h, self.h)
+            self.h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Server:

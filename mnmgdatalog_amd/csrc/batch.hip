@@ -152,6 +152,7 @@ struct Layout {
     std::vector<int64_t> lo, shift;   // id' = id + shift[i]
     std::vector<u32> ub;              // flipped key of range starts, B + 1 entries
     int64_t total_rows = 0;
+    int exhausted = 0;                // first instance that starts past the last id (B: none)
 };
 
 static Layout make_layout(int B, const int32_t *const *rows, const int64_t *n) {
@@ -161,8 +162,10 @@ static Layout make_layout(int B, const int32_t *const *rows, const int64_t *n) {
     L.shift.resize(B);
     L.ub.resize(B + 1);
     int64_t base = INT32_MIN;
+    L.exhausted = B;
     for (int i = 0; i < B; i++) {
         if (n[i] < 0 || (n[i] > 0 && !rows[i])) fail(DLG_INVALID_INPUT, "bad instance rows");
+        if (base > (int64_t)INT32_MAX && L.exhausted == B) L.exhausted = i;
         int64_t lo = INT64_MAX, hi = INT64_MIN;
         for (int64_t r = 0; r < 2 * n[i]; r++) {
             lo = std::min<int64_t>(lo, rows[i][r]);
@@ -279,6 +282,22 @@ static void split_rows(const std::vector<int32_t> &all, const std::vector<u64> &
 void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const int64_t *n,
                       const Opts &o, dlg_result *res, dlg_batch_stats *st) {
     Layout L = make_layout(B, rows, n);
+    if (L.exhausted < B) {
+        // The instances from L.exhausted on start where the ids are used up, so
+        // they are empty (a wider one overflows above); their range start would
+        // be the last id of the instance before them, whose tuples the range
+        // searches would then credit to them.  They run as a batch of their own.
+        const int E = L.exhausted;
+        dlg_batch_stats s2;
+        std::memset(&s2, 0, sizeof(s2));
+        batch_run(ctx, prog, E, rows, n, o, res, st);
+        batch_run(ctx, prog, B - E, rows + E, n + E, o, res + E, &s2);
+        st->device_seconds += s2.device_seconds;
+        st->rounds = std::max(st->rounds, s2.rounds);
+        st->total_output += s2.total_output;
+        st->union_rows += s2.union_rows;
+        return;
+    }
     if (L.total_rows >= 0x7fffffffLL) fail(DLG_OVERFLOW, "batch has more than 2^31-2 rows");
     std::vector<int32_t> un((size_t)(2 * L.total_rows + 2));
     {

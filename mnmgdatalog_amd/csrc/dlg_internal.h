@@ -104,6 +104,7 @@ struct dlg_ctx {
     uint64_t *h_mapped = nullptr, *d_mapped = nullptr;
     uint64_t mapped_seq = 0;
     static constexpr int MAPPED_SEQ = 7;
+    static constexpr int MAPPED_SPIN_US = 2000;   // busy-wait this long, then yield between checks
 
     void prof_begin();
     void prof_end(int kid, double bytes);
@@ -266,6 +267,10 @@ __device__ __forceinline__ u64 upper_bound_u64(const u64 *a, u64 lo, u64 hi, u64
 u64 scan_u32_u64(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out);
 // Same without reading the total back; total left at d_total (device).
 void scan_u32_u64_async(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *d_total);
+// Partials (u64s) a scan of n elements uses.
+u64 scan_partials(u64 n);
+// Same over caller-provided partials bsum[scan_partials(n)] (allocates nothing).
+void scan_u32_u64_with(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *bsum, u64 *d_total);
 // Exclusive scan of u64 values (out may alias in); total left at d_total.
 void scan_u64_async(dlg_ctx *ctx, const u64 *in, u64 n, u64 *out, u64 *d_total);
 // Stable LSD radix sort of u64 keys; result in `data`.  `tmp` is scratch of >= n.
@@ -288,6 +293,7 @@ enum SelectMode {
     SEL_UNIQUE_NOT_IN = 2,  // ... and drop elements present in sorted `other`
     SEL_NOT_IN = 3,         // drop elements present in sorted `other`
     SEL_KEY_HEAD = 4,       // output INDEX of first element of each equal-key run
+    SEL_NODIAG = 5,         // keep every element with key != value (any order, duplicates kept)
 };
 // Writes kept elements (or indices for SEL_KEY_HEAD, as u32) to out; returns count.
 // `out` must not alias `in`.
@@ -474,6 +480,13 @@ struct JoinScratch {
 u64 join_count(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js);
 void join_expand(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, const JoinScratch &js,
                  u64 total, int proj, u64 *out);
+// The same two phases over caller-provided scratch (no allocation): cnt and
+// pstart of n u32s, offs of n + 1 u64s, bsum of scan_partials(n) u64s and
+// one u64 at d_total.  join_count_with returns |J| (synchronises).
+u64 join_count_with(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, u32 *cnt, u32 *pstart, u64 *offs,
+                    u64 *bsum, u64 *d_total);
+void join_expand_with(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, const u32 *pstart, const u64 *offs,
+                      u64 total, int proj, u64 *out);
 // Dense lookup of every value of `ix.rows` as a key: out[i] = group id (u32) or ~0u.
 void index_lookup_values(dlg_ctx *ctx, const Index &ix, u32 *out);
 
@@ -488,6 +501,12 @@ struct WccState {
 void wcc_setup(dlg_ctx *ctx, WccState &s, u64 V);
 // One round over frontier of size nf; returns the new frontier size.
 u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf, u32 round);
+// The reference's round on tuples (dlg_wcc_relax): cc (one (node, label)
+// tuple per group of `ix`, in group order) lowered in place to the per-node
+// least label of cc and Δ ⋈ E; returns |Δ'| with Δ' = cc' minus cc (sorted) in
+// `next`.  nbr[r] = the group of row r's value.
+u64 wcc_relax_tuples(dlg_ctx *ctx, const Index &ix, const u32 *nbr, const u64 *delta, u64 nd, u64 *cc, u64 ncc,
+                     u64 *next);
 // #components, largest component size and the (label, 0) tuples in `out`.
 u64 wcc_finalize(dlg_ctx *ctx, WccState &s, const Index &ix, DBuf<u64> &out, i64 *largest);
 // All rounds through replays of a captured graph of `kr` rounds; returns the

@@ -104,3 +104,13 @@ void download_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, int32_t *h_row
 void check_rows_arg(int64_t n, const void *rows);
 
 }  // namespace dlg
+
+// The index handed across the C ABI (dlg_hash_build).  `nbr` (the index's
+// group of every row's value, ~0 when the value is no key) is built on the
+// first dlg_wcc_relax call that needs it.
+struct dlg_hash {
+    dlg::Index ix;
+    mutable dlg::DBuf<dlg::u32> nbr;
+    mutable bool nbr_built = false;
+    explicit dlg_hash(dlg_ctx *c) : ix(c), nbr(c) {}
+};

@@ -575,11 +575,6 @@ static void run_wcc(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, d
 // ====================================================================== C ABI
 using namespace dlg;
 
-struct dlg_hash {
-    Index ix;
-    explicit dlg_hash(dlg_ctx *c) : ix(c) {}
-};
-
 typedef void (*runner_t)(dlg_ctx *, const int32_t *, u64, const Opts &, dlg_result *);
 
 static dlg_status run_device(dlg_ctx *ctx, runner_t fn, const int32_t *d_rows, int64_t n,
@@ -705,21 +700,52 @@ void dlg_hash_free(dlg_ctx *c, dlg_hash *h) {
 
 dlg_status dlg_join(dlg_ctx *c, const dlg_hash *h, const uint64_t *d_probe, int64_t n, int proj,
                     uint64_t **d_out, int64_t *out_n) {
-    if (!c || !h || !d_out || !out_n || n < 0 || (n && !d_probe) ||
-        (proj != DLG_PROJ_HV_PV && proj != DLG_PROJ_PV_HV))
+    if (!c || !h || !d_out || !out_n || n < 0 || (n && !d_probe) || proj < DLG_PROJ_HV_PV ||
+        proj > DLG_PROJ_PV_HV_DROP_EQ)
         return DLG_INVALID_INPUT;
     *d_out = nullptr;
     *out_n = 0;
     API_BEGIN(c)
     JoinScratch js(c);
     u64 total = join_count(c, h->ix, d_probe, (u64)n, js);
-    if (total) {
+    const bool drop_eq = proj >= DLG_PROJ_HV_PV_DROP_EQ;
+    const int base = drop_eq ? proj - DLG_PROJ_HV_PV_DROP_EQ : proj;
+    if (total && !drop_eq) {
         u64 *o = (u64 *)c->pool->get(total * sizeof(u64));
-        join_expand(c, h->ix, d_probe, (u64)n, js, total, proj, o);
+        join_expand(c, h->ix, d_probe, (u64)n, js, total, base, o);
         c->sync();
         *d_out = o;
+    } else if (total) {
+        // get_join, then remove_if(is_key_equal_value): a stable compaction
+        DBuf<u64> all(c, total), kept(c, total);
+        join_expand(c, h->ix, d_probe, (u64)n, js, total, base, all.p);
+        total = select(c, SEL_NODIAG, all.p, total, nullptr, 0, kept.p);
+        c->sync();
+        if (total) {   // the kept rows' buffer goes to the caller as it is
+            *d_out = kept.p;
+            kept.p = nullptr;
+            kept.cap = 0;
+        }
     }
     *out_n = (int64_t)total;
+    API_END(c)
+}
+
+dlg_status dlg_wcc_relax(dlg_ctx *c, const dlg_hash *h, const uint64_t *d_delta, int64_t n_delta, uint64_t *d_cc,
+                         int64_t n_cc, uint64_t *d_next, int64_t *n_next) {
+    if (!c || !h || !n_next || n_delta < 0 || n_cc < 0 || (n_delta && !d_delta) || (n_cc && (!d_cc || !d_next)))
+        return DLG_INVALID_INPUT;
+    *n_next = 0;
+    API_BEGIN(c)
+    if ((u64)n_cc != h->ix.groups) fail(DLG_INVALID_INPUT, "dlg_wcc_relax: n_cc differs from the index's key count");
+    if (n_delta && n_cc) {
+        if (!h->nbr_built) {   // the index's group of every row's value, kept with the index
+            h->nbr.ensure(h->ix.n);
+            index_lookup_values(c, h->ix, h->nbr.p);
+            h->nbr_built = true;
+        }
+        *n_next = (int64_t)wcc_relax_tuples(c, h->ix, h->nbr.p, d_delta, (u64)n_delta, d_cc, (u64)n_cc, d_next);
+    }
     API_END(c)
 }
 

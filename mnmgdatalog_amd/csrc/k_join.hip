@@ -161,30 +161,42 @@ __global__ __launch_bounds__(1024) void join_count_sorted_k(const u64 *probe, u6
     }
 }
 
+// The count kernel of the index's lookup kind: cnt[i], pstart[i] per probe.
+static void join_count_launch(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, u32 *cnt, u32 *pstart) {
+    if (ix.sorted) {
+        // probe read + ~2 log2(n) row reads per probe + count/start writes
+        double lg = std::log2((double)ix.n + 1.0);
+        DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 16.0 * lg * n + 8.0 * n, join_count_sorted_k, sgrid(ctx, n),
+                   ctx->block_size, 0, probe, n, (const u64 *)ix.rows.p, ix.n, cnt, pstart);
+    } else if (ix.dense_w) {
+        // probe read + an offset pair per probe + count/start writes
+        DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 16.0 * n + 8.0 * n, join_count_dense_k, sgrid(ctx, n),
+                   ctx->block_size, 0, probe, n, (const u64 *)ix.dense.p, ix.dense_lo, ix.dense_w, cnt, pstart);
+    } else {
+        // probe read + one slot line per probe + gstart pair + count/start writes
+        DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 8.0 * n + 8.0 * n + 8.0 * n, join_count_k, sgrid(ctx, n),
+                   ctx->block_size, 0, probe, n, (const u64 *)ix.slots.p, ix.nslots - 1, (const u32 *)ix.gstart.p,
+                   cnt, pstart);
+    }
+}
+
 u64 join_count(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js) {
     if (n == 0 || ix.n == 0) return 0;
     js.cnt.ensure(n);
     js.pstart.ensure(n);
     js.offs.ensure(n + 1);
-    if (ix.sorted) {
-        // probe read + ~2 log2(n) row reads per probe + count/start writes
-        double lg = std::log2((double)ix.n + 1.0);
-        DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 16.0 * lg * n + 8.0 * n, join_count_sorted_k, sgrid(ctx, n),
-                   ctx->block_size, 0, probe, n, (const u64 *)ix.rows.p, ix.n, js.cnt.p, js.pstart.p);
-        return scan_u32_u64(ctx, js.cnt.p, n, js.offs.p);
-    }
-    if (ix.dense_w) {
-        // probe read + an offset pair per probe + count/start writes
-        DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 16.0 * n + 8.0 * n, join_count_dense_k, sgrid(ctx, n),
-                   ctx->block_size, 0, probe, n, (const u64 *)ix.dense.p, ix.dense_lo, ix.dense_w, js.cnt.p,
-                   js.pstart.p);
-        return scan_u32_u64(ctx, js.cnt.p, n, js.offs.p);
-    }
-    // probe read + one slot line per probe + gstart pair + count/start writes
-    DLG_LAUNCH(ctx, DLG_K_JOIN_COUNT, 8.0 * n + 8.0 * n + 8.0 * n + 8.0 * n, join_count_k,
-               sgrid(ctx, n), ctx->block_size, 0, probe, n, (const u64 *)ix.slots.p, ix.nslots - 1, (const u32 *)ix.gstart.p,
-               js.cnt.p, js.pstart.p);
+    join_count_launch(ctx, ix, probe, n, js.cnt.p, js.pstart.p);
     return scan_u32_u64(ctx, js.cnt.p, n, js.offs.p);
+}
+
+u64 join_count_with(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, u32 *cnt, u32 *pstart, u64 *offs,
+                    u64 *bsum, u64 *d_total) {
+    if (n == 0 || ix.n == 0) return 0;
+    join_count_launch(ctx, ix, probe, n, cnt, pstart);
+    scan_u32_u64_with(ctx, cnt, n, offs, bsum, d_total);
+    u64 h = 0;
+    ctx->read_scalars(d_total, 1, &h);
+    return h;
 }
 
 constexpr int EX_T = 256, EX_I = 8, EX_TILE = EX_T * EX_I;
@@ -250,19 +262,22 @@ __global__ __launch_bounds__(EX_T) void join_expand_k(const u64 *probe, u64 np, 
     }
 }
 
-void join_expand(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, const JoinScratch &js,
-                 u64 total, int proj, u64 *out) {
+void join_expand_with(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, const u32 *pstart, const u64 *offs,
+                      u64 total, int proj, u64 *out) {
     if (total == 0) return;
     unsigned g = grid_for(total, EX_TILE);
     double bytes = 24.0 * total;  // relation value read, output write, probe/offset reads amortised
     if (proj == DLG_PROJ_PV_HV)
-        DLG_LAUNCH(ctx, DLG_K_JOIN_EXPAND, bytes, join_expand_k<DLG_PROJ_PV_HV>, g, EX_T, 0, probe,
-                   n, (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total,
-                   (const u64 *)ix.rows.p, out);
+        DLG_LAUNCH(ctx, DLG_K_JOIN_EXPAND, bytes, join_expand_k<DLG_PROJ_PV_HV>, g, EX_T, 0, probe, n, pstart, offs,
+                   total, (const u64 *)ix.rows.p, out);
     else
-        DLG_LAUNCH(ctx, DLG_K_JOIN_EXPAND, bytes, join_expand_k<DLG_PROJ_HV_PV>, g, EX_T, 0, probe,
-                   n, (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total,
-                   (const u64 *)ix.rows.p, out);
+        DLG_LAUNCH(ctx, DLG_K_JOIN_EXPAND, bytes, join_expand_k<DLG_PROJ_HV_PV>, g, EX_T, 0, probe, n, pstart, offs,
+                   total, (const u64 *)ix.rows.p, out);
+}
+
+void join_expand(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, const JoinScratch &js,
+                 u64 total, int proj, u64 *out) {
+    join_expand_with(ctx, ix, probe, n, js.pstart.p, js.offs.p, total, proj, out);
 }
 
 __global__ void lookup_values_k(const u64 *rows, u64 n, const u64 *slots, u64 mask, u32 *out) {

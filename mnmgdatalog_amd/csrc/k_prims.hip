@@ -147,16 +147,30 @@ __global__ __launch_bounds__(SCAN_T) void scan_apply_k(const T *in, u64 n, const
     }
 }
 
+u64 scan_partials(u64 n) {
+    u64 nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    return nb ? nb : 1;
+}
+
+// The three launches of a scan over caller-provided partials (scan_partials(n) u64s).
+template <typename T>
+static void scan_to_u64_with(dlg_ctx *ctx, const T *in, u64 n, u64 *out, u64 *bsum, u64 *d_total) {
+    const u64 nb = scan_partials(n);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, (double)sizeof(T) * n + 8.0 * nb, scan_reduce_k<T>, (unsigned)nb, SCAN_T, 0,
+               in, n, bsum);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum, nb, d_total);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, (sizeof(T) + 8.0) * n, scan_apply_k<T>, (unsigned)nb, SCAN_T, 0, in, n,
+               (const u64 *)bsum, out);
+}
+
 template <typename T>
 static void scan_to_u64_async(dlg_ctx *ctx, const T *in, u64 n, u64 *out, u64 *d_total) {
-    u64 nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb == 0) nb = 1;
-    DBuf<u64> bsum(ctx, nb);
-    DLG_LAUNCH(ctx, DLG_K_SCAN, (double)sizeof(T) * n + 8.0 * nb, scan_reduce_k<T>, (unsigned)nb, SCAN_T, 0,
-               in, n, bsum.p);
-    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb, d_total);
-    DLG_LAUNCH(ctx, DLG_K_SCAN, (sizeof(T) + 8.0) * n, scan_apply_k<T>, (unsigned)nb, SCAN_T, 0, in, n, bsum.p,
-               out);
+    DBuf<u64> bsum(ctx, scan_partials(n));
+    scan_to_u64_with<T>(ctx, in, n, out, bsum.p, d_total);
+}
+
+void scan_u32_u64_with(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *bsum, u64 *d_total) {
+    scan_to_u64_with<u32>(ctx, in, n, out, bsum, d_total);
 }
 
 void scan_u32_u64_async(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *d_total) {
@@ -550,6 +564,8 @@ __global__ __launch_bounds__(SEL_T) void select_count_k(const u64 *in, u64 n, co
                 keep = idx == 0 || tkey(in[idx - 1]) != tkey(x);
             } else if (MODE == SEL_NOT_IN) {
                 keep = !found_sorted(other, s_lo, s_hi, x);
+            } else if (MODE == SEL_NODIAG) {
+                keep = tkey(x) != tval(x);
             } else {
                 keep = idx == 0 || in[idx - 1] != x;
                 if (MODE == SEL_UNIQUE_NODIAG) keep = keep && tkey(x) != tval(x);
@@ -631,6 +647,10 @@ u64 select(dlg_ctx *ctx, int mode, const u64 *in, u64 n, const u64 *other, u64 n
         case SEL_KEY_HEAD:
             DLG_LAUNCH(ctx, DLG_K_SELECT, rb, select_count_k<SEL_KEY_HEAD>, (unsigned)nb, SEL_T,
                        0, in, n, other, n_other, flags.p, bsum.p);
+            break;
+        case SEL_NODIAG:
+            DLG_LAUNCH(ctx, DLG_K_SELECT, rb, select_count_k<SEL_NODIAG>, (unsigned)nb, SEL_T, 0,
+                       in, n, other, n_other, flags.p, bsum.p);
             break;
         default: fail(DLG_INVALID_INPUT, "select: bad mode");
     }

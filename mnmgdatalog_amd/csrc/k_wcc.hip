@@ -150,6 +150,66 @@ u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf
     return h;
 }
 
+// One thread per join row of Δ ⋈ E (load-balanced like wcc_relax_k): row r
+// of probe i gives node m = value of rows[r] at cc position nbr[r] and the
+// candidate (m, label of Δ[i]); cc[nbr[r]] takes the lesser.  A value that is
+// no key, or a cc tuple of another node, raises *bad.
+__global__ __launch_bounds__(WR_T) void wcc_tuple_relax_k(const u64 *delta, u64 nd, const u32 *pstart,
+                                                          const u64 *offs, u64 total, const u64 *rows,
+                                                          const u32 *nbr, u64 ncc, u64 *cc, u32 *bad) {
+    __shared__ u64 s_lo, s_hi;
+    const u64 base = (u64)blockIdx.x * WR_TILE;
+    if (threadIdx.x == 0) {
+        u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
+        u64 lo = upper_bound_u64(offs, 0, nd, base) - 1;
+        s_lo = lo;
+        s_hi = upper_bound_u64(offs, lo, nd, last);
+    }
+    __syncthreads();
+    const u64 lo = s_lo, hi = s_hi;
+#pragma unroll
+    for (int r = 0; r < WR_I; r++) {
+        const u64 o = base + (u64)r * WR_T + threadIdx.x;
+        if (o < total) {
+            const u64 i = upper_bound_u64(offs, lo, hi, o) - 1;
+            const u64 row = (u64)pstart[i] + (o - offs[i]);
+            const u32 g = nbr[row];
+            const u64 t = tmake(tval(rows[row]), tval(delta[i]));
+            if (g >= ncc) {
+                *bad = 1u;
+            } else {
+                const u64 cur = cc[g];
+                if (tkey(cur) != tkey(t)) *bad = 1u;
+                else if (cur > t) atomicMin((unsigned long long *)&cc[g], (unsigned long long)t);
+            }
+        }
+    }
+}
+
+u64 wcc_relax_tuples(dlg_ctx *ctx, const Index &ix, const u32 *nbr, const u64 *delta, u64 nd, u64 *cc, u64 ncc,
+                     u64 *next) {
+    if (nd == 0 || ncc == 0 || ix.n == 0) return 0;
+    DBuf<u64> old(ctx, ncc);
+    copy_u64(ctx, old.p, cc, ncc);
+    JoinScratch js(ctx);
+    const u64 total = join_count(ctx, ix, delta, nd, js);
+    DBuf<u32> bad(ctx, 1);
+    fill_u32(ctx, bad.p, 1, 0u);
+    if (total)
+        // 4 B group + 8 B row + 8 B cc read per join row (the atomic only when lower), Δ reads amortised
+        DLG_LAUNCH(ctx, DLG_K_WCC_RELAX, 20.0 * total + 8.0 * nd, wcc_tuple_relax_k, grid_for(total, WR_TILE), WR_T,
+                   0, delta, nd, (const u32 *)js.pstart.p, (const u64 *)js.offs.p, total, (const u64 *)ix.rows.p,
+                   nbr, ncc, cc, bad.p);
+    u32 hb = 0;
+    HIPCHK(hipMemcpyAsync(ctx->h_scalars, bad.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    hb = *(u32 *)ctx->h_scalars;
+    if (hb) fail(DLG_INVALID_INPUT, "dlg_wcc_relax: a join row's node is not in cc (edges not symmetric, or cc "
+                                    "not one tuple per key in key order)");
+    // cc' keeps cc's node order, so both are sorted: Δ' = cc' minus cc
+    return select(ctx, SEL_NOT_IN, cc, ncc, old.p, ncc, next);
+}
+
 void wcc_setup(dlg_ctx *ctx, WccState &s, u64 V) {
     s.V = V;
     s.L.ensure(V);

@@ -3,7 +3,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdio>
+#include <thread>
 
 namespace dlg {
 
@@ -160,6 +162,11 @@ void dlg_ctx::sync() { HIPCHK(hipStreamSynchronize(stream)); }
 
 void dlg_ctx::wait_mapped(uint64_t seq) {
     volatile uint64_t *w = h_mapped + MAPPED_SEQ;
+    // Spin for about one long round (the next launch goes out the moment the
+    // number lands), then yield the core between checks so that other host
+    // threads (the served form's slots and admission) keep running.
+    const auto t0 = std::chrono::steady_clock::now();
+    bool yielding = false;
     for (unsigned spin = 1; *w != seq; spin++) {
         if (spin % 1024 == 0) {   // now and then: has the stream finished without writing it?
             const hipError_t e = hipStreamQuery(stream);
@@ -168,10 +175,16 @@ void dlg_ctx::wait_mapped(uint64_t seq) {
                 fail(DLG_DEVICE_ERROR, "a round's totals never reached the host");
             }
             if (e != hipErrorNotReady) HIPCHK(e);
+            if (!yielding)
+                yielding = std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(MAPPED_SPIN_US);
         }
+        if (yielding) {
+            std::this_thread::yield();
+        } else {
 #if defined(__x86_64__)
-        __builtin_ia32_pause();
+            __builtin_ia32_pause();
 #endif
+        }
     }
     std::atomic_thread_fence(std::memory_order_acquire);   // the totals were written before the number
 }

@@ -93,6 +93,20 @@ def test_batch_id_space_overflow_is_reported(ctx):
     assert e.value.status == dlg.DLG_OVERFLOW
 
 
+@pytest.mark.parametrize("prog", ["tc", "wcc"])
+def test_batch_empty_instances_after_the_whole_id_space(ctx, orc, prog):
+    """An instance spanning every int32 id, then empty ones: the empties start past the
+    last id and must not take the tuples keyed by INT32_MAX."""
+    whole = np.array([[-2**31, 2**31 - 1], [2**31 - 1, 5], [5, -2**31]], np.int32)
+    empty = np.zeros((0, 2), np.int32)
+    inst = [whole, empty, empty]
+    res, st = ctx.batch_run(prog, inst)
+    for r, rows in zip(res, inst):
+        check(r, ctx.run(prog, rows), secondary=prog == "wcc")
+    check(res[0], getattr(orc, prog)(whole), secondary=prog == "wcc")
+    assert st["total_output"] == res[0]["output_size"]
+
+
 def test_autotune_picks_a_candidate_and_keeps_results(ctx, orc):
     rows = read_bin("data_23874.bin")
     best = ctx.autotune("tc", rows, load_factors=[0.4, 0.8], radix_bits=[6, 8], block_sizes=[128, 256])

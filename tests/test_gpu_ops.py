@@ -148,3 +148,95 @@ def test_invalid_arguments_report_status(ctx):
     with pytest.raises(dlg.DlgError) as e:
         ctx.run("tc", np.zeros((3, 2), np.int32), load_factor=2.0)
     assert e.value.status == dlg.DLG_INVALID_INPUT
+
+
+@pytest.mark.parametrize("proj", [dlg.PROJ_HV_PV_DROP_EQ, dlg.PROJ_PV_HV_DROP_EQ])
+def test_hash_join_drop_eq(ctx, proj):
+    """get_join + remove_if(is_key_equal_value) (sg.cu:166-182): the plain join's rows, in the
+    same order, minus the rows with key == value."""
+    rng = np.random.default_rng(proj)
+    rows = rng.integers(0, 300, size=(20_000, 2)).astype(np.int32)
+    rows[:50, 1] = rows[:50, 0]                                   # self-loops give diagonal rows
+    rel, probe = dlg.pack(rows), dlg.pack(rows)
+    with ctx.hash_index(rel) as hx:
+        plain = hx.join(probe, proj - 2)
+        got = hx.join(probe, proj)
+    diag = (plain >> np.uint64(32)) == (plain & np.uint64(0xffffffff))
+    assert diag.any()
+    assert np.array_equal(got, plain[~diag])
+    want = numpy_join(rel, probe, proj - 2)
+    want = want[(want >> np.uint64(32)) != (want & np.uint64(0xffffffff))]
+    assert np.array_equal(np.sort(got), want)
+
+
+def symmetric_edges(rows: np.ndarray) -> np.ndarray:
+    """E u E^-1, sorted and deduplicated (wcc.cu:140-176)."""
+    return np.unique(np.concatenate([dlg.pack(rows), dlg.pack(rows, reverse=True)]))
+
+
+def numpy_wcc_round(E: np.ndarray, delta: np.ndarray, cc: np.ndarray):
+    """The reference's round on sorted tuples: J = delta join E on the node, cc' = per-node least
+    label of cc u J, delta' = cc' minus cc."""
+    J = numpy_join(E, delta, dlg.PROJ_HV_PV)                       # (m, l)
+    allt = np.sort(np.concatenate([cc, J]))
+    keys = allt >> np.uint64(32)
+    first = np.ones(len(allt), bool)
+    first[1:] = keys[1:] != keys[:-1]
+    new_cc = allt[first]
+    return new_cc, np.setdiff1d(new_cc, cc)
+
+
+def test_wcc_relax_round_matches_numpy(ctx):
+    rng = np.random.default_rng(11)
+    rows = rng.integers(-500, 500, size=(3000, 2)).astype(np.int32)
+    E = symmetric_edges(rows)
+    nodes = np.unique(E >> np.uint64(32))
+    # labels: any node id not above the node's own (as the loop keeps them), Δ a random subset
+    lab = np.minimum(nodes, rng.choice(nodes, size=len(nodes)))
+    cc = (nodes << np.uint64(32)) | (lab & np.uint64(0xffffffff))
+    delta = cc[rng.random(len(cc)) < 0.3]
+    with ctx.hash_index(E) as hx:
+        got_cc, got_d = hx.wcc_relax(delta, cc)
+    want_cc, want_d = numpy_wcc_round(E, delta, cc)
+    assert np.array_equal(got_cc, want_cc)
+    assert np.array_equal(got_d, want_d)
+
+
+@pytest.mark.parametrize("name", ["paper.bin", "dummy.bin", "data_7035.bin", "data_214078.bin"])
+def test_wcc_relax_drives_the_reference_loop(ctx, orc, name):
+    """wcc.cu:160-383 written against the C ABI roles (dlg_hash_build of E u E^-1, then
+    dlg_wcc_relax per round, stop when two consecutive |Δ| are equal) gives the oracle's
+    iterations, components and largest component."""
+    from conftest import read_bin
+    rows = read_bin(name)
+    E = symmetric_edges(rows)
+    nodes = np.unique(E >> np.uint64(32))
+    cc = (nodes << np.uint64(32)) | nodes
+    delta, prev, it = cc.copy(), len(cc), 0
+    with ctx.hash_index(E) as hx:
+        while True:
+            cc, delta = hx.wcc_relax(delta, cc)
+            it += 1
+            if len(delta) == prev:
+                break
+            prev = len(delta)
+    labels, counts = np.unique(cc & np.uint64(0xffffffff), return_counts=True)
+    want = orc.wcc(rows)
+    assert it == want["iterations"]
+    assert len(labels) == want["output_size"]
+    assert int(counts.max()) == want["secondary"]
+    assert np.array_equal(dlg.unpack(labels << np.uint64(32) | np.uint64(0x80000000)), want["rows"])
+
+
+def test_wcc_relax_rejects_bad_input(ctx):
+    rows = np.array([[1, 2], [2, 3]], np.int32)
+    E = np.unique(dlg.pack(rows))                                   # not symmetric: 3 is no key
+    nodes = np.unique(E >> np.uint64(32))
+    cc = (nodes << np.uint64(32)) | nodes
+    with ctx.hash_index(E) as hx:
+        with pytest.raises(dlg.DlgError) as e:
+            hx.wcc_relax(cc, cc)
+        assert e.value.status == dlg.DLG_INVALID_INPUT
+        with pytest.raises(dlg.DlgError) as e:
+            hx.wcc_relax(cc, cc[:1])                                # n_cc != key count
+        assert e.value.status == dlg.DLG_INVALID_INPUT
