@@ -254,7 +254,9 @@ void dlg_result_free(dlg_result *res);
  * plain run (an instance's iteration count is the first round in which it
  * derived nothing; WCC instances stop by their own termination rule and are
  * then frozen).  The summed id ranges of one batch must fit 2^32 ids
- * (DLG_OVERFLOW otherwise: split the batch).  results[i].raw_join_rows is -1. */
+ * (DLG_OVERFLOW otherwise: split the batch).  results[i].raw_join_rows is -1.
+ * With check_output, results[i].output_sum / output_order_breaks are the
+ * instance's own digest (its rows in its own ids), equal to its plain run's. */
 typedef enum dlg_prog { DLG_PROG_TC = 0, DLG_PROG_SG = 1, DLG_PROG_WCC = 2, DLG_PROG_SJ = 3 } dlg_prog;
 typedef struct dlg_batch_stats {
     double device_seconds;   /* upload-to-resident wall time of the whole batch      */
@@ -339,6 +341,14 @@ dlg_status dlg_write_rows(const char *path, const int32_t *h_rows, int64_t n);
  * srand(1); v[i] = rand() % range + 1 for i < 2*rows. */
 dlg_status dlg_generate_rand_rows(int64_t rows, int32_t range, int32_t **h_rows);
 void dlg_host_free(void *p);
+
+/* No counterpart in the reference.  |TC| of the rows (src, dst) before
+ * running it, for admission by predicted output and for output caps, on the
+ * host: SCC condensation (Tarjan), then per component C the nodes reachable
+ * from it, |TC| = sum over C of |C| * |reach(C)|.  Exact (*exact = 1) when the
+ * condensation has at most 16,384 components; otherwise estimated from the
+ * reach of up to 4,096 components drawn by size (*exact = 0).  `exact` may be NULL. */
+dlg_status dlg_predict_tc_size(const int32_t *h_rows, int64_t n, int64_t *tc_size, int32_t *exact);
 
 /* Synthetic workloads (deterministic: SplitMix64-seeded PCG32, seed per
  * instance).  All return malloc'd int32 rows freed with dlg_host_free. */

@@ -118,6 +118,7 @@ def lib() -> ctypes.CDLL:
         "dlg_write_rows": (c_int, [c_char_p, vp, i64]),
         "dlg_generate_rand_rows": (c_int, [i64, c_int32, POINTER(POINTER(c_int32))]),
         "dlg_host_free": (None, [vp]),
+        "dlg_predict_tc_size": (c_int, [vp, i64, POINTER(c_int64), POINTER(c_int32)]),
         "dlg_gen_powerlaw": (c_int, [c_int32, i64, c_double, c_uint64, POINTER(POINTER(c_int32)), POINTER(c_int64)]),
         "dlg_gen_grid": (c_int, [c_int32, c_double, c_double, c_int32, c_uint64, POINTER(POINTER(c_int32)), POINTER(c_int64)]),
         "dlg_gen_tetmesh": (c_int, [c_int32, POINTER(POINTER(c_int32)), POINTER(c_int64)]),
@@ -188,6 +189,16 @@ def _gen(fn, *args) -> np.ndarray:
     if s != DLG_OK:
         raise DlgError(s, fn.__name__)
     return _host_rows(p, n.value)
+
+
+def predict_tc_size(rows: np.ndarray):
+    """|TC| of the rows by SCC condensation (host only): (size, exact)."""
+    rows = np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1, 2))
+    k, ex = c_int64(), c_int32()
+    s = lib().dlg_predict_tc_size(rows.ctypes.data, rows.shape[0], byref(k), byref(ex))
+    if s != DLG_OK:
+        raise DlgError(s, "predict_tc_size")
+    return k.value, bool(ex.value)
 
 
 def gen_powerlaw(n: int, m: int, alpha: float, seed: int) -> np.ndarray:
@@ -404,7 +415,8 @@ class Ctx:
             for i in range(B):
                 r = res[i]
                 out.append({"iterations": r.iterations, "output_size": r.output_size, "secondary": r.secondary,
-                            "nodes": r.nodes, "rows": _rows_from(r.rows, r.n_rows) if r.rows else None})
+                            "nodes": r.nodes, "rows": _rows_from(r.rows, r.n_rows) if r.rows else None,
+                            "output_sum": r.output_sum, "output_order_breaks": r.output_order_breaks})
         finally:
             for i in range(B):
                 self.L.dlg_result_free(byref(res[i]))

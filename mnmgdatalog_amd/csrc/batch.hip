@@ -265,6 +265,13 @@ struct BatchHook : RoundHook {
     }
 };
 
+static void instance_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u32 sub0, u32 sub1, dlg_result *r) {
+    u64 sum = 0, breaks = 0;
+    rows_digest(ctx, d, n, orient, &sum, &breaks, sub0, sub1);
+    r->output_sum = sum;
+    r->output_order_breaks = (int64_t)breaks;
+}
+
 static void split_rows(const std::vector<int32_t> &all, const std::vector<u64> &b, const Layout &L,
                        bool keep, dlg_result *res) {
     for (int i = 0; i < L.B; i++) {
@@ -338,6 +345,12 @@ void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const 
         split_rows(all, b, L, o.keep, res);
         for (int i = 0; i < B; i++)
             res[i].iterations = prog == DLG_PROG_SJ ? 1 : (hook.iters[i] ? hook.iters[i] : rounds);
+        if (o.check) {   // each instance's digest over its own ids
+            const int orient = prog == DLG_PROG_SG ? DLG_ORIENT_KV : DLG_ORIENT_VK;
+            for (int i = 0; i < B; i++)
+                instance_digest(ctx, f.full.p + b[i], b[i + 1] - b[i], orient, (u32)L.shift[i], (u32)L.shift[i],
+                                &res[i]);
+        }
     } else if (prog == DLG_PROG_WCC) {
         WccGraph g(ctx);
         wcc_graph_build(ctx, d_rows.p, N, o, pc, g);
@@ -432,6 +445,9 @@ void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const 
         if (o.keep) all.resize(2 * ncomp + 2);
         if (o.keep && ncomp) download_rows(ctx, out.p, ncomp, DLG_ORIENT_KV, all.data());
         split_rows(all, b, L, o.keep, res);
+        if (o.check)   // (label, 0): only the label column is shifted
+            for (int i = 0; i < B; i++)
+                instance_digest(ctx, out.p + b[i], b[i + 1] - b[i], DLG_ORIENT_KV, (u32)L.shift[i], 0u, &res[i]);
         for (int i = 0; i < B; i++) {
             res[i].iterations = iters[i];
             res[i].secondary = res[i].output_size ? (i64)hl[i] : -1;

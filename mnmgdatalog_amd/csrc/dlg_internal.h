@@ -283,7 +283,10 @@ void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits =
                 const u32 *bounds = nullptr);
 // Digest of tuples as output rows in `orient` (dlg_result.output_sum) and the
 // number of i with d[i] >= d[i + 1] (synchronises).
-void rows_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u64 *sum, u64 *breaks);
+// sub0 / sub1 are subtracted (mod 2^32) from the rows' two columns first: the
+// batched form's id shift of an instance.
+void rows_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u64 *sum, u64 *breaks, u32 sub0 = 0,
+                 u32 sub1 = 0);
 // Ranges of the two u64 halves over data[0..n): out = {min_hi, max_hi, min_lo, max_lo}.
 void tuple_bounds(dlg_ctx *ctx, const u64 *data, u64 n, u32 *out);
 // Compaction modes.

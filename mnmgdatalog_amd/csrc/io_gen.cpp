@@ -73,21 +73,98 @@ inline uint64_t mix64(uint64_t x) {
     return Rng::splitmix(s);
 }
 
+int host_threads(size_t n) {
+    unsigned hw = std::thread::hardware_concurrency();
+    size_t t = std::min<size_t>(hw ? hw : 1, 16);
+    return (int)std::max<size_t>(1, std::min(t, n / (1u << 20)));
+}
+
+// Runs f(t, lo, hi) over T equal slices of [0, n) on T threads.
+template <typename F>
+void parallel_slices(size_t n, int T, F f) {
+    if (T <= 1) {
+        f(0, (size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++) th.emplace_back([&, t] { f(t, n * t / T, n * (t + 1) / T); });
+    for (auto &x : th) x.join();
+}
+
+// Sort of u64 keys: LSD radix with 8-bit digits on up to 16 host threads
+// (per-thread digit counts, a prefix over (digit, thread), stable scatter);
+// passes whose digit is the same for every key are skipped.
+void sort_u64(std::vector<uint64_t> &e) {
+    const size_t n = e.size();
+    const int T = host_threads(n);
+    if (T <= 1) {
+        std::sort(e.begin(), e.end());
+        return;
+    }
+    uint64_t vary = 0;
+    {
+        std::vector<uint64_t> o(T, 0), a(T, ~0ull);
+        parallel_slices(n, T, [&](int t, size_t lo, size_t hi) {
+            uint64_t x = 0, y = ~0ull;
+            for (size_t i = lo; i < hi; i++) {
+                x |= e[i];
+                y &= e[i];
+            }
+            o[t] = x;
+            a[t] = y;
+        });
+        uint64_t x = 0, y = ~0ull;
+        for (int t = 0; t < T; t++) {
+            x |= o[t];
+            y &= a[t];
+        }
+        vary = x ^ y;   // bits that differ somewhere
+    }
+    std::vector<uint64_t> tmp(n);
+    std::vector<size_t> cnt((size_t)T * 256);
+    for (int shift = 0; shift < 64; shift += 8) {
+        if (((vary >> shift) & 0xff) == 0) continue;
+        std::fill(cnt.begin(), cnt.end(), 0);
+        parallel_slices(n, T, [&](int t, size_t lo, size_t hi) {
+            size_t *c = &cnt[(size_t)t * 256];
+            for (size_t i = lo; i < hi; i++) c[(e[i] >> shift) & 0xff]++;
+        });
+        size_t sum = 0;
+        for (int d = 0; d < 256; d++)
+            for (int t = 0; t < T; t++) {
+                size_t c = cnt[(size_t)t * 256 + d];
+                cnt[(size_t)t * 256 + d] = sum;
+                sum += c;
+            }
+        parallel_slices(n, T, [&](int t, size_t lo, size_t hi) {
+            size_t *c = &cnt[(size_t)t * 256];
+            for (size_t i = lo; i < hi; i++) tmp[c[(e[i] >> shift) & 0xff]++] = e[i];
+        });
+        e.swap(tmp);
+    }
+}
+
 // Keep exactly m distinct edges out of a larger sorted-unique set: the m with
-// the smallest seeded hash, returned sorted.
+// the smallest seeded hash (a bijection of the edge, so no ties), kept in
+// their sorted order.
 void trim_to(std::vector<uint64_t> &e, size_t m, uint64_t seed) {
     if (e.size() <= m) return;
-    std::vector<std::pair<uint64_t, uint64_t>> h(e.size());
-    for (size_t i = 0; i < e.size(); i++) h[i] = {mix64(e[i] ^ (seed * 0x2545f4914f6cdd1dull)), e[i]};
-    std::nth_element(h.begin(), h.begin() + (ptrdiff_t)m, h.end());
-    h.resize(m);
-    e.resize(m);
-    for (size_t i = 0; i < m; i++) e[i] = h[i].second;
-    std::sort(e.begin(), e.end());
+    const uint64_t salt = seed * 0x2545f4914f6cdd1dull;
+    std::vector<uint64_t> h(e.size());
+    parallel_slices(e.size(), host_threads(e.size()), [&](int, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; i++) h[i] = mix64(e[i] ^ salt);
+    });
+    std::vector<uint64_t> sel(h);
+    std::nth_element(sel.begin(), sel.begin() + (ptrdiff_t)(m - 1), sel.end());
+    const uint64_t cut = sel[m - 1];   // the m-th smallest hash
+    size_t k = 0;
+    for (size_t i = 0; i < e.size(); i++)
+        if (h[i] <= cut) e[k++] = e[i];
+    e.resize(k);
 }
 
 void sort_unique(std::vector<uint64_t> &e) {
-    std::sort(e.begin(), e.end());
+    sort_u64(e);
     e.erase(std::unique(e.begin(), e.end()), e.end());
 }
 

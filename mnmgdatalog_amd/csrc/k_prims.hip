@@ -442,14 +442,15 @@ void tuple_bounds(dlg_ctx *ctx, const u64 *data, u64 n, u32 *h) {
 
 // Per block: the digest sum and the order breaks of a grid-stride share,
 // reduced through the wave and LDS, then one atomic each.
-__global__ __launch_bounds__(256) void rows_digest_k(const u64 *d, u64 n, int orient, unsigned long long *acc) {
+__global__ __launch_bounds__(256) void rows_digest_k(const u64 *d, u64 n, int orient, u32 sub0, u32 sub1,
+                                                     unsigned long long *acc) {
     __shared__ u64 s_sum[256 / WAVE], s_brk[256 / WAVE];
     u64 sum = 0, brk = 0;
     u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         u64 t = d[i];
-        u64 k = (u64)(tkey(t) ^ 0x80000000u), v = (u64)(tval(t) ^ 0x80000000u);   // the int32 bits
-        u64 r0 = orient == DLG_ORIENT_KV ? k : v, r1 = orient == DLG_ORIENT_KV ? v : k;
+        u32 k = tkey(t) ^ 0x80000000u, v = tval(t) ^ 0x80000000u;   // the int32 bits
+        u64 r0 = (u32)((orient == DLG_ORIENT_KV ? k : v) - sub0), r1 = (u32)((orient == DLG_ORIENT_KV ? v : k) - sub1);
         sum += r0 * 0x9E3779B97F4A7C15ull + r1 * 0xC2B2AE3D27D4EB4Full + r0 * r1 + 0x632BE59BD9B4E019ull;
         if (i + 1 < n && t >= d[i + 1]) brk++;
     }
@@ -474,12 +475,12 @@ __global__ __launch_bounds__(256) void rows_digest_k(const u64 *d, u64 n, int or
     }
 }
 
-void rows_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u64 *sum, u64 *breaks) {
+void rows_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u64 *sum, u64 *breaks, u32 sub0, u32 sub1) {
     DBuf<u64> acc(ctx, 2);
     fill_u64(ctx, acc.p, 2, 0ull);
     if (n) {
         unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 8);
-        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, rows_digest_k, g, 256, 0, d, n, orient,
+        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, rows_digest_k, g, 256, 0, d, n, orient, sub0, sub1,
                    (unsigned long long *)acc.p);
     }
     u64 h[2];

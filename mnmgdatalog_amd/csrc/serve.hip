@@ -17,6 +17,7 @@
 // Reservations are enforced: a slot's pool may hold at most its reservation
 // plus `retain_bytes` of cache.  A request that outgrows it is re-queued at
 // the front, first to run without batching, then alone with the whole budget.
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <memory>
@@ -76,7 +77,18 @@ namespace {
 
 double now_s(const dlg_server *s) { return secs(s->t0, Clock::now()); }
 
-// Predicted peak device bytes of one request.
+// Predicted peak device bytes of one request, by the layout its run will
+// use.  TC and SG keep the full relation as a bitmap when the id range W is
+// at most BITMAP_MAX_W wide: F and its transposed copy for the read-out,
+// W * ceil(W / 32) words each (SG: F, the round's maps D and R), plus the
+// sorted output and the round buffers, about 32 B per output tuple, plus
+// the edge index.  Wider ranges keep F as sorted tuples with a hash set and
+// a final sort: about 48 B per output tuple.  The output is |TC| by SCC
+// condensation (dlg_predict_tc_size, estimates padded by a quarter), or the
+// bound nodes^2 (SG; TC when even the bound fits or the request has more than
+// PREDICT_MAX_ROWS rows), with nodes bounded by the id range and twice the rows.
+constexpr int64_t PREDICT_MAX_ROWS = 4000000;
+
 size_t predict(const dlg_server *s, int prog, const int32_t *rows, int64_t n, int graph) {
     const size_t slack = 32ull << 20;
     if (graph >= 0) return 64 * (size_t)s->graphs[graph].g->V + slack;
@@ -85,13 +97,27 @@ size_t predict(const dlg_server *s, int prog, const int32_t *rows, int64_t n, in
         lo = std::min<int64_t>(lo, rows[i]);
         hi = std::max<int64_t>(hi, rows[i]);
     }
-    double nodes = n ? (double)std::min<int64_t>(2 * n, hi - lo + 1) : 0.0;
+    const double W = n ? (double)(hi - lo + 1) : 0.0;
+    const double nodes = std::min(2.0 * (double)n, W);
     double b;
     switch (prog) {
         case DLG_PROG_TC:
-        case DLG_PROG_SG:   // F <= nodes^2 tuples; runs, one merge copy, J and sort scratch
-            b = 32.0 * nodes * nodes + 160.0 * (double)n;
+        case DLG_PROG_SG: {
+            const bool bitmap = (s->opts.strategy == 0 && W <= (double)BITMAP_MAX_W) ||
+                                (s->opts.strategy == 4 && W <= (double)BITMAP_FORCED_MAX_W);
+            const double maps = bitmap ? (prog == DLG_PROG_SG ? 3.0 : 2.0) * W * std::ceil(W / 32.0) * 4.0 : 0.0;
+            const double per_tuple = bitmap ? 32.0 : 48.0;
+            double out = nodes * nodes;
+            // the condensation costs O(rows log rows) host time: up to PREDICT_MAX_ROWS rows
+            if (prog == DLG_PROG_TC && n <= PREDICT_MAX_ROWS &&
+                maps + per_tuple * out + 160.0 * (double)n > (double)(256ull << 20)) {
+                int64_t k = 0;
+                int32_t exact = 0;
+                if (dlg_predict_tc_size(rows, n, &k, &exact) == DLG_OK) out = (double)k * (exact ? 1.0 : 1.25);
+            }
+            b = maps + per_tuple * out + 160.0 * (double)n;
             break;
+        }
         case DLG_PROG_WCC:
             b = 200.0 * (double)n + 64.0 * nodes;
             break;

@@ -78,6 +78,18 @@ def test_batch_wcc_stop_rule_per_instance(ctx, orc):
     check(res[2], orc.wcc(inst[2]), secondary=True)
 
 
+@pytest.mark.parametrize("prog", ["tc", "sg", "wcc", "single_join"])
+def test_batch_digests_equal_plain(ctx, prog):
+    """check_output in the batched form: each instance's digest over its own ids equals its
+    plain run's (the per-instance row check of runs too large to download)."""
+    inst = instances(13, 30)
+    res, _ = ctx.batch_run(prog, inst, keep_output=False, check_output=1)
+    for r, rows in zip(res, inst):
+        p = ctx.run(prog, rows, keep_output=False, check_output=1)
+        assert (r["output_sum"], r["output_order_breaks"]) == (p["output_sum"], p["output_order_breaks"])
+        assert r["output_size"] == p["output_size"] and r["output_order_breaks"] == 0
+
+
 def test_batch_keep_output_off(ctx):
     inst = instances(3, 10)
     res, _ = ctx.batch_run("tc", inst, keep_output=False)

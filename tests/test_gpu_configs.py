@@ -1,0 +1,127 @@
+"""The benchmark configurations at their stated shapes, checked against the CPU oracle
+(or fixtures the oracle wrote):
+
+* batched TC, 512 instances from the C4 generators (edges log-uniform from 1,000
+  up to a bound set by predicted output): every instance's rows and iterations;
+* the served mix's resident R-MAT-23 graph: WCC alone and queried through a server,
+  against tests/golden/rmat23_wcc.json;
+* a served mix of TC / WCC requests: every result against the oracle (and its plain run);
+* SG on the 35 x 35 x 35 tetrahedral mesh: the first rounds against the oracle.
+"""
+import json
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import mnmgdatalog_amd as dlg
+from conftest import ROOT
+from mnmgdatalog_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+P, Q, K = 0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F, 0x632BE59BD9B4E019
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dlg.Ctx(0)
+    yield c
+    c.close()
+
+
+def rows_digest(rows: np.ndarray) -> int:
+    r0 = rows[:, 0].astype(np.int64).view(np.uint64) & np.uint64(0xffffffff)
+    r1 = rows[:, 1].astype(np.int64).view(np.uint64) & np.uint64(0xffffffff)
+    with np.errstate(over="ignore"):
+        return int(np.sum(r0 * np.uint64(P) + r1 * np.uint64(Q) + r0 * r1 + np.uint64(K), dtype=np.uint64))
+
+
+def oracle_all(orc, prog, instances, threads=16):
+    fn = getattr(orc, prog)
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(fn, instances))
+
+
+def test_c4_batched_512_instances_match_oracle(ctx, orc):
+    inst, info = W.c4_instances(512, output_cap=2e8)
+    print(f"c4: {len(inst)} instances, edges 1000..{info['max_edges']}, "
+          f"predicted {info['predicted_output']} tuples", flush=True)
+    assert len(inst) == 512 and info["max_edges"] > 1000
+    res, st = ctx.batch_run("tc", inst)
+    want = oracle_all(orc, "tc", inst)
+    for i, (r, w) in enumerate(zip(res, want)):
+        assert (r["iterations"], r["output_size"]) == (w["iterations"], w["output_size"]), i
+        assert np.array_equal(r["rows"], w["rows"]), i
+    assert st["total_output"] == sum(w["output_size"] for w in want)
+    assert st["rounds"] == max(w["iterations"] for w in want)
+
+
+@pytest.fixture(scope="module")
+def rmat23():
+    with open(os.path.join(ROOT, "tests", "golden", "rmat23_wcc.json")) as f:
+        fix = json.load(f)
+    rows = dlg.gen_rmat(23, 16 << 23, 0.57, 0.19, 0.19, W.SEED + 7, undirected=True)
+    assert len(rows) == fix["rows"]
+    return rows, fix
+
+
+def test_c5_resident_rmat23_wcc_matches_fixture(ctx, rmat23):
+    rows, fix = rmat23
+    for strategy in (0, 1):      # graph-captured rounds, then rounds launched one by one
+        r = ctx.run("wcc", rows, keep_output=False, check_output=1, strategy=strategy)
+        assert (r["iterations"], r["output_size"], r["secondary"], r["nodes"]) == \
+            (fix["iterations"], fix["components"], fix["largest"], fix["nodes"])
+        assert r["output_sum"] == fix["output_sum"] and r["output_order_breaks"] == 0
+
+
+def test_c5_resident_rmat23_served_queries_match_fixture(rmat23):
+    rows, fix = rmat23
+    small = W.served_requests(24, 2_000, 1.0, 20_000, W.SEED + 5)
+    with dlg.Server(0, workers=4) as srv:
+        gid = srv.add_graph(rows)
+        tickets = [srv.submit("wcc", graph=gid)] + [srv.submit(p, r) for p, r in small] + \
+            [srv.submit("wcc", graph=gid)]
+        got = [srv.wait(t) for t in tickets]
+    for g in (got[0], got[-1]):
+        assert g["status"] == dlg.DLG_OK
+        assert (g["iterations"], g["output_size"], g["secondary"], g["nodes"]) == \
+            (fix["iterations"], fix["components"], fix["largest"], fix["nodes"])
+        assert rows_digest(g["rows"]) == fix["output_sum"]
+
+
+def test_c5_served_mix_matches_oracle(ctx, orc):
+    """Every request of a Poisson-free served mix (all submitted at once, so they overlap and
+    batch) against the oracle, and against its plain run."""
+    reqs = W.served_requests(96, 3_000, 1.5, 60_000, W.SEED + 9)
+    with dlg.Server(0, workers=4) as srv:
+        tickets = [srv.submit(p, r) for p, r in reqs]
+        got = [srv.wait(t) for t in tickets]
+    with ThreadPoolExecutor(16) as ex:
+        want = list(ex.map(lambda pr: getattr(orc, pr[0])(pr[1]), reqs))
+    assert max(g["batch_size"] for g in got) > 1
+    for (prog, rows), g, w in zip(reqs, got, want):
+        assert g["status"] == dlg.DLG_OK
+        assert (g["iterations"], g["output_size"]) == (w["iterations"], w["output_size"])
+        if prog == "wcc":
+            assert g["secondary"] == w["secondary"]
+        assert np.array_equal(g["rows"], w["rows"])
+    for (prog, rows), g in list(zip(reqs, got))[:12]:
+        alone = ctx.run(prog, rows)
+        assert np.array_equal(g["rows"], alone["rows"]) and g["iterations"] == alone["iterations"]
+
+
+@pytest.mark.parametrize("k", [4])
+def test_c3_tetmesh_35_first_rounds_match_oracle(ctx, orc, k):
+    """SG on the full 35^3-cell mesh (268,380 edges): the relation after the base rule and k
+    rounds equals the oracle's.  The whole fixpoint (52 rounds, 2.08e9 tuples) is beyond
+    what the oracle finishes in a test; it is checked for order and size consistency below."""
+    rows = dlg.gen_tetmesh(35)
+    assert len(rows) == 268_380
+    got = ctx.run("sg", rows, max_iterations=k)
+    want = orc.sg(rows, threads=16, max_iterations=k)
+    assert (got["iterations"], got["output_size"]) == (want["iterations"], want["output_size"])
+    assert np.array_equal(got["rows"], want["rows"])
+    full = ctx.run("sg", rows, keep_output=False, check_output=1)
+    assert full["output_order_breaks"] == 0 and full["output_size"] > want["output_size"]

@@ -66,16 +66,26 @@ def test_served_results_equal_alone(ctx):
         assert g["submit_s"] <= g["start_s"] <= g["end_s"]
 
 
-def test_served_refuses_what_cannot_fit():
+def test_served_refuses_what_cannot_fit(orc):
+    """Admission by predicted output: an Erdos-Renyi graph of mean out-degree 4 on 300,000
+    vertices closes over ~8.6e10 tuples (SCC condensation) and is refused for TC, not for WCC;
+    400,000 random edges over the whole id range close over few tuples and run (the bound
+    nodes^2 alone would have refused them)."""
+    huge = dlg.gen_erdos_renyi(300_000, 1_200_000, 5)
+    assert dlg.predict_tc_size(huge)[0] > 5e10
     rng = np.random.default_rng(1)
-    big = rng.integers(0, 2**31 - 1, size=(400_000, 2)).astype(np.int32)   # predicted closure far too large
+    sparse = rng.integers(0, 2**31 - 1, size=(400_000, 2)).astype(np.int32)
     with dlg.Server(0, workers=2) as srv:
         with pytest.raises(dlg.DlgError) as e:
-            srv.submit("tc", big)
+            srv.submit("tc", huge)
         assert e.value.status == dlg.DLG_REFUSED
-        t = srv.submit("wcc", big)                                   # the same rows fit for WCC
-        r = srv.wait(t)
-        assert r["status"] == dlg.DLG_OK and r["nodes"] > 0
+        t = srv.submit("wcc", huge)                                  # the same rows fit for WCC
+        ts = srv.submit("tc", sparse)
+        r, rs = srv.wait(t), srv.wait(ts)
+    assert r["status"] == dlg.DLG_OK and r["nodes"] > 0
+    want = orc.tc(sparse)
+    assert rs["status"] == dlg.DLG_OK and rs["output_size"] == want["output_size"]
+    assert np.array_equal(rs["rows"], want["rows"])
 
 
 def test_served_wait_timeout_and_unknown_ticket():

@@ -230,3 +230,42 @@ def test_format_tool_errors(tmp_path):
 def test_row_size():
     p = tool("row_size", os.path.join(DATA, "data_7035.bin"))
     assert p.returncode == 0 and p.stdout.strip() == f"{os.path.join(DATA, 'data_7035.bin')}: 7035"
+
+
+@pytest.mark.parametrize("name,want", [("paper.bin", 18), ("dummy.bin", 13), ("hipc_2019.bin", 9),
+                                       ("data_22.bin", 253), ("data_7035.bin", 146120)])
+def test_predict_tc_size_exact_on_fixtures(name, want):
+    """SCC-condensation count of |TC| equals the README / oracle counts (exact below 16,384
+    components)."""
+    from conftest import read_bin
+    assert dlg.predict_tc_size(read_bin(name)) == (want, True)
+
+
+def test_predict_tc_size_closed_forms_and_random(orc):
+    n = 300
+    chain = np.array([(i, i + 1) for i in range(n)], np.int32)
+    assert dlg.predict_tc_size(chain) == (n * (n + 1) // 2, True)           # path: no self pairs
+    cycle = np.concatenate([chain, [[n, 0]]]).astype(np.int32)
+    assert dlg.predict_tc_size(cycle) == ((n + 1) ** 2, True)               # one SCC reaches itself
+    loops = np.array([[5, 5], [5, 6], [7, 8]], np.int32)
+    assert dlg.predict_tc_size(loops) == (3, True)                          # (5,5), (5,6), (7,8)
+    assert dlg.predict_tc_size(np.zeros((0, 2), np.int32)) == (0, True)
+    rng = np.random.default_rng(3)
+    for seed in range(4):
+        rows = rng.integers(-400, 400, size=(int(rng.integers(100, 1500)), 2)).astype(np.int32)
+        got, exact = dlg.predict_tc_size(rows)
+        assert exact and got == orc.tc(rows)["output_size"]
+    for seed in range(2):
+        rows = dlg.gen_erdos_renyi(3000, 7000, seed)
+        assert dlg.predict_tc_size(rows) == (orc.tc(rows, threads=8)["output_size"], True)
+        rows = dlg.gen_rmat(11, 6000, 0.57, 0.19, 0.19, seed)
+        assert dlg.predict_tc_size(rows) == (orc.tc(rows, threads=8)["output_size"], True)
+
+
+def test_predict_tc_size_estimate_on_a_wide_graph():
+    """Above 16,384 components the count is estimated from sampled reach sets: SF.cedge's
+    published closure is 80,498,014 tuples (logs/small-merged-job-output.txt:108)."""
+    from conftest import read_bin
+    got, exact = dlg.predict_tc_size(read_bin("data_223001.bin"))
+    assert not exact
+    assert abs(got - 80_498_014) < 0.05 * 80_498_014
