@@ -372,6 +372,26 @@ dlg_status dlg_gen_erdos_renyi(int32_t n, int64_t m, uint64_t seed, int32_t **h_
 dlg_status dlg_gen_rmat(int32_t scale, int64_t m, double a, double b, double c, uint64_t seed,
                         int undirected, int32_t **h_rows, int64_t *n_rows);
 
+/* ------------------------------------------------- no-allocation join */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dlg_join with the output in a caller buffer and every
+ * scratch word (per-probe match counts and row starts, their scan and its
+ * partials) in one static device array of DLG_NOALLOC_SCRATCH_BYTES that each
+ * GPU the library runs on holds until the process ends.  The call makes no
+ * hipMalloc / hipFree and no other allocating library call, and records no
+ * profiling events.  Limits: at most DLG_JOIN_NOALLOC_MAX_PROBES probes
+ * (DLG_INVALID_INPUT beyond, with a message); projections DLG_PROJ_HV_PV and
+ * DLG_PROJ_PV_HV.  *out_n = |J|; when |J| > out_cap nothing is written and
+ * DLG_OVERFLOW is returned with *out_n the capacity needed.  All calls on one
+ * GPU share its array: calls that overlap on one GPU (several contexts or
+ * threads) race and must be serialised.  Synchronises the context's stream.
+ * Same rows in the same order as dlg_join. */
+#define DLG_NOALLOC_SCRATCH_BYTES (64ll << 20)
+#define DLG_JOIN_NOALLOC_MAX_PROBES 4000000ll
+dlg_status dlg_join_noalloc(dlg_ctx *ctx, const dlg_hash *h, const uint64_t *d_probe, int64_t n, int proj,
+                            uint64_t *d_out, int64_t out_cap, int64_t *out_n);
+
 #ifdef __cplusplus
 }
 #endif

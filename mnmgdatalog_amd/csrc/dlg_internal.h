@@ -34,6 +34,10 @@ struct Error {
 [[noreturn]] void fail(dlg_status s, const std::string &msg);
 void check_hip(hipError_t e, const char *what, const char *file, int line);
 hipEvent_t timing_event();   // an event for timing only (no system-scope fence)
+// Looks up, once per device and with that device current, the address of the
+// no-allocation entry points' static scratch array (noalloc.hip); called by
+// dlg_ctx_create.  A failure is kept and reported by those entry points.
+void noalloc_init(int device);
 #define HIPCHK(call) ::dlg::check_hip((call), #call, __FILE__, __LINE__)
 
 // ------------------------------------------------------------------ pool

@@ -221,6 +221,7 @@ dlg_status dlg_ctx_create(int device, dlg_ctx **out) {
                              hipHostMallocMapped));
         std::memset(c->h_mapped, 0, (dlg_ctx::MAPPED_SEQ + 1) * sizeof(uint64_t));   // no sequence number yet
         HIPCHK(hipHostGetDevicePointer((void **)&c->d_mapped, c->h_mapped, 0));
+        noalloc_init(device);
     } catch (const Error &e) {
         fprintf(stderr, "dlg_ctx_create: %s\n", e.msg.c_str());
         dlg_ctx_destroy(c);
