@@ -192,7 +192,13 @@ typedef struct dlg_opts {
                                 binary search per probe, TC's edges deduplicated first
                                 (tc_nl.cu, sg_nl.cu; common/join.cu:89-180).  Same
                                 relations and iteration counts either way             */
-    int32_t reserved[4];
+    int32_t reorder;         /* TC in the bitmap layout: renumber the ids into a
+                                locality order (recursive BFS bisection) before the
+                                fixpoint and map the result back at read-out, so a
+                                round's new ends of one source share words and cache
+                                lines of F.  0 auto (plain runs with id ranges of at
+                                least 16,384 ids), 1 never, 2 always.  Same output   */
+    int32_t reserved[3];
 } dlg_opts;
 
 /* Phase timers in the reference's CSV order (tc.cu:413-420). */

@@ -129,9 +129,10 @@ __global__ void comp_largest_k(const u32 *ccount, u64 V, const u32 *vstart, u32 
 }
 
 __global__ void comp_count_k(const u32 *L, u64 V, u32 *ccount) {
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    u64 stride = (u64)gridDim.x * blockDim.x;
-    for (; i < V; i += stride) atomicAdd(&ccount[L[i]], 1u);
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 lim = (V + WAVE - 1) / WAVE * WAVE;   // whole waves iterate together
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += stride)
+        wave_count_by_key(ccount, i < V ? L[i] : 0u, i < V);
 }
 
 __global__ void comp_flags_k(const u32 *ccount, u64 V, u32 *flags) {

@@ -208,6 +208,23 @@ __device__ __forceinline__ u64 lanemask_lt() {
     return (1ull << (threadIdx.x & (WAVE - 1))) - 1ull;
 }
 
+// cnt[key] += 1 for every active lane, one atomic per distinct key of the
+// wave: lanes with the first pending lane's key are counted together (a
+// label shared by most nodes, such as a giant component's, would otherwise
+// serialise one atomic per node on one address).  Every lane of the wave
+// must call it.
+__device__ __forceinline__ void wave_count_by_key(u32 *cnt, u32 key, bool active) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    u64 pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const u32 k = __shfl(key, leader, WAVE);
+        const u64 same = __ballot(active && key == k) & pending;
+        if (lane == leader) atomicAdd(&cnt[k], (u32)__popcll(same));
+        pending &= ~same;
+    }
+}
+
 // Block-wide exclusive scan (blockDim.x a multiple of 64, <= 1024).
 // `scratch` holds >= blockDim.x/64 + 1 elements of T in LDS.
 template <typename T>
@@ -366,6 +383,12 @@ u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symme
 // scratch copy (transposed back when `transposed`).  Returns its set bits.
 u64 bitmap_emit_block(dlg_ctx *ctx, const u32 *words, u32 lo, u64 W, u64 wpr, bool transposed, DBuf<u32> &scratch,
                       u64 *out);
+// The same for a transposed block whose ids were renumbered (row sigma(x),
+// column sigma(z) holds tuple (z, x)): rows gathered back into input order,
+// the map transposed, its rows gathered again, then emitted.  `words` is
+// clobbered.
+u64 bitmap_emit_block_renumbered(dlg_ctx *ctx, u32 *words, u32 lo, u64 W, u64 wpr, const u32 *sigma,
+                                 DBuf<u32> &scratch, u64 *out);
 
 // ------------------------------------------------------------- TC rows (k_tcrows.hip)
 // The full TC relation as a block-diagonal set of transposed bitmaps: one
@@ -422,6 +445,18 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr);
 u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw);
 // The current Δ as (z, x) tuples.
 void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out);
+// Locality order of the ids 0 .. W-1 of the edges (src[i], dst[i]) (order.cpp):
+// sigma[old] = new.
+std::vector<u32> locality_order(const std::vector<u32> &src, const std::vector<u32> &dst, u32 W);
+// out = the int32 rows with every id renumbered: key k -> sigma[k - lo] + lo
+// (k the id as a flipped key, so the id range is kept).
+void relabel_rows(dlg_ctx *ctx, const int32_t *in, u64 n, const u32 *sigma, u32 lo, int32_t *out);
+// rowmap_emit of a one-block map whose ids were renumbered by sigma (the map
+// holds row sigma(x), column sigma(z) for tuple (z, x)): the tuples in the
+// input's ids and order.  The map's words are clobbered.
+u64 rowmap_emit_renumbered(dlg_ctx *ctx, RowMap &m, DBuf<u64> &out, u64 expect, const u32 *sigma);
+// Fraction of the index's rows (k, v) whose reverse (v, k) is also a row.
+double reciprocal_fraction(dlg_ctx *ctx, const Index &ix);
 
 // ------------------------------------------------------------- SG row ORs (k_sgrows.hip)
 struct SgRows {

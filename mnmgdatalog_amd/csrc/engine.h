@@ -19,6 +19,7 @@ struct Opts {
     int block_size;  // 0 = the context's
     bool check;      // output digest
     int join;        // 0 hash index, 1 sorted index (binary search)
+    int reorder;     // TC bitmap layout: 0 auto, 1 never, 2 always renumber into a locality order
 };
 Opts read_opts(dlg_ctx *ctx, const dlg_opts *o);
 

@@ -43,7 +43,7 @@ void PhaseClock::finish(double *out) {
 }
 
 Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
-    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0, false, 0};
+    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0, false, 0, 0};
     if (o) {
         if (o->load_factor < 0 || o->load_factor > 0.95)
             fail(DLG_INVALID_INPUT, "load_factor out of (0, 0.95]");
@@ -62,6 +62,8 @@ Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
         if (o->join_method < 0 || o->join_method > 1) fail(DLG_INVALID_INPUT, "join_method out of 0..1");
         r.join = o->join_method;
         r.strategy = o->strategy;
+        if (o->reorder < 0 || o->reorder > 2) fail(DLG_INVALID_INPUT, "reorder out of 0..2");
+        r.reorder = o->reorder;
     }
     return r;
 }
@@ -267,24 +269,67 @@ static bool tc_use_rows(const Opts &o, u64 n, const u32 *ib, const std::vector<u
     return wmax <= BITMAP_MAX_W && bits <= BITMAP_MAX_W * BITMAP_MAX_W;
 }
 
+// The locality order of a plain TC run in the bitmap layout (order.cpp):
+// sigma (device, sigma[old - lo] = new - lo) from the rows, which are
+// downloaded for it, and the renumbered rows.
+static void tc_order(dlg_ctx *ctx, const int32_t *d_rows, u64 n, u32 lo, u64 W, DBuf<u32> &sigma,
+                     DBuf<int32_t> &rows) {
+    std::vector<int32_t> h(2 * n);
+    HIPCHK(hipMemcpyAsync(h.data(), d_rows, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    std::vector<u32> src(n), dst(n);
+    for (u64 i = 0; i < n; i++) {
+        src[i] = ((u32)h[2 * i] ^ 0x80000000u) - lo;
+        dst[i] = ((u32)h[2 * i + 1] ^ 0x80000000u) - lo;
+    }
+    std::vector<u32> sg = locality_order(src, dst, (u32)W);
+    sigma.ensure(W);
+    HIPCHK(hipMemcpyAsync(sigma.p, sg.data(), 4 * W, hipMemcpyHostToDevice, ctx->stream));
+    rows.ensure(2 * n);
+    relabel_rows(ctx, d_rows, n, sigma.p, lo, rows.p);
+    ctx->sync();   // the host vectors go out of scope
+}
+
+constexpr u64 ORDER_MIN_W = 1ull << 14;      // automatic locality order from this id range on,
+constexpr u64 ORDER_MAX_ROWS = 1ull << 24;   // up to this many edges (host time),
+constexpr double ORDER_MIN_RECIPROCAL = 0.9; // for (nearly) symmetric graphs: their closures are
+                                             // complete per component, so each source's rounds
+                                             // sweep rings through all of its component
+
 void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
                  RoundHook *hook, FixpointOut &out, const std::vector<u32> *blocks) {
     pc.mark(DLG_PH_INIT);
     DBuf<u64> edge(ctx, n), delta(ctx, n), J(ctx, n), N(ctx), tmp(ctx);
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, edge.p);   // e(src, dst) keyed by src
-    pack_rows(ctx, d_rows, n, DLG_ORIENT_VK, J.p);      // Δ0 rows (dst, src)
     u32 ib[4];
     id_bounds(ctx, edge.p, n, ib);
     pc.mark(DLG_PH_HASHTABLE);
     Index ix(ctx);
-    if (o.join == 1) {
-        // tc_nl.cu:182-185 sorts and deduplicates the edge relation it probes
-        DBuf<u64> ue(ctx);
-        u64 ne = sort_unique_into(ctx, edge, n, tmp, ue, SEL_UNIQUE, nullptr, 0, o.bits, ib);
-        index_build(ctx, ix, ue.p, ne, o.lf, true, true);
-    } else {
-        index_build(ctx, ix, edge.p, n, o.lf, false);  // duplicates kept, as tc.cu:210
+    auto build_index = [&]() {
+        if (o.join == 1) {
+            // tc_nl.cu:182-185 sorts and deduplicates the edge relation it probes
+            DBuf<u64> ue(ctx);
+            u64 ne = sort_unique_into(ctx, edge, n, tmp, ue, SEL_UNIQUE, nullptr, 0, o.bits, ib);
+            index_build(ctx, ix, ue.p, ne, o.lf, true, true);
+        } else {
+            index_build(ctx, ix, edge.p, n, o.lf, false);  // duplicates kept, as tc.cu:210
+        }
+    };
+    build_index();
+    // renumber into a locality order (order.cpp): plain runs in the bitmap layout
+    DBuf<u32> sigma(ctx);
+    DBuf<int32_t> reordered(ctx);
+    const u64 W = (u64)(ib[1] - ib[0]) + 1;
+    bool reorder = !blocks && !hook && o.reorder != 1 && tc_use_rows(o, n, ib, blocks);
+    if (reorder && o.reorder != 2)
+        reorder = W >= ORDER_MIN_W && n <= ORDER_MAX_ROWS && reciprocal_fraction(ctx, ix) >= ORDER_MIN_RECIPROCAL;
+    if (reorder) {
+        tc_order(ctx, d_rows, n, ib[0], W, sigma, reordered);
+        d_rows = reordered.p;
+        pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, edge.p);
+        build_index();
     }
+    pack_rows(ctx, d_rows, n, DLG_ORIENT_VK, J.p);      // Δ0 rows (dst, src)
     edge.release();
     JoinScratch js(ctx);
     int it = 0;
@@ -313,7 +358,13 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         pc.mark(DLG_PH_FINAL);
         out.raw += (i64)raw;
         out.nfull = total;
-        rowmap_emit(ctx, F, out.full, total);
+        if (reorder) {   // back to the input's ids
+            tr = TcRows(ctx);
+            ix = Index(ctx);
+            rowmap_emit_renumbered(ctx, F, out.full, total, sigma.p);
+        } else {
+            rowmap_emit(ctx, F, out.full, total);
+        }
         out.iterations = it;
         return;
     }

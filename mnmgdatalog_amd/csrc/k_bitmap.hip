@@ -674,6 +674,44 @@ u64 bitmap_emit_block(dlg_ctx *ctx, const u32 *words, u32 lo, u64 W, u64 wpr, bo
     return h;
 }
 
+// dst row r = src row sigma[r] (rows of wpr words): one workgroup per 8 rows, coalesced.
+__global__ __launch_bounds__(256) void row_gather_k(const u32 *src, u32 *dst, const u32 *sigma, u64 W, u64 wpr) {
+    const u64 r0 = (u64)blockIdx.x * 8;
+    for (u64 r = r0; r < r0 + 8 && r < W; r++) {
+        const u32 *s = src + (u64)sigma[r] * wpr;
+        u32 *d = dst + r * wpr;
+        for (u64 j = threadIdx.x; j < wpr; j += 256) d[j] = s[j];
+    }
+}
+
+u64 bitmap_emit_block_renumbered(dlg_ctx *ctx, u32 *words, u32 lo, u64 W, u64 wpr, const u32 *sigma,
+                                 DBuf<u32> &scratch, u64 *out) {
+    const u64 nwords = W * wpr;
+    if (nwords == 0) return 0;
+    const u64 padded = (nwords + BE_TILE - 1) / BE_TILE * BE_TILE;
+    scratch.ensure(padded);
+    fill_u32(ctx, scratch.p + nwords, padded - nwords, 0u);
+    // rows back to input order: row x = the map's row sigma(x) (columns still renumbered)
+    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, row_gather_k, grid_for(W, 8), 256, 0, (const u32 *)words, scratch.p,
+               sigma, W, wpr);
+    // transposed: row = renumbered end, columns = sources in input order
+    const u64 nt = (wpr + BT - 1) / BT;
+    if (wpr <= BT_SMALL_WPR)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, bm_transpose_small_k, grid_for(wpr * wpr * 32, 256), 256, 0,
+                   (const u32 *)scratch.p, words, W, wpr);
+    else
+        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, bm_transpose_k, (unsigned)(nt * nt), 256, 0, (const u32 *)scratch.p,
+                   words, W, wpr);
+    // rows back to input order: row z = row sigma(z)
+    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nwords, row_gather_k, grid_for(W, 8), 256, 0, (const u32 *)words, scratch.p,
+               sigma, W, wpr);
+    DBuf<u64> bsum(ctx);
+    u64 nb = 0;
+    u64 h = words_count(ctx, scratch.p, nwords, bsum, nb);
+    if (h) emit_pass<false, false>(ctx, lo, wpr, scratch.p, bsum, nb, h, out);
+    return h;
+}
+
 u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symmetric) {
     out.ensure(expect ? expect : 1);
     if (expect == 0) return 0;

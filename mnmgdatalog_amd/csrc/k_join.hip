@@ -280,6 +280,34 @@ void join_expand(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, const J
     join_expand_with(ctx, ix, probe, n, js.pstart.p, js.offs.p, total, proj, out);
 }
 
+// Rows whose reverse is a row too (binary search in the sorted rows),
+// counted per wave.
+__global__ void reciprocal_k(const u64 *rows, u64 n, unsigned long long *count) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 lim = (n + WAVE - 1) / WAVE * WAVE;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += stride) {
+        bool f = false;
+        if (i < n) {
+            const u64 t = rows[i], r = tmake(tval(t), tkey(t));
+            const u64 p = lower_bound_u64(rows, 0, n, r);
+            f = p < n && rows[p] == r;
+        }
+        const u64 m = __ballot(f);
+        if ((threadIdx.x & (WAVE - 1)) == 0 && m) atomicAdd(count, (unsigned long long)__popcll(m));
+    }
+}
+
+double reciprocal_fraction(dlg_ctx *ctx, const Index &ix) {
+    if (ix.n == 0) return 0.0;
+    DBuf<u64> c(ctx, 1);
+    fill_u64(ctx, c.p, 1, 0ull);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * ix.n, reciprocal_k, sgrid(ctx, ix.n), ctx->block_size, 0,
+               (const u64 *)ix.rows.p, ix.n, (unsigned long long *)c.p);
+    u64 h = 0;
+    ctx->read_scalars(c.p, 1, &h);
+    return (double)h / (double)ix.n;
+}
+
 __global__ void lookup_values_k(const u64 *rows, u64 n, const u64 *slots, u64 mask, u32 *out) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     u64 stride = (u64)gridDim.x * blockDim.x;

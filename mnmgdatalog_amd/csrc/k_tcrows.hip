@@ -808,6 +808,31 @@ u64 rowmap_emit(dlg_ctx *ctx, RowMap &m, DBuf<u64> &out, u64 expect) {
     return pos;
 }
 
+// ---------------------------------------------------------------- locality order
+__global__ void relabel_rows_k(const int32_t *in, u64 m, const u32 *sigma, u32 lo, int32_t *out) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const u32 k = (u32)in[i] ^ 0x80000000u;
+        out[i] = (int32_t)((sigma[k - lo] + lo) ^ 0x80000000u);
+    }
+}
+
+void relabel_rows(dlg_ctx *ctx, const int32_t *in, u64 n, const u32 *sigma, u32 lo, int32_t *out) {
+    if (n)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * n, relabel_rows_k, sgrid(ctx, 2 * n), ctx->block_size, 0, in, 2 * n, sigma,
+                   lo, out);
+}
+
+u64 rowmap_emit_renumbered(dlg_ctx *ctx, RowMap &m, DBuf<u64> &out, u64 expect, const u32 *sigma) {
+    if (m.blocks.size() != 1) fail(DLG_DEVICE_ERROR, "renumbered emission: a map of one block expected");
+    out.ensure(expect ? expect : 1);
+    DBuf<u32> scratch(ctx);
+    const auto &b = m.blocks[0];
+    const u64 pos = bitmap_emit_block_renumbered(ctx, m.words.p + b.base, b.lo, b.W, b.wpr, sigma, scratch, out.p);
+    if (pos != expect) fail(DLG_DEVICE_ERROR, "bitmap emission count mismatch");
+    return pos;
+}
+
 void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out) {
     out.ensure(tr.nd ? tr.nd : 1);
     if (!tr.nd) return;

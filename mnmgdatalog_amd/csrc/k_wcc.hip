@@ -93,9 +93,10 @@ __global__ void wcc_gather_k(const u32 *node, u64 n, const u32 *L, u32 *label) {
 }
 
 __global__ void wcc_comp_count_k(const u32 *L, u64 V, u32 *ccount) {
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    u64 stride = (u64)gridDim.x * blockDim.x;
-    for (; i < V; i += stride) atomicAdd(&ccount[L[i]], 1u);
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 lim = (V + WAVE - 1) / WAVE * WAVE;   // whole waves iterate together
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += stride)
+        wave_count_by_key(ccount, i < V ? L[i] : 0u, i < V);
 }
 
 // flags[v] = ccount[v] > 0; largest via atomicMax.

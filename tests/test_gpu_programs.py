@@ -442,3 +442,26 @@ def test_tc_grid_400_full_size():
     assert got["output_sum"] == int(fx["tc_output_sum"])
     assert got["output_order_breaks"] == 0
     assert got["iterations"] == fx["iterations"]
+
+
+# ------------------------------------------------------------------ locality order (TC bitmap layout)
+@pytest.mark.parametrize("name", ["data_7035.bin", "data_23874.bin", "data_22.bin", "paper.bin",
+                                  "hipc_2019.bin"])
+def test_tc_locality_order_forced_matches_oracle(ctx, orc, name):
+    """reorder = 2 renumbers the ids by recursive BFS bisection before the fixpoint and maps the
+    relation back at read-out: rows, size and iterations stay the oracle's."""
+    rows = read_bin(name)
+    got = ctx.run("tc", rows, reorder=2, strategy=4)
+    assert_same(got, orc.tc(rows))
+    assert got["raw_join_rows"] == ctx.run("tc", rows, reorder=1, strategy=4)["raw_join_rows"]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_tc_locality_order_random(ctx, orc, seed):
+    """Negative ids, self-loops, duplicates, isolated ids inside the range, several components."""
+    rng = np.random.default_rng(100 + seed)
+    parts = [rand_rows(seed, 300, 700, neg=True)]
+    parts.append(np.array([(j, j + 1) for j in range(2000, 2400)] + [(2400, 2000)], np.int32))
+    parts.append(np.array([[5000, 5000], [5000, 5001], [-7000, 5000]], np.int32))
+    rows = np.concatenate(parts)[rng.permutation(sum(len(p) for p in parts))]
+    assert_same(ctx.run("tc", rows, reorder=2, strategy=4), orc.tc(rows))
