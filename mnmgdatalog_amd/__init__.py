@@ -147,17 +147,40 @@ def lib() -> ctypes.CDLL:
     return L
 
 
-def _rows_from(ptr, n: int) -> np.ndarray:
-    if n == 0:
-        return np.zeros((0, 2), dtype=np.int32)
-    a = np.ctypeslib.as_array(ptr, shape=(2 * n,)).copy().reshape(n, 2)
-    return a
+class _HostBlock:
+    """Owner of a host block malloc'd by libdlg: freed with dlg_host_free when the
+    last numpy view of it goes away."""
+
+    def __init__(self, addr: int):
+        self.addr = addr
+
+    def __del__(self):
+        if _LIB is not None and self.addr:
+            _LIB.dlg_host_free(c_void_p(self.addr))
+            self.addr = 0
 
 
 def _host_rows(ptr, n: int) -> np.ndarray:
-    out = _rows_from(ptr, n)
-    lib().dlg_host_free(ctypes.cast(ptr, c_void_p))
-    return out
+    """(n, 2) int32 view of a malloc'd row block (a pointer or its address), which it takes
+    over (no copy)."""
+    addr = ptr if isinstance(ptr, int) else ctypes.cast(ptr, c_void_p).value
+    if n == 0 or not addr:
+        if addr:
+            lib().dlg_host_free(c_void_p(addr))
+        return np.zeros((0, 2), dtype=np.int32)
+    buf = (c_int32 * (2 * n)).from_address(addr)
+    buf._owner = _HostBlock(addr)
+    return np.frombuffer(buf, dtype=np.int32).reshape(n, 2)
+
+
+def _adopt_rows(r) -> np.ndarray | None:
+    """The rows of a dlg_result as a numpy view that owns them (the result no longer does)."""
+    if not r.rows:
+        return None
+    addr, n = ctypes.cast(r.rows, c_void_p).value, r.n_rows   # the address, before the field is cleared
+    r.rows = POINTER(c_int32)()
+    r.n_rows = 0
+    return _host_rows(addr, n)
 
 
 # ----------------------------------------------------------------- formats / generators
@@ -416,7 +439,7 @@ class Ctx:
             for i in range(B):
                 r = res[i]
                 out.append({"iterations": r.iterations, "output_size": r.output_size, "secondary": r.secondary,
-                            "nodes": r.nodes, "rows": _rows_from(r.rows, r.n_rows) if r.rows else None,
+                            "nodes": r.nodes, "rows": _adopt_rows(r),
                             "output_sum": r.output_sum, "output_order_breaks": r.output_order_breaks})
         finally:
             for i in range(B):
@@ -460,7 +483,7 @@ class Ctx:
                    "nodes": r.nodes, "raw_join_rows": r.raw_join_rows, "device_seconds": r.device_seconds,
                    "phases": {PHASES[i]: r.phase_seconds[i] for i in range(10)},
                    "output_sum": r.output_sum, "output_order_breaks": r.output_order_breaks}
-            out["rows"] = _rows_from(r.rows, r.n_rows) if r.rows else None
+            out["rows"] = _adopt_rows(r)
             return out
         finally:
             self.L.dlg_result_free(byref(r))
@@ -570,7 +593,7 @@ class Server:
         try:
             out = {"status": info.status, "iterations": r.iterations, "output_size": r.output_size,
                    "secondary": r.secondary, "nodes": r.nodes,
-                   "rows": _rows_from(r.rows, r.n_rows) if r.rows else None,
+                   "rows": _adopt_rows(r),
                    "submit_s": info.submit_s, "start_s": info.start_s, "end_s": info.end_s,
                    "latency_s": info.end_s - info.submit_s, "batch_size": info.batch_size,
                    "attempts": info.attempts, "slot": info.slot, "reserved_bytes": info.reserved_bytes}

@@ -273,18 +273,55 @@ static void instance_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u32 s
     r->output_order_breaks = (int64_t)breaks;
 }
 
-static void split_rows(const std::vector<int32_t> &all, const std::vector<u64> &b, const Layout &L,
+// Rows of every instance in its own ids: tuple i of the sorted output (of
+// the instance j with bnd[j] <= i < bnd[j + 1], found by a binary search)
+// unpacked in `orient` with shift[j] taken off the first column (and off the
+// second too when both hold ids).
+__global__ void unpack_instances_k(const u64 *t, u64 n, const u64 *bnd, const u32 *shift, u32 B, int orient,
+                                   int both, int32_t *rows) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        u32 lo = 0, hi = B;   // last j with bnd[j] <= i
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) >> 1;
+            if (bnd[mid] <= i) lo = mid; else hi = mid;
+        }
+        const u32 k = tkey(t[i]) ^ 0x80000000u, v = tval(t[i]) ^ 0x80000000u, s = shift[lo];
+        const u32 r0 = orient == DLG_ORIENT_KV ? k : v, r1 = orient == DLG_ORIENT_KV ? v : k;
+        rows[2 * i] = (int32_t)(r0 - s);
+        rows[2 * i + 1] = (int32_t)(both ? r1 - s : r1);
+    }
+}
+
+// Every instance's size and, with `keep`, its host rows: unpacked and shifted
+// back on the device, then one copy into each instance's own array.
+static void split_rows(dlg_ctx *ctx, const u64 *d, const std::vector<u64> &b, const Layout &L, int orient, bool both,
                        bool keep, dlg_result *res) {
+    for (int i = 0; i < L.B; i++) res[i].output_size = (int64_t)(b[i + 1] - b[i]);
+    if (!keep) return;
+    const u64 n = b[L.B];
+    DBuf<int32_t> rows(ctx, 2 * (n ? n : 1));
+    if (n) {
+        std::vector<u32> sh(L.B);
+        for (int i = 0; i < L.B; i++) sh[i] = (u32)L.shift[i];
+        DBuf<u64> bnd(ctx, L.B + 1);
+        DBuf<u32> dsh(ctx, L.B);
+        HIPCHK(hipMemcpyAsync(bnd.p, b.data(), 8 * (L.B + 1), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(dsh.p, sh.data(), 4 * L.B, hipMemcpyHostToDevice, ctx->stream));
+        DLG_LAUNCH(ctx, DLG_K_PACK, 16.0 * n, unpack_instances_k, sgrid(ctx, n), ctx->block_size, 0, d, n,
+                   (const u64 *)bnd.p, (const u32 *)dsh.p, (u32)L.B, orient, both ? 1 : 0, rows.p);
+        ctx->sync();   // the host vectors go out of scope
+    }
     for (int i = 0; i < L.B; i++) {
-        u64 cnt = b[i + 1] - b[i];
-        res[i].output_size = (int64_t)cnt;
-        if (!keep) continue;
+        const u64 cnt = b[i + 1] - b[i];
         res[i].n_rows = (int64_t)cnt;
         res[i].rows = (int32_t *)std::malloc(sizeof(int32_t) * 2 * (cnt ? cnt : 1));
         if (!res[i].rows) fail(DLG_OUT_OF_MEMORY, "host allocation of batch rows");
-        int64_t s = L.shift[i];
-        for (u64 r = 0; r < 2 * cnt; r++) res[i].rows[r] = (int32_t)((int64_t)all[2 * b[i] + r] - s);
+        if (cnt)
+            HIPCHK(hipMemcpyAsync(res[i].rows, rows.p + 2 * b[i], sizeof(int32_t) * 2 * cnt, hipMemcpyDeviceToHost,
+                                  ctx->stream));
     }
+    ctx->sync();
 }
 
 void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const int64_t *n,
@@ -327,7 +364,6 @@ void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const 
     DBuf<u32> ub(ctx, B + 1);
     HIPCHK(hipMemcpyAsync(ub.p, L.ub.data(), 4 * (B + 1), hipMemcpyHostToDevice, ctx->stream));
     PhaseClock pc(ctx);
-    std::vector<int32_t> all;
     int rounds = 0;
     if (prog == DLG_PROG_TC || prog == DLG_PROG_SG || prog == DLG_PROG_SJ) {
         FixpointOut f(ctx);
@@ -339,11 +375,7 @@ void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const 
         std::vector<u64> b = boundaries(ctx, f.full.p, f.nfull, ub, B, hook.pos);
         ctx->sync();
         st->device_seconds = secs(t0, Clock::now());
-        if (o.keep) all.resize(2 * f.nfull + 2);   // host rows only when asked for
-        if (o.keep && f.nfull)
-            download_rows(ctx, f.full.p, f.nfull, prog == DLG_PROG_SG ? DLG_ORIENT_KV : DLG_ORIENT_VK,
-                          all.data());
-        split_rows(all, b, L, o.keep, res);
+        split_rows(ctx, f.full.p, b, L, prog == DLG_PROG_SG ? DLG_ORIENT_KV : DLG_ORIENT_VK, true, o.keep, res);
         for (int i = 0; i < B; i++)
             res[i].iterations = prog == DLG_PROG_SJ ? 1 : (hook.iters[i] ? hook.iters[i] : rounds);
         if (o.check) {   // each instance's digest over its own ids
@@ -443,9 +475,7 @@ void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const 
         HIPCHK(hipMemcpyAsync(hl.data(), largest.p, 4 * B, hipMemcpyDeviceToHost, ctx->stream));
         ctx->sync();
         st->device_seconds = secs(t0, Clock::now());
-        if (o.keep) all.resize(2 * ncomp + 2);
-        if (o.keep && ncomp) download_rows(ctx, out.p, ncomp, DLG_ORIENT_KV, all.data());
-        split_rows(all, b, L, o.keep, res);
+        split_rows(ctx, out.p, b, L, DLG_ORIENT_KV, false, o.keep, res);   // (label, 0): labels are ids
         if (o.check)   // (label, 0): only the label column is shifted
             for (int i = 0; i < B; i++)
                 instance_digest(ctx, out.p + b[i], b[i + 1] - b[i], DLG_ORIENT_KV, (u32)L.shift[i], 0u, &res[i]);
@@ -453,8 +483,6 @@ void batch_run(dlg_ctx *ctx, int prog, int B, const int32_t *const *rows, const 
             res[i].iterations = iters[i];
             res[i].secondary = res[i].output_size ? (i64)hl[i] : -1;
             res[i].nodes = (i64)(hv[i + 1] - hv[i]);
-            if (o.keep)   // labels are ids; the second column stays 0
-                for (i64 r = 0; r < res[i].n_rows; r++) res[i].rows[2 * r + 1] = 0;
         }
     } else {
         fail(DLG_INVALID_INPUT, "unknown program");

@@ -36,6 +36,8 @@ constexpr int TR_U = 4;       // words / rows per lane with their loads in fligh
 constexpr int TR_BATCH = TR_U * WAVE;        // rows per batch of the load-balanced path
 constexpr u32 TR_DMAX_BOUND = 16;   // degrees up to this: output regions bounded by new ends * max degree
 constexpr u64 TR_ENTRY_COST = 512;  // an entry's fixed cost (loads, final flush) in join rows
+constexpr int TR_PART = 6;          // totals per workgroup of the round kernel
+static_assert(TR_PART < dlg_ctx::MAPPED_SEQ, "the round's totals precede its number in the mapped words");
 
 // The word table of a wave: 2^LOG slots, linear probing.  A larger table
 // flushes a long segment fewer times; a smaller one leaves LDS for more waves
@@ -80,6 +82,7 @@ struct TrWave {
     u32 key[TrCfg<LOG>::H];
     u32 bits[TrCfg<LOG>::H];
     u16 used[TrCfg<LOG>::LIMIT];   // occupied slots, in insertion order (a flush keeps them <= LIMIT)
+    u32 touched, changed;          // words of F the wave's flushes read, and wrote
 };
 // The CSR join's load-balanced path also keeps the chunk's probes.
 template <int LOG>
@@ -163,6 +166,10 @@ __device__ u64 tr_flush(TrWave<LOG> &s, u32 &nused, u32 *F, u64 rowbase, u32 lo,
             }
             c += (u32)__popc(nb[q]);
         }
+        u32 changed = 0;
+#pragma unroll
+        for (int q = 0; q < TR_U; q++) changed += (u32)__popcll(__ballot(nb[q] != 0u));
+        if (lane == 0) s.changed += changed;
         u32 tot;
         u32 off = wave_excl_scan32(c, &tot);
         u32 *d = dst + cnt;
@@ -172,6 +179,7 @@ __device__ u64 tr_flush(TrWave<LOG> &s, u32 &nused, u32 *F, u64 rowbase, u32 lo,
         }
         cnt += tot;
     }
+    if (lane == 0) s.touched += n;
     wave_sync();
     nused = 0;
     return cnt;
@@ -291,8 +299,9 @@ __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32
 // rbound[i].  The bound is the source's next join rows, Σ deg over its new
 // ends (exact; `dmax` = 0), or, when every degree is at most dmax,
 // min(new ends * dmax, W) (no degree lookups; the output has gaps).
-// Workgroup b leaves its totals at bpart[4 b ..] (entries with new ends,
-// their bounds, new ends, join rows) and its first entry at brange[b].  An
+// Workgroup b leaves its totals at bpart[TR_PART b ..] (entries with new
+// ends, their bounds, new ends, join rows, words of F read, words of F
+// written) and its first entry at brange[b].  An
 // entry costs about its bound (join rows) plus a fixed TR_ENTRY_COST, so
 // with cost prefix c(i) = aoff[i] + TR_ENTRY_COST * i, workgroup b takes the
 // entries with c(i) in [b, b + 1) * c(nact) / grid (a binary search; no
@@ -314,6 +323,7 @@ __global__ __launch_bounds__(TR_T, (tr_blocks_per_cu<LOG, ELL>())) void tc_rows_
         s.key[i] = TR_EMPTY;
         s.bits[i] = 0u;
     }
+    if (lane == 0) s.touched = s.changed = 0u;
     if (threadIdx.x < 2) {   // the first entries of this workgroup's share and of the next one's
         const u64 b = (u64)blockIdx.x + threadIdx.x, tot = span + TR_ENTRY_COST * nact;
         const u64 key = (b * tot + gridDim.x - 1) / gridDim.x;
@@ -471,18 +481,21 @@ __global__ __launch_bounds__(TR_T, (tr_blocks_per_cu<LOG, ELL>())) void tc_rows_
     }
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) wave_rows += __shfl_xor(wave_rows, o, WAVE);
-    __shared__ u64 s_tot[4][TR_W];
+    __shared__ u64 s_tot[TR_PART][TR_W];
+    wave_sync();
     if (lane == 0) {
         s_tot[0][wv] = wave_act;
         s_tot[1][wv] = wave_bound;
         s_tot[2][wv] = wave_new;
         s_tot[3][wv] = wave_rows;
+        s_tot[4][wv] = s.touched;
+        s_tot[5][wv] = s.changed;
     }
     __syncthreads();
-    if (threadIdx.x < 4) {   // the workgroup's totals, for the compaction
+    if (threadIdx.x < TR_PART) {   // the workgroup's totals, for the compaction
         u64 t = 0;
         for (int w = 0; w < TR_W; w++) t += s_tot[threadIdx.x][w];
-        bpart[4 * blockIdx.x + threadIdx.x] = t;
+        bpart[TR_PART * blockIdx.x + threadIdx.x] = t;
     }
     if (threadIdx.x == 0) {   // and its range of entries
         brange[blockIdx.x] = r0;
@@ -510,7 +523,8 @@ __global__ void tc_rows_segrows_k(const u32 *dy, const u64 *abeg, const u32 *ale
 // the round's row kernel, over the same entries: it adds the totals of the
 // blocks before its own and compacts its entries TA_CHUNK at a time.  The
 // last block writes the round's totals (new ends, Σ bounds, join rows, the
-// list's length) into counters and into the context's mapped host words.
+// list's length, words of F read and written) into counters and into the
+// context's mapped host words.
 constexpr int TA_T = 256, TA_I = 8;
 constexpr u64 TA_CHUNK = (u64)TA_T * TA_I;
 
@@ -521,22 +535,26 @@ __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u
     __shared__ u64 scr[TA_T / WAVE + 1];
     const bool last = blockIdx.x == gridDim.x - 1;
     // the blocks before this one (all of them, for the last block's totals)
-    u64 pc = 0, ps = 0, pn = 0, pr = 0;
+    u64 pc = 0, ps = 0, pn = 0, pr = 0, pt = 0, pw = 0;
     for (u32 j = threadIdx.x; j < (last ? gridDim.x : blockIdx.x); j += TA_T) {
         const bool before = j < blockIdx.x;
         if (before) {
-            pc += bpart[4 * j];
-            ps += bpart[4 * j + 1];
+            pc += bpart[TR_PART * j];
+            ps += bpart[TR_PART * j + 1];
         }
-        pn += bpart[4 * j + 2];
-        pr += bpart[4 * j + 3];
+        pn += bpart[TR_PART * j + 2];
+        pr += bpart[TR_PART * j + 3];
+        pt += bpart[TR_PART * j + 4];
+        pw += bpart[TR_PART * j + 5];
     }
-    u64 ec0, es0, tn, trw;
+    u64 ec0, es0, tn = 0, trw = 0, ttw = 0, tcw = 0;
     block_exclusive_scan(pc, scr, &ec0);
     block_exclusive_scan(ps, scr, &es0);
     if (last) {   // uniform
         block_exclusive_scan(pn, scr, &tn);
         block_exclusive_scan(pr, scr, &trw);
+        block_exclusive_scan(pt, scr, &ttw);
+        block_exclusive_scan(pw, scr, &tcw);
     }
     const u64 e0 = brange[blockIdx.x], e1 = brange[blockIdx.x + 1];
     for (u64 t0 = e0; t0 < e1; t0 += TA_CHUNK) {   // uniform
@@ -570,8 +588,8 @@ __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u
         es0 += ts;
     }
     if (last && threadIdx.x == 0) {   // the round's totals, also into mapped host memory, then its number
-        const u64 v[4] = {tn, es0, trw, ec0};
-        for (int q = 0; q < 4; q++) {
+        const u64 v[TR_PART] = {tn, es0, trw, ec0, ttw, tcw};
+        for (int q = 0; q < TR_PART; q++) {
             counters[q] = v[q];
             hcounters[q] = v[q];
         }
@@ -668,7 +686,7 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
     tr.abeg.ensure(nh ? nh : 1);
     tr.alen.ensure(nh ? nh : 1);
     tr.aoff.ensure(nh ? nh : 1);
-    tr.counters.ensure(4);
+    tr.counters.ensure(TR_PART);
     tr.nd = nu;
     tr.nact = nh;
     tr.rows = 0;
@@ -692,7 +710,7 @@ constexpr u64 TR_LONG_SEGMENTS = 64;   // average probes per segment above which
 template <int LOG, bool ELL>
 static unsigned launch_rows(dlg_ctx *ctx, RowMap &F, TcRows &tr) {
     unsigned g = (unsigned)std::min<u64>((tr.nact + TR_W - 1) / TR_W, (u64)ctx->num_cus * tr_blocks_per_cu<LOG, ELL>());
-    tr.bpart.ensure(4 * (u64)g);
+    tr.bpart.ensure(TR_PART * (u64)g);
     tr.brange.ensure((u64)g + 1);
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, (tc_rows_k<LOG, ELL>), g, TR_T, 0, (const u32 *)tr.dy.p,
                (const u32 *)tr.ax.p, (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, (const u64 *)tr.aoff.p, tr.nact,
@@ -714,7 +732,9 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     tr.rbound.ensure(na);
     // algorithmic bytes, added once the round's join rows are known: 4 B per
     // join row (its matched end) + 20 B per probe (its end, CSR offset,
-    // degree) + 4 B per new end (written as the next round's Δ)
+    // degree) + 4 B per new end (written as the next round's Δ) + 4 B per
+    // word of F read and 4 B per word of F written (the set difference and
+    // merge of the bitmap layout)
     const u64 nd0 = tr.nd;
     const bool big = tr.nd >= TR_LONG_SEGMENTS * na;   // long segments: big tables; short ones: more waves
     unsigned g;
@@ -736,8 +756,8 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     // of the buffers they use can grow (the list never lengthens), and the
     // next launches queue behind them on the stream.
     ctx->wait_mapped(seq);
-    u64 h[4];
-    for (int q = 0; q < 4; q++) h[q] = ((volatile u64 *)ctx->h_mapped)[q];
+    u64 h[TR_PART];
+    for (int q = 0; q < TR_PART; q++) h[q] = ((volatile u64 *)ctx->h_mapped)[q];
     tr.dy.swap(tr.ny);
     tr.ax.swap(tr.nx);
     tr.abeg.swap(tr.nbeg);
@@ -747,7 +767,7 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
     tr.rows = h[1];
     tr.nact = h[3];
     if (raw) *raw += h[2];
-    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[2] + 20.0 * nd0 + 4.0 * h[0]);
+    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[2] + 20.0 * nd0 + 4.0 * h[0] + 4.0 * h[4] + 4.0 * h[5]);
     return h[0];
 }
 

@@ -45,6 +45,22 @@ def test_sort_all_equal_and_presorted(ctx):
     assert np.array_equal(ctx.sort(s[::-1].copy()), s)
 
 
+@pytest.mark.parametrize("n,kb,vb", [(2_097_152, 32, 32), (3_000_001, 9, 3), (20_000_000, 17, 17),
+                                     (8_500_000, 32, 0), (4_200_000, 1, 1)])
+def test_sort_onesweep_sizes(ctx, n, kb, vb):
+    """Large sorts (>= 2^21 keys) take the one-sweep passes with decoupled look-back: equal
+    to numpy's sort at several key widths (1 to 8 passes), and stable under presorted and
+    reversed input."""
+    rng = np.random.default_rng(n + 7 * kb)
+    t = rand_tuples(rng, n, kb, vb)
+    assert np.array_equal(ctx.sort(t), np.sort(t))
+    if n == 3_000_001:
+        s = np.sort(t)
+        assert np.array_equal(ctx.sort(s), s)
+        assert np.array_equal(ctx.sort(s[::-1].copy()), s)
+        assert np.array_equal(ctx.sort_unique(t), np.unique(t))
+
+
 @pytest.mark.parametrize("n,kb,vb", SORT_CASES)
 def test_sort_unique(ctx, n, kb, vb):
     rng = np.random.default_rng(7 * n + vb)
