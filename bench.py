@@ -243,15 +243,16 @@ def bench_batched(args):
 
 
 # ---------------------------------------------------------------------- served
-def served_run(reqs, resident=None, n_resident_q=0, workers=4, max_request_bytes=0, rng_seed=SEED + 5,
-               small_rows=20000):
+def served_run(reqs, resident=None, n_resident_q=0, workers=16, max_batch=16, max_request_bytes=0,
+               rng_seed=SEED + 5, small_rows=20000):
     """Saturated and Poisson-arrival runs of a request list through the server, and the same
     requests run serially in the plain form.  Results equal to the plain runs (rows)."""
     import numpy as np
 
     import mnmgdatalog_amd as dlg
     rng = np.random.default_rng(rng_seed)
-    with dlg.Server(0, workers=workers, max_request_bytes=max_request_bytes, small_rows=small_rows) as srv:
+    with dlg.Server(0, workers=workers, max_batch=max_batch, max_request_bytes=max_request_bytes,
+                    small_rows=small_rows) as srv:
         gid = srv.add_graph(resident) if resident is not None else -1
         admitted, refused = [], 0
         for p, r in reqs:
@@ -297,7 +298,8 @@ def served_run(reqs, resident=None, n_resident_q=0, workers=4, max_request_bytes
         fin = begin + plain_s[j]
         plat.append(fin - arrive[k])
     total = sum(g["output_size"] for g in got)
-    return {"requests": len(work), "refused": refused, "value": total / t_sat, "ms": t_sat * 1e3,
+    return {"requests": len(work), "workers": workers, "max_batch": max_batch, "refused": refused,
+            "value": total / t_sat, "ms": t_sat * 1e3,
             "plain_value": total / t_plain, "plain_ms": t_plain * 1e3, "speedup_vs_plain": t_plain / t_sat,
             "results_equal_plain": bool(same), "batched_requests": int(sum(g["batch_size"] > 1 for g in got)),
             "poisson": {"rate_per_s": lam, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
@@ -308,7 +310,7 @@ def bench_served_small(args):
     from mnmgdatalog_amd import workloads as W
     reqs = W.served_requests(args.small_requests, 2_000, 1.0, 50_000, SEED + 11)
     log(f"served (small): {len(reqs)} requests generated")
-    out = served_run(reqs, workers=args.workers)
+    out = served_run(reqs, workers=args.workers, max_batch=args.max_batch)
     out["workload"] = "served_small_tc_wcc"
     out["sizes"] = (f"{len(reqs)} requests, edges log-normal median 2k, sigma 1.0, max 50k; TC:WCC 1:1; "
                     "ER / R-MAT alternating by pairs; all arrive together (saturated), then Poisson")
@@ -323,7 +325,7 @@ def bench_served_c5(args):
     resident = dlg.gen_rmat(args.rmat_scale, 16 << args.rmat_scale, 0.57, 0.19, 0.19, SEED + 7, undirected=True)
     log(f"served (C5): {len(reqs)} requests and the resident graph generated ({time.perf_counter() - t0:.1f} s)")
     out = served_run(reqs, resident=resident, n_resident_q=max(1, args.requests // 16), workers=args.workers,
-                     max_request_bytes=args.max_request_gb << 30)
+                     max_batch=args.max_batch, max_request_bytes=args.max_request_gb << 30)
     out["workload"] = "served_c5_tc_wcc_mix"
     out["resident_graph"] = f"rmat_scale{args.rmat_scale}_{len(resident)}_rows"
     out["sizes"] = ("edges log-normal median 50k, sigma 1.5, max 50M; TC:WCC 1:1; ER / R-MAT alternating by "
@@ -461,6 +463,7 @@ def bench_plain(args):
                      "traffic": measured_traffic(args.workload, name),
                      "alg_bytes_per_launch": alg_per_launch, "avg_launch_ms": avg_ms, "launches": st["launches"]},
         "kernels": kernel_table(ks),
+        "phases_ms": {k: round(v * 1e3, 3) for k, v in res["phases"].items() if v},   # the last step's
         "cpu_baseline": cpu,
         "energy": None if joules is None else {
             "joules_per_step": joules / args.steps, "avg_watts": joules / elapsed,
@@ -483,7 +486,8 @@ def main():
     ap.add_argument("--output-cap", type=float, default=3e9, help="batched: predicted output cap (tuples)")
     ap.add_argument("--requests", type=int, default=48, help="served C5: request count")
     ap.add_argument("--small-requests", type=int, default=512, help="served small mix: request count")
-    ap.add_argument("--workers", type=int, default=4, help="served: slots")
+    ap.add_argument("--workers", type=int, default=16, help="served: slots")
+    ap.add_argument("--max-batch", type=int, default=16, help="served: instances per batched launch")
     ap.add_argument("--rmat-scale", type=int, default=23, help="served C5: resident R-MAT scale")
     ap.add_argument("--max-request-gb", type=int, default=8, help="served C5: per-request limit")
     ap.add_argument("--launch-steps", type=int, default=3, help="launch: timed runs per variant")

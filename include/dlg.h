@@ -249,6 +249,10 @@ dlg_status dlg_run_wcc_device(dlg_ctx *ctx, const int32_t *d_rows, int64_t n, co
                               dlg_result *res);
 dlg_status dlg_run_single_join_device(dlg_ctx *ctx, const int32_t *d_rows, int64_t n,
                                       const dlg_opts *o, dlg_result *res);
+/* Frees a result's host rows.  Rows of up to 4 GiB come in page-locked host
+ * memory (while the library holds at most 32 GiB of it), larger ones in
+ * malloc'd memory; free them only with dlg_result_free (or dlg_host_free
+ * once taken out of the result). */
 void dlg_result_free(dlg_result *res);
 
 /* ------------------------------------------------------- batched form */
@@ -346,6 +350,8 @@ dlg_status dlg_write_rows(const char *path, const int32_t *h_rows, int64_t n);
 /* Replaces parallel_generate (common/parallel_io.cu:79-108) on rank 0:
  * srand(1); v[i] = rand() % range + 1 for i < 2*rows. */
 dlg_status dlg_generate_rand_rows(int64_t rows, int32_t range, int32_t **h_rows);
+/* Frees any host block the library handed out (rows of results and of the
+ * generators and readers below). */
 void dlg_host_free(void *p);
 
 /* No counterpart in the reference.  |TC| of the rows (src, dst) before

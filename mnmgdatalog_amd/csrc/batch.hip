@@ -315,7 +315,7 @@ static void split_rows(dlg_ctx *ctx, const u64 *d, const std::vector<u64> &b, co
     for (int i = 0; i < L.B; i++) {
         const u64 cnt = b[i + 1] - b[i];
         res[i].n_rows = (int64_t)cnt;
-        res[i].rows = (int32_t *)std::malloc(sizeof(int32_t) * 2 * (cnt ? cnt : 1));
+        res[i].rows = (int32_t *)host_block_alloc(sizeof(int32_t) * 2 * (cnt ? cnt : 1));
         if (!res[i].rows) fail(DLG_OUT_OF_MEMORY, "host allocation of batch rows");
         if (cnt)
             HIPCHK(hipMemcpyAsync(res[i].rows, rows.p + 2 * b[i], sizeof(int32_t) * 2 * cnt, hipMemcpyDeviceToHost,

@@ -99,7 +99,7 @@ void download_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, int32_t *h_row
 
 static void result_rows(dlg_ctx *ctx, const u64 *d, u64 n, int orient, dlg_result *res) {
     res->n_rows = (int64_t)n;
-    res->rows = (int32_t *)std::malloc(sizeof(int32_t) * 2 * (n ? n : 1));
+    res->rows = (int32_t *)host_block_alloc(sizeof(int32_t) * 2 * (n ? n : 1));
     if (!res->rows) fail(DLG_OUT_OF_MEMORY, "host allocation of result rows");
     download_rows(ctx, d, n, orient, res->rows);
 }
@@ -696,7 +696,7 @@ dlg_status dlg_run_single_join_device(dlg_ctx *c, const int32_t *d, int64_t n, c
 
 void dlg_result_free(dlg_result *r) {
     if (!r) return;
-    std::free(r->rows);
+    host_block_free(r->rows);
     r->rows = nullptr;
     r->n_rows = 0;
 }

@@ -172,7 +172,6 @@ void sort_unique(std::vector<uint64_t> &e) {
 
 extern "C" {
 
-void dlg_host_free(void *p) { std::free(p); }
 
 dlg_status dlg_read_rows(const char *path, int32_t **h_rows, int64_t *n) {
     if (!path || !h_rows || !n) return DLG_INVALID_INPUT;

@@ -83,6 +83,7 @@ struct TrWave {
     u32 bits[TrCfg<LOG>::H];
     u16 used[TrCfg<LOG>::LIMIT];   // occupied slots, in insertion order (a flush keeps them <= LIMIT)
     u32 touched, changed;          // words of F the wave's flushes read, and wrote
+    u32 pad[WAVE];                 // each lane's dummy CAS target (never TR_EMPTY)
 };
 // The CSR join's load-balanced path also keeps the chunk's probes.
 template <int LOG>
@@ -105,23 +106,24 @@ constexpr int tr_blocks_per_cu() { return LOG >= 10 && !ELL ? 3 : 4; }
 // `used`; nused is the wave's (uniform) count.
 template <int LOG, int N>
 __device__ __forceinline__ void tr_insert(TrWave<LOG> &s, u32 &nused, const u32 (&w)[N], const u32 (&mk)[N]) {
+    const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const u64 any = __ballot(mk[i] != 0);
         if (!any) continue;   // uniform
-        bool fresh = false;
+        // Linear probing without branches in the loop body: every lane issues
+        // the CAS, lanes with no entry (or already placed) on their own pad
+        // word, which never holds TR_EMPTY; a lane whose slot holds another
+        // word moves on; the loop ends when no lane does.
         u32 h = tr_hash<LOG>(w[i]);
-        if (mk[i]) {
-            while (true) {
-                const u32 k = atomicCAS(&s.key[h], TR_EMPTY, w[i]);
-                if (k == TR_EMPTY || k == w[i]) {
-                    fresh = k == TR_EMPTY;
-                    break;
-                }
-                h = (h + 1) & (TrCfg<LOG>::H - 1);
-            }
-            atomicOr(&s.bits[h], mk[i]);
-        }
+        bool act = mk[i] != 0, fresh = false;
+        do {
+            const u32 k = atomicCAS(act ? &s.key[h] : &s.pad[lane], TR_EMPTY, w[i]);
+            fresh = fresh || (act && k == TR_EMPTY);
+            act = act && k != TR_EMPTY && k != w[i];
+            h = act ? ((h + 1) & (TrCfg<LOG>::H - 1)) : h;
+        } while (__ballot(act));
+        atomicOr(&s.bits[h], mk[i]);   // h is a slot for every lane; no entry ORs 0
         const u64 bal = __ballot(fresh);
         if (fresh)
             s.used[nused + __builtin_amdgcn_mbcnt_hi((u32)(bal >> 32), __builtin_amdgcn_mbcnt_lo((u32)bal, 0u))] =
@@ -324,6 +326,7 @@ __global__ __launch_bounds__(TR_T, (tr_blocks_per_cu<LOG, ELL>())) void tc_rows_
         s.bits[i] = 0u;
     }
     if (lane == 0) s.touched = s.changed = 0u;
+    s.pad[lane] = 0u;
     if (threadIdx.x < 2) {   // the first entries of this workgroup's share and of the next one's
         const u64 b = (u64)blockIdx.x + threadIdx.x, tot = span + TR_ENTRY_COST * nact;
         const u64 key = (b * tot + gridDim.x - 1) / gridDim.x;

@@ -5,6 +5,8 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
+#include <mutex>
 #include <thread>
 
 namespace dlg {
@@ -106,6 +108,45 @@ void Pool::trim_to(size_t keep) {
         cached_bytes_ -= it->first;
         if (it->second.empty()) free_.erase(it);
     }
+}
+
+namespace {
+constexpr size_t PINNED_BLOCK_MAX = 4ull << 30;    // larger blocks stay pageable
+constexpr size_t PINNED_TOTAL_MAX = 32ull << 30;   // page-locked host memory held at once
+std::mutex pinned_mu;
+std::unordered_map<void *, size_t> pinned_blocks;
+size_t pinned_total = 0;
+}  // namespace
+
+void *host_block_alloc(size_t bytes) {
+    if (bytes <= PINNED_BLOCK_MAX) {
+        std::lock_guard<std::mutex> lk(pinned_mu);
+        if (pinned_total + bytes <= PINNED_TOTAL_MAX) {
+            void *p = nullptr;
+            if (hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess && p) {
+                pinned_blocks[p] = bytes;
+                pinned_total += bytes;
+                return p;
+            }
+            (void)hipGetLastError();
+        }
+    }
+    return std::malloc(bytes);
+}
+
+void host_block_free(void *p) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(pinned_mu);
+        auto it = pinned_blocks.find(p);
+        if (it != pinned_blocks.end()) {
+            pinned_total -= it->second;
+            pinned_blocks.erase(it);
+            (void)hipHostFree(p);
+            return;
+        }
+    }
+    std::free(p);
 }
 
 // Timing events only (read by hipEventElapsedTime, never used to order
@@ -248,6 +289,8 @@ void dlg_ctx_destroy(dlg_ctx *c) {
 }
 
 const char *dlg_last_error(const dlg_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
+
+void dlg_host_free(void *p) { host_block_free(p); }
 
 dlg_status dlg_ctx_sync(dlg_ctx *c) {
     if (!c) return DLG_INVALID_INPUT;
