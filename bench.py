@@ -426,7 +426,8 @@ def bench_plain(args):
         d_rows = ctx.upload(rows)
         res = None
         for _ in range(args.warmup):
-            res = ctx.run(prog, None, keep_output=False, device_rows=d_rows, n=n, strategy=args.strategy)
+            res = ctx.run(prog, None, keep_output=False, device_rows=d_rows, n=n, strategy=args.strategy,
+                          relax=args.relax)
         meter = energy_meter(ctx)
         ctx.kernel_stats_reset()
         ctx.profiling(True)
@@ -434,7 +435,8 @@ def bench_plain(args):
         e0 = meter() if meter else None
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            res = ctx.run(prog, None, keep_output=False, device_rows=d_rows, n=n, strategy=args.strategy)
+            res = ctx.run(prog, None, keep_output=False, device_rows=d_rows, n=n, strategy=args.strategy,
+                          relax=args.relax)
         ctx.check(ctx.L.dlg_ctx_sync(ctx.h), "sync")
         t1 = time.perf_counter()
         e1 = meter() if meter else None
@@ -457,7 +459,7 @@ def bench_plain(args):
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": dict(cfg, iterations=res["iterations"], output_tuples=int(out_tuples),
-                       raw_join_rows=int(res["raw_join_rows"]), strategy=args.strategy),
+                       raw_join_rows=int(res["raw_join_rows"]), strategy=args.strategy, relax=args.relax),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.workload, name),
@@ -492,6 +494,7 @@ def main():
     ap.add_argument("--max-request-gb", type=int, default=8, help="served C5: per-request limit")
     ap.add_argument("--launch-steps", type=int, default=3, help="launch: timed runs per variant")
     ap.add_argument("--strategy", type=int, default=0, help="full-relation layout (0 auto, 1..4)")
+    ap.add_argument("--relax", type=int, default=0, help="WCC round form (0 auto, 1 push, 2 pull)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process: every GPU box here has one GPU, so "

@@ -198,7 +198,16 @@ typedef struct dlg_opts {
                                 round's new ends of one source share words and cache
                                 lines of F.  0 auto (plain runs with id ranges of at
                                 least 16,384 ids), 1 never, 2 always.  Same output   */
-    int32_t reserved[3];
+    int32_t relax;           /* WCC rounds launched one by one: 0 auto (round one read
+                                off the sorted adjacency lists, later rounds pulled
+                                when the frontier's edges are at least half of all
+                                edges, pushed otherwise); 1 push every round (one
+                                thread per frontier edge, atomic minimum at the far
+                                end: the reference's join order); 2 pull every round
+                                (each node takes the least label of its neighbours).
+                                1 and 2 also select rounds launched one by one under
+                                strategy 0.  Same labels and iteration counts        */
+    int32_t reserved[2];
 } dlg_opts;
 
 /* Phase timers in the reference's CSV order (tc.cu:413-420). */

@@ -542,14 +542,20 @@ void index_lookup_values(dlg_ctx *ctx, const Index &ix, u32 *out);
 // ------------------------------------------------------------- WCC (k_wcc.hip)
 struct WccState {
     u64 V = 0;
-    DBuf<u32> L, mark, fnode, flabel, nnode, cnt, pstart, counter;
+    DBuf<u32> L, Ln, mark, fnode, flabel, nnode, cnt, pstart, counter;
     DBuf<u64> offs;
     explicit WccState(dlg_ctx *c)
-        : L(c), mark(c), fnode(c), flabel(c), nnode(c), cnt(c), pstart(c), counter(c), offs(c) {}
+        : L(c), Ln(c), mark(c), fnode(c), flabel(c), nnode(c), cnt(c), pstart(c), counter(c), offs(c) {}
 };
 void wcc_setup(dlg_ctx *ctx, WccState &s, u64 V);
+// WCC round forms: WCC_AUTO pulls when the frontier's edges are at least half
+// of the graph's, WCC_PUSH walks the frontier's edges, WCC_PULL every node's.
+enum { WCC_AUTO = 0, WCC_PUSH = 1, WCC_PULL = 2 };
 // One round over frontier of size nf; returns the new frontier size.
-u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf, u32 round);
+u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf, u32 round, int form = WCC_PUSH);
+// Round one from the identity labels (every node in the frontier) when each
+// node's neighbour list is sorted: returns the new frontier size.
+u64 wcc_first_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr);
 // The reference's round on tuples (dlg_wcc_relax): cc (one (node, label)
 // tuple per group of `ix`, in group order) lowered in place to the per-node
 // least label of cc and Δ ⋈ E; returns |Δ'| with Δ' = cc' minus cc (sorted) in

@@ -43,7 +43,7 @@ void PhaseClock::finish(double *out) {
 }
 
 Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
-    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0, false, 0, 0};
+    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0, false, 0, 0, 0};
     if (o) {
         if (o->load_factor < 0 || o->load_factor > 0.95)
             fail(DLG_INVALID_INPUT, "load_factor out of (0, 0.95]");
@@ -64,6 +64,8 @@ Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
         r.strategy = o->strategy;
         if (o->reorder < 0 || o->reorder > 2) fail(DLG_INVALID_INPUT, "reorder out of 0..2");
         r.reorder = o->reorder;
+        if (o->relax < 0 || o->relax > 2) fail(DLG_INVALID_INPUT, "relax out of 0..2");
+        r.relax = o->relax;
     }
     return r;
 }
@@ -581,7 +583,7 @@ void wcc_solve(dlg_ctx *ctx, const WccGraph &g, const Opts &o, PhaseClock &pc, d
     WccState s(ctx);
     wcc_setup(ctx, s, g.V);
     int it = 0;
-    const bool graph = o.strategy == 2 || (o.strategy == 0 && g.ne <= (1ull << 25));
+    const bool graph = o.strategy == 2 || (o.strategy == 0 && o.relax == WCC_AUTO && g.ne <= (1ull << 25));
     pc.mark(DLG_PH_JOIN);
     if (graph) {
         it = wcc_rounds_graph(ctx, s, g.ix, g.nbr.p, g.ne, o.max_it, 16);
@@ -590,7 +592,11 @@ void wcc_solve(dlg_ctx *ctx, const WccGraph &g, const Opts &o, PhaseClock &pc, d
         u64 prev = g.V;                                    // wcc.cu:236-242
         while (true) {
             pc.mark(DLG_PH_JOIN);
-            u64 cur = nf ? wcc_round(ctx, s, g.ix, g.nbr.p, nf, (u32)(it + 1)) : 0;
+            u64 cur = 0;
+            if (nf && it == 0 && o.relax == WCC_AUTO)
+                cur = wcc_first_round(ctx, s, g.ix, g.nbr.p);      // labels are still the node ids
+            else if (nf)
+                cur = wcc_round(ctx, s, g.ix, g.nbr.p, nf, (u32)(it + 1), o.relax);
             it++;
             nf = cur;
             if (cur == prev) break;                        // wcc.cu:366-383

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace dlg {
 
@@ -335,11 +336,94 @@ __global__ void lookup_values_sorted_k(const u64 *rows, u64 n, const u32 *gstart
     }
 }
 
+// Rank directory over a dense key range [lo, lo + W): entry j holds the
+// number of keys below id lo + 32j (high half) and the presence bits of ids
+// lo + 32j .. lo + 32j + 31 (low half).  A key's group is its rank among the
+// keys (groups are in key order), so a lookup is one 8-byte read of a table
+// of W/4 bytes (2 MiB for 2^23 ids: it stays in L2) and a popcount.
+//
+// Presence bits, one thread per group: the groups' keys ascend, so lanes of
+// one word are adjacent; a segmented OR leaves one atomic per word and wave.
+__global__ void rank_bits_k(const u64 *rows, const u32 *gstart, u64 G, u32 lo, u32 *bits) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 lim = (G + WAVE - 1) / WAVE * WAVE;   // whole waves iterate together
+    for (u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x; g < lim; g += stride) {
+        u32 w = 0xffffffffu, b = 0;
+        if (g < G) {
+            const u32 d = tkey(rows[gstart[g]]) - lo;
+            w = d >> 5;
+            b = 1u << (d & 31);
+        }
+#pragma unroll
+        for (int s = 1; s < WAVE; s <<= 1) {
+            const u32 tb = __shfl_up(b, s, WAVE), tw = __shfl_up(w, s, WAVE);
+            if (lane >= s && tw == w) b |= tb;
+        }
+        const u32 nw = __shfl_down(w, 1, WAVE);
+        if (g < G && (lane == WAVE - 1 || nw != w)) atomicOr(&bits[w], b);
+    }
+}
+
+__global__ void rank_count_k(const u32 *bits, u64 nw, u32 *cnt) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nw; j += stride) cnt[j] = (u32)__popc(bits[j]);
+}
+
+__global__ void rank_pack_k(const u32 *bits, const u64 *below, u64 nw, u64 *dir) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nw; j += stride)
+        dir[j] = (below[j] << 32) | bits[j];
+}
+
+__global__ void lookup_values_rank_k(const u64 *rows, u64 n, const u64 *dir, u32 lo, u64 W, u32 *out) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32 d = tval(rows[i]) - lo;
+        u32 g = 0xffffffffu;
+        if ((u64)d < W) {
+            const u64 e = dir[d >> 5];
+            const u32 b = (u32)e, bit = d & 31;
+            if ((b >> bit) & 1u) g = (u32)(e >> 32) + (u32)__popc(b & ((1u << bit) - 1u));
+        }
+        out[i] = g;
+    }
+}
+
+// Rank lookups when the key range is at most 64 ids per key (plus 2^20):
+// the directory then takes at most 2 bytes per key over the hash table's slots.
+constexpr u64 RANK_IDS_PER_KEY = 64, RANK_SLACK = 1ull << 20;
+
 void index_lookup_values(dlg_ctx *ctx, const Index &ix, u32 *out) {
     if (ix.n == 0) return;
     if (ix.sorted) {
         DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * ix.n, lookup_values_sorted_k, sgrid(ctx, ix.n), ctx->block_size, 0,
                    (const u64 *)ix.rows.p, ix.n, (const u32 *)ix.gstart.p, ix.groups, out);
+        return;
+    }
+    u64 ends[2];
+    HIPCHK(hipMemcpyAsync(ctx->h_scalars, ix.rows.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync((char *)ctx->h_scalars + 8, ix.rows.p + (ix.n - 1), 8, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    ctx->sync();
+    std::memcpy(ends, ctx->h_scalars, 16);
+    const u32 lo = (u32)(ends[0] >> 32);
+    const u64 W = (u64)(u32)(ends[1] >> 32) - lo + 1;   // rows are sorted: first and last key
+    if (W <= RANK_IDS_PER_KEY * ix.groups + RANK_SLACK) {
+        const u64 nw = (W + 31) / 32;
+        DBuf<u32> bits(ctx, nw), cnt(ctx, nw);
+        DBuf<u64> below(ctx, nw + 1), dir(ctx, nw);
+        fill_u32(ctx, bits.p, nw, 0u);
+        DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * ix.groups, rank_bits_k, sgrid(ctx, ix.groups), ctx->block_size, 0,
+                   (const u64 *)ix.rows.p, (const u32 *)ix.gstart.p, ix.groups, lo, bits.p);
+        DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nw, rank_count_k, sgrid(ctx, nw), ctx->block_size, 0,
+                   (const u32 *)bits.p, nw, cnt.p);
+        scan_u32_u64(ctx, cnt.p, nw, below.p);
+        DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * nw, rank_pack_k, sgrid(ctx, nw), ctx->block_size, 0,
+                   (const u32 *)bits.p, (const u64 *)below.p, nw, dir.p);
+        // 8 B row + 4 B group per row; the directory reads hit L2
+        DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * ix.n, lookup_values_rank_k, sgrid(ctx, ix.n), ctx->block_size, 0,
+                   (const u64 *)ix.rows.p, ix.n, (const u64 *)dir.p, lo, W, out);
         return;
     }
     DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * ix.n, lookup_values_k, sgrid(ctx, ix.n), ctx->block_size, 0,

@@ -441,37 +441,61 @@ constexpr u32 OS_SPIN_LIMIT = 1u << 22;
 constexpr u64 OS_VALUE = (1ull << 56) - 1;
 
 // Digits of every pass, counted in one LDS histogram per wave (8 passes x
-// 256 bins x 4 waves = 32 KiB), then added into the global counts.  A pass
-// whose digits span fewer than 8 bits (the top one) has few distinct digits:
-// its count goes through ballot matching, one atomic per distinct digit per
-// wave, instead of colliding lanes.
-__global__ __launch_bounds__(256) void os_hist_k(const u64 *in, u64 n, RadixKey key, u32 passes, u32 top_bits,
-                                                 u32 *hist) {
-    constexpr int BINS = 256, W = 256 / WAVE;
+// 256 bins x 4 waves = 32 KiB), then added into the global counts.  Lanes
+// that hit one LDS address in one atomic are serialised, and clustered input
+// (sorted runs, hub keys) puts most of a wave's lanes on one digit: so a
+// wave counts runs of equal digits over adjacent lanes, one atomic per run
+// head (one per lane on shuffled input, one per wave on sorted input).  A
+// pass whose digits span fewer than 8 bits (the top one) has few distinct
+// digits: its count goes through ballot matching, one atomic per distinct
+// digit per wave.  Each thread loads OH_I keys of a tile before counting any.
+constexpr int OH_T = 256, OH_I = 8, OH_TILE = OH_T * OH_I;
+
+__global__ __launch_bounds__(OH_T) void os_hist_k(const u64 *in, u64 n, RadixKey key, u32 passes, u32 top_bits,
+                                                  u32 *hist) {
+    constexpr int BINS = 256, W = OH_T / WAVE;
     __shared__ u32 h[W][8][BINS];
-    for (int i = threadIdx.x; i < W * 8 * BINS; i += 256) (&h[0][0][0])[i] = 0;
+    for (int i = threadIdx.x; i < W * 8 * BINS; i += OH_T) (&h[0][0][0])[i] = 0;
     __syncthreads();
     const int wave = threadIdx.x / WAVE;
     const u64 lt = lanemask_lt();
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    const u64 lim = (n + WAVE - 1) / WAVE * WAVE;   // whole waves iterate together
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += stride) {
-        const bool valid = i < n;
-        const u64 x = valid ? in[i] : 0ull;
-        const u64 c = ((u64)(tkey(x) - key.min_hi) << key.vb) | (u64)(tval(x) - key.min_lo);
-        for (u32 p = 0; p + 1 < passes; p++)
-            if (valid) atomicAdd(&h[wave][p][(u32)(c >> (8 * p)) & 0xffu], 1u);
-        const u32 p = passes - 1, d = (u32)(c >> (8 * p)) & 0xffu;
-        u64 m = __ballot(valid);
-        for (u32 b = 0; b < top_bits; b++) {
-            const bool bit = (d >> b) & 1u;
-            const u64 bb = __ballot(bit);
-            m &= bit ? bb : ~bb;
+    const u32 p = passes - 1;
+    for (u64 base = (u64)blockIdx.x * OH_TILE; base < n; base += (u64)gridDim.x * OH_TILE) {   // block-uniform
+        u64 x[OH_I];
+#pragma unroll
+        for (int r = 0; r < OH_I; r++) {
+            const u64 i = base + (u64)r * OH_T + threadIdx.x;
+            x[r] = i < n ? in[i] : 0ull;
         }
-        if (valid && (m & lt) == 0) atomicAdd(&h[wave][p][d], (u32)__popcll(m));
+#pragma unroll
+        for (int r = 0; r < OH_I; r++) {
+            const bool valid = base + (u64)r * OH_T + threadIdx.x < n;   // valid lanes: a prefix of the wave
+            const u64 c = ((u64)(tkey(x[r]) - key.min_hi) << key.vb) | (u64)(tval(x[r]) - key.min_lo);
+            const u64 vm = __ballot(valid);
+            const int nvalid = __popcll(vm), lane = threadIdx.x & (WAVE - 1);
+            for (u32 q = 0; q < p; q++) {
+                const u32 dq = (u32)(c >> (8 * q)) & 0xffu;
+                const u32 prev = __shfl_up(dq, 1, WAVE);
+                const bool head = valid && (lane == 0 || prev != dq);
+                const u64 heads = __ballot(head);
+                if (head) {
+                    const u64 after = heads & ~((2ull << lane) - 1ull);   // heads above this lane
+                    const int end = after ? __ffsll((unsigned long long)after) - 1 : nvalid;
+                    atomicAdd(&h[wave][q][dq], (u32)(end - lane));
+                }
+            }
+            const u32 d = (u32)(c >> (8 * p)) & 0xffu;
+            u64 m = vm;
+            for (u32 b = 0; b < top_bits; b++) {
+                const bool bit = (d >> b) & 1u;
+                const u64 bb = __ballot(bit);
+                m &= bit ? bb : ~bb;
+            }
+            if (valid && (m & lt) == 0) atomicAdd(&h[wave][p][d], (u32)__popcll(m));
+        }
     }
     __syncthreads();
-    for (u32 i = threadIdx.x; i < passes * BINS; i += 256) {
+    for (u32 i = threadIdx.x; i < passes * BINS; i += OH_T) {
         u32 v = 0;
 #pragma unroll
         for (int w = 0; w < W; w++) v += (&h[w][0][0])[i];
@@ -613,8 +637,10 @@ static void radix_onesweep(dlg_ctx *ctx, u64 *&src, u64 *&dst, u64 n, RadixKey k
     fill_u64(ctx, status.p, tiles * 256, 0ull);
     key.shift = 0;
     const u32 top_bits = (u32)(total_bits - 8 * (int)(passes - 1));
-    DLG_LAUNCH(ctx, DLG_K_SORT_UPSWEEP, 8.0 * n, os_hist_k, sgrid(ctx, n), 256, 0, (const u64 *)src, n, key, passes,
-               top_bits, hist.p);
+    // 4 blocks per CU (32 KiB of LDS each), each looping over tiles
+    const unsigned hg = (unsigned)std::min<u64>((n + OH_TILE - 1) / OH_TILE, (u64)ctx->num_cus * 4);
+    DLG_LAUNCH(ctx, DLG_K_SORT_UPSWEEP, 8.0 * n, os_hist_k, hg, OH_T, 0, (const u64 *)src, n, key, passes, top_bits,
+               hist.p);
     DLG_LAUNCH(ctx, DLG_K_SORT_UPSWEEP, 8.0 * 256 * passes, os_starts_k, 1, 256, 0, (const u32 *)hist.p, passes,
                starts);
     for (u32 p = 0; p < passes; p++) {

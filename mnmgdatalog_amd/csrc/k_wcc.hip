@@ -15,6 +15,20 @@
 //             (mark[m] == round), giving Δ' = {(m, L'(m)) : L'(m) < L(m)}
 // Edge traversal is load balanced: degree counts -> scan -> one thread per
 // edge with a binary search in the frontier offsets.
+//
+// Two other forms of the same round (same labels, same Δ'):
+//   round one  every node is in Δ with its own id as label, so L'(m) is the
+//              least of m and its neighbours; neighbour lists are sorted
+//              (the index keeps each key's rows in value order), so that is
+//              min(m, first neighbour): one read per node instead of per edge
+//   pull       L'(m) = min(L(m), min over ALL neighbours v of L(v)).  Equal to
+//              the push round because a neighbour v outside Δ has already
+//              pushed its current label: v's label last changed in a round
+//              after which v was in Δ, and the next round lowered m to it
+//              (round one has every node in Δ).  One thread per edge again,
+//              but reading labels instead of taking atomics: a segmented
+//              minimum over the wave's lanes of one node, then one atomic per
+//              node and wave into a copy Ln of L; Δ' = {m : Ln(m) < L(m)}.
 #include "dlg_internal.h"
 
 #include <algorithm>
@@ -43,6 +57,39 @@ __global__ void wcc_degree_k(const u32 *fnode, u64 nf, const u32 *gstart, u32 *c
     }
 }
 
+// Appends of a block's tile with ONE counter atomic per block: an atomic per
+// wave on one address serialises over the whole grid (~70k waves for a
+// round that lowers 4.6M nodes).  Each thread offers I items; the slots
+// follow (item, wave, lane) order.  s_cnt: I * T / WAVE + 1 u32 of LDS, at
+// most 65.  Every thread of the block must call it.
+template <int T, int I>
+__device__ __forceinline__ void block_append(const bool (&app)[I], u32 *count, u32 *s_cnt, u32 (&slot)[I]) {
+    constexpr int W = T / WAVE, K = I * W;
+    static_assert(K <= WAVE, "one wave scans the per-wave counts");
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    u64 bm[I];
+#pragma unroll
+    for (int r = 0; r < I; r++) {
+        bm[r] = __ballot(app[r]);
+        if (lane == 0) s_cnt[r * W + wave] = (u32)__popcll(bm[r]);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const u32 c = lane < K ? s_cnt[lane] : 0u;
+        u32 tot = 0;
+        const u32 ex = wave_excl_scan32(c, &tot);
+        u32 b = 0;
+        if (lane == 0 && tot) b = atomicAdd(count, tot);
+        b = __shfl(b, 0, WAVE);
+        if (lane < K) s_cnt[lane] = b + ex;
+    }
+    __syncthreads();
+    const u64 lt = lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < I; r++) slot[r] = s_cnt[r * W + wave] + (u32)__popcll(bm[r] & lt);
+    __syncthreads();   // s_cnt may be reused
+}
+
 constexpr int WR_T = 256, WR_I = 8, WR_TILE = WR_T * WR_I;
 
 __global__ __launch_bounds__(WR_T) void wcc_relax_k(const u32 *flabel, u64 nf, const u32 *pstart,
@@ -50,8 +97,8 @@ __global__ __launch_bounds__(WR_T) void wcc_relax_k(const u32 *flabel, u64 nf, c
                                                     u32 *L, u32 *mark, u32 round, u32 *next_node,
                                                     u32 *next_count) {
     __shared__ u64 s_lo, s_hi;
+    __shared__ u32 s_cnt[WR_I * WR_T / WAVE + 1];
     const u64 base = (u64)blockIdx.x * WR_TILE;
-    const int lane = threadIdx.x & (WAVE - 1);
     if (threadIdx.x == 0) {
         u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
         u64 lo = upper_bound_u64(offs, 0, nf, base) - 1;
@@ -60,30 +107,132 @@ __global__ __launch_bounds__(WR_T) void wcc_relax_k(const u32 *flabel, u64 nf, c
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
-    const u64 lt = lanemask_lt();
+    bool app[WR_I];
+    u32 m[WR_I], slot[WR_I];
 #pragma unroll
     for (int r = 0; r < WR_I; r++) {
         u64 o = base + (u64)r * WR_T + threadIdx.x;
-        bool app = false;
-        u32 m = 0;
+        app[r] = false;
+        m[r] = 0;
         if (o < total) {
             u64 i = upper_bound_u64(offs, lo, hi, o) - 1;
-            m = nbr[(u64)pstart[i] + (o - offs[i])];
+            m[r] = nbr[(u64)pstart[i] + (o - offs[i])];
             u32 l = flabel[i];
-            if (L[m] > l) {
-                u32 old = atomicMin(&L[m], l);
-                if (old > l) app = atomicExch(&mark[m], round) != round;
+            if (L[m[r]] > l) {
+                u32 old = atomicMin(&L[m[r]], l);
+                if (old > l) app[r] = atomicExch(&mark[m[r]], round) != round;
             }
         }
-        u64 bm = __ballot(app);
-        if (bm) {
-            int leader = __ffsll((unsigned long long)bm) - 1;
-            u32 b = 0;
-            if (lane == leader) b = atomicAdd(next_count, (u32)__popcll(bm));
-            b = __shfl(b, leader, WAVE);
-            if (app) next_node[b + (u32)__popcll(bm & lt)] = m;
+    }
+    block_append<WR_T, WR_I>(app, next_count, s_cnt, slot);
+#pragma unroll
+    for (int r = 0; r < WR_I; r++)
+        if (app[r]) next_node[slot[r]] = m[r];
+}
+
+constexpr int WA_T = 256, WA_I = 16, WA_TILE = WA_T * WA_I;   // node-parallel kernels: one tile per block
+
+__device__ __forceinline__ u64 upper_bound_u32(const u32 *a, u64 lo, u64 hi, u64 x) {
+    while (lo < hi) {
+        const u64 mid = (lo + hi) >> 1;
+        if ((u64)a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Round one: L(m) = min(m, first neighbour of m); lowered nodes appended with
+// their label.  Every node has a neighbour (the nodes are the edge keys).
+__global__ __launch_bounds__(WA_T) void wcc_first_k(const u32 *gstart, const u32 *nbr, u64 V, u32 *L,
+                                                    u32 *next_node, u32 *next_label, u32 *count) {
+    __shared__ u32 s_cnt[WA_I * WA_T / WAVE + 1];
+    const u64 base = (u64)blockIdx.x * WA_TILE;
+    bool app[WA_I];
+    u32 l[WA_I], slot[WA_I];
+#pragma unroll
+    for (int r = 0; r < WA_I; r++) {
+        const u64 i = base + (u64)r * WA_T + threadIdx.x;
+        app[r] = false;
+        l[r] = 0;
+        if (i < V) {
+            const u32 f = nbr[gstart[i]];
+            app[r] = f < (u32)i;
+            l[r] = app[r] ? f : (u32)i;
+            if (app[r]) L[i] = l[r];
         }
     }
+    block_append<WA_T, WA_I>(app, count, s_cnt, slot);
+#pragma unroll
+    for (int r = 0; r < WA_I; r++)
+        if (app[r]) {
+            next_node[slot[r]] = (u32)(base + (u64)r * WA_T + threadIdx.x);
+            next_label[slot[r]] = l[r];
+        }
+}
+
+// Pull round over all ne edges (CSR gstart/nbr): Ln(m) = min(Ln(m), L(v)) for
+// every edge (m, v), with Ln == L on entry.  Lanes of one node are adjacent
+// (edges in node order), so a segmented minimum over the wave leaves one
+// candidate per node and wave.
+constexpr int WP_T = 256, WP_I = 8, WP_TILE = WP_T * WP_I;
+
+__global__ __launch_bounds__(WP_T) void wcc_pull_k(const u32 *gstart, u64 V, u64 ne, const u32 *nbr,
+                                                   const u32 *L, u32 *Ln) {
+    __shared__ u64 s_lo, s_hi;
+    const u64 base = (u64)blockIdx.x * WP_TILE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (threadIdx.x == 0) {
+        const u64 last = (base + WP_TILE < ne ? base + WP_TILE : ne) - 1;
+        const u64 lo = upper_bound_u32(gstart, 0, V, base) - 1;
+        s_lo = lo;
+        s_hi = upper_bound_u32(gstart, lo, V, last);
+    }
+    __syncthreads();
+    const u64 lo = s_lo, hi = s_hi;
+#pragma unroll
+    for (int r = 0; r < WP_I; r++) {
+        const u64 o = base + (u64)r * WP_T + threadIdx.x;
+        u32 m = 0xffffffffu, val = 0xffffffffu;
+        if (o < ne) {
+            m = (u32)(upper_bound_u32(gstart, lo, hi, o) - 1);
+            val = L[nbr[o]];
+        }
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const u32 tv = __shfl_up(val, d, WAVE), tm = __shfl_up(m, d, WAVE);
+            if (lane >= d && tm == m) val = min(val, tv);
+        }
+        const u32 nm = __shfl_down(m, 1, WAVE);
+        const bool tail = o < ne && (lane == WAVE - 1 || nm != m);
+        if (tail && val < Ln[m]) atomicMin(&Ln[m], val);
+    }
+}
+
+// Δ' of a pull round: nodes whose label dropped, appended with the new label;
+// L takes the new labels, so L == Ln again afterwards.
+__global__ __launch_bounds__(WA_T) void wcc_changed_k(u32 *L, const u32 *Ln, u64 V, u32 *next_node,
+                                                      u32 *next_label, u32 *count) {
+    __shared__ u32 s_cnt[WA_I * WA_T / WAVE + 1];
+    const u64 base = (u64)blockIdx.x * WA_TILE;
+    bool app[WA_I];
+    u32 l[WA_I], slot[WA_I];
+#pragma unroll
+    for (int r = 0; r < WA_I; r++) {
+        const u64 i = base + (u64)r * WA_T + threadIdx.x;
+        app[r] = false;
+        l[r] = 0;
+        if (i < V) {
+            l[r] = Ln[i];
+            app[r] = l[r] < L[i];
+            if (app[r]) L[i] = l[r];
+        }
+    }
+    block_append<WA_T, WA_I>(app, count, s_cnt, slot);
+#pragma unroll
+    for (int r = 0; r < WA_I; r++)
+        if (app[r]) {
+            next_node[slot[r]] = (u32)(base + (u64)r * WA_T + threadIdx.x);
+            next_label[slot[r]] = l[r];
+        }
 }
 
 __global__ void wcc_gather_k(const u32 *node, u64 n, const u32 *L, u32 *label) {
@@ -92,11 +241,32 @@ __global__ void wcc_gather_k(const u32 *node, u64 n, const u32 *L, u32 *label) {
     for (; i < n; i += stride) label[i] = L[node[i]];
 }
 
-__global__ void wcc_comp_count_k(const u32 *L, u64 V, u32 *ccount) {
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    const u64 lim = (V + WAVE - 1) / WAVE * WAVE;   // whole waves iterate together
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += stride)
-        wave_count_by_key(ccount, i < V ? L[i] : 0u, i < V);
+// Nodes per label.  A giant component's label covers most of every tile, and
+// one atomic per wave on its counter would serialise over the grid: the
+// tile's first label is counted in registers and added once per block; the
+// other labels go through per-wave grouping.
+__global__ __launch_bounds__(WA_T) void wcc_comp_count_k(const u32 *L, u64 V, u32 *ccount) {
+    __shared__ u32 s_part[WA_T / WAVE];
+    const u64 base = (u64)blockIdx.x * WA_TILE;
+    const u32 hot = L[base];
+    u32 mine = 0;
+#pragma unroll
+    for (int r = 0; r < WA_I; r++) {
+        const u64 i = base + (u64)r * WA_T + threadIdx.x;
+        const bool valid = i < V;
+        const u32 k = valid ? L[i] : 0u;
+        mine += (valid && k == hot) ? 1u : 0u;
+        wave_count_by_key(ccount, k, valid && k != hot);
+    }
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) mine += (u32)__shfl_xor((int)mine, d, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0) s_part[threadIdx.x / WAVE] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u32 s = 0;
+        for (int w = 0; w < WA_T / WAVE; w++) s += s_part[w];
+        atomicAdd(&ccount[hot], s);
+    }
 }
 
 // flags[v] = ccount[v] > 0; largest via atomicMax.
@@ -123,14 +293,49 @@ __global__ void wcc_emit_k(const u32 *flags, const u64 *offs, u64 V, const u64 *
         if (flags[i]) out[offs[i]] = tmake(tkey(rows[gstart[i]]), 0x80000000u);
 }
 
+static u64 read_counter(dlg_ctx *ctx, WccState &s) {
+    HIPCHK(hipMemcpyAsync(ctx->h_scalars, s.counter.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    return *(u32 *)ctx->h_scalars;
+}
+
+u64 wcc_first_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr) {
+    const u64 V = s.V;
+    if (V == 0) return 0;
+    fill_u32(ctx, s.counter.p, 1, 0u);
+    // 4 B start + 4 B first neighbour per node; lowered nodes: 4 B label + 8 B appended
+    DLG_LAUNCH(ctx, DLG_K_WCC_RELAX, 8.0 * V, wcc_first_k, grid_for(V, WA_TILE), WA_T, 0,
+               (const u32 *)ix.gstart.p, nbr, V, s.L.p, s.fnode.p, s.flabel.p, s.counter.p);
+    return read_counter(ctx, s);
+}
+
+static u64 wcc_pull_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr) {
+    const u64 V = s.V, ne = ix.n;
+    s.Ln.ensure(V);
+    HIPCHK(hipMemcpyAsync(s.Ln.p, s.L.p, V * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    fill_u32(ctx, s.counter.p, 1, 0u);
+    if (ne)
+        // 4 B neighbour id + 4 B label read per edge; per node 4 B start and 4 B Ln
+        DLG_LAUNCH(ctx, DLG_K_WCC_RELAX, 8.0 * ne + 8.0 * V, wcc_pull_k, grid_for(ne, WP_TILE), WP_T, 0,
+                   (const u32 *)ix.gstart.p, V, ne, nbr, (const u32 *)s.L.p, s.Ln.p);
+    DLG_LAUNCH(ctx, DLG_K_WCC_RELAX, 8.0 * V, wcc_changed_k, grid_for(V, WA_TILE), WA_T, 0, s.L.p,
+               (const u32 *)s.Ln.p, V, s.nnode.p, s.flabel.p, s.counter.p);
+    s.fnode.swap(s.nnode);
+    return read_counter(ctx, s);
+}
+
 // One round.  Returns |Δ'| (the new frontier size); the frontier is replaced.
-u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf, u32 round) {
+u64 wcc_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 nf, u32 round, int form) {
+    if (form == WCC_PULL) return wcc_pull_round(ctx, s, ix, nbr);
     s.cnt.ensure(nf);
     s.pstart.ensure(nf);
     s.offs.ensure(nf + 1);
     DLG_LAUNCH(ctx, DLG_K_WCC_RELAX, 16.0 * nf, wcc_degree_k, sgrid(ctx, nf), ctx->block_size, 0,
                (const u32 *)s.fnode.p, nf, (const u32 *)ix.gstart.p, s.cnt.p, s.pstart.p);
     u64 total = scan_u32_u64(ctx, s.cnt.p, nf, s.offs.p);
+    // a pull round reads every edge without atomics; measured on R-MAT it costs about
+    // what pushing an eighth of the edges does
+    if (form == WCC_AUTO && 8 * total >= (u64)ix.n) return wcc_pull_round(ctx, s, ix, nbr);
     fill_u32(ctx, s.counter.p, 1, 0u);
     if (total) {
         // 4 B neighbour id + 4 B label read + 4 B atomicMin per edge, frontier reads amortised
@@ -233,7 +438,7 @@ u64 wcc_finalize(dlg_ctx *ctx, WccState &s, const Index &ix, DBuf<u64> &out, i64
     DBuf<u64> offs(ctx, V + 1);
     fill_u32(ctx, ccount.p, V, 0u);
     fill_u32(ctx, s.counter.p, 1, 0u);
-    DLG_LAUNCH(ctx, DLG_K_WCC_FINAL, 12.0 * V, wcc_comp_count_k, sgrid(ctx, V), ctx->block_size, 0,
+    DLG_LAUNCH(ctx, DLG_K_WCC_FINAL, 12.0 * V, wcc_comp_count_k, grid_for(V, WA_TILE), WA_T, 0,
                (const u32 *)s.L.p, V, ccount.p);
     DLG_LAUNCH(ctx, DLG_K_WCC_FINAL, 8.0 * V, wcc_comp_flags_k, sgrid(ctx, V), ctx->block_size, 0,
                (const u32 *)ccount.p, V, flags.p, s.counter.p);
@@ -338,11 +543,11 @@ __global__ __launch_bounds__(WR_T) void gw_relax_k(const u32 *flabel, const u32 
                                                    const u32 *nbr, u32 *L, u32 *mark, u32 *next_node,
                                                    WccDev *st) {
     __shared__ u64 s_lo, s_hi;
+    __shared__ u32 s_cnt[WR_I * WR_T / WAVE + 1];
     const u64 total = st->total, nf = st->nf;
     const u64 base = (u64)blockIdx.x * WR_TILE;
-    if (st->done || base >= total) return;
+    if (st->done || base >= total) return;   // block-uniform
     const u32 round = st->it + 1;
-    const int lane = threadIdx.x & (WAVE - 1);
     if (threadIdx.x == 0) {
         u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
         u64 lo = upper_bound_u64(offs, 0, nf, base) - 1;
@@ -351,30 +556,27 @@ __global__ __launch_bounds__(WR_T) void gw_relax_k(const u32 *flabel, const u32 
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
-    const u64 lt = lanemask_lt();
+    bool app[WR_I];
+    u32 m[WR_I], slot[WR_I];
 #pragma unroll
     for (int r = 0; r < WR_I; r++) {
         u64 o = base + (u64)r * WR_T + threadIdx.x;
-        bool app = false;
-        u32 m = 0;
+        app[r] = false;
+        m[r] = 0;
         if (o < total) {
             u64 i = upper_bound_u64(offs, lo, hi, o) - 1;
-            m = nbr[(u64)pstart[i] + (o - offs[i])];
+            m[r] = nbr[(u64)pstart[i] + (o - offs[i])];
             u32 l = flabel[i];
-            if (L[m] > l) {
-                u32 old = atomicMin(&L[m], l);
-                if (old > l) app = atomicExch(&mark[m], round) != round;
+            if (L[m[r]] > l) {
+                u32 old = atomicMin(&L[m[r]], l);
+                if (old > l) app[r] = atomicExch(&mark[m[r]], round) != round;
             }
         }
-        u64 bm = __ballot(app);
-        if (bm) {
-            int leader = __ffsll((unsigned long long)bm) - 1;
-            u32 b = 0;
-            if (lane == leader) b = atomicAdd(&st->count, (u32)__popcll(bm));
-            b = __shfl(b, leader, WAVE);
-            if (app) next_node[b + (u32)__popcll(bm & lt)] = m;
-        }
     }
+    block_append<WR_T, WR_I>(app, &st->count, s_cnt, slot);
+#pragma unroll
+    for (int r = 0; r < WR_I; r++)
+        if (app[r]) next_node[slot[r]] = m[r];
 }
 
 __global__ void gw_gather_k(const u32 *node, const u32 *L, u32 *label, const WccDev *st) {

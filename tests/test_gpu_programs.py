@@ -331,6 +331,43 @@ def test_wcc_graph_captured_and_eager_agree(ctx, orc, name):
         assert_same(ctx.run("wcc", rows, strategy=strategy), want, secondary=True)
 
 
+@pytest.mark.parametrize("name", ["data_223001.bin", "data_214078.bin", "paper.bin", "dummy.bin",
+                                  "data_88234.bin"])
+def test_wcc_round_forms_agree(ctx, orc, name):
+    """Push rounds, pull rounds, and auto (round one off the sorted lists, then pull or push by
+    frontier size) give the oracle's labels and iteration count."""
+    rows = read_bin(name)
+    want = orc.wcc(rows)
+    for relax in (1, 2, 0):
+        assert_same(ctx.run("wcc", rows, strategy=1, relax=relax), want, secondary=True)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_wcc_round_forms_random(ctx, orc, seed):
+    rows = rand_rows(700 + seed, 500 * (seed + 1), 400 * (seed + 1), neg=seed % 2 == 0)
+    want = orc.wcc(rows)
+    for relax in (2, 0):
+        assert_same(ctx.run("wcc", rows, strategy=1, relax=relax), want, secondary=True)
+
+
+def test_wcc_round_forms_truncated(ctx):
+    """Stopped after k rounds, every form leaves the same labels (the pull round is the push
+    round, not just the same fixpoint)."""
+    rows = read_bin("data_214078.bin")
+    for k in (1, 2, 3, 5):
+        runs = [ctx.run("wcc", rows, strategy=1, relax=relax, max_iterations=k) for relax in (1, 2, 0)]
+        for r in runs[1:]:
+            assert r["iterations"] == runs[0]["iterations"] == k
+            assert r["output_size"] == runs[0]["output_size"]
+            assert np.array_equal(r["rows"], runs[0]["rows"])
+
+
+def test_wcc_relax_out_of_range(ctx):
+    with pytest.raises(dlg.DlgError) as e:
+        ctx.run("wcc", read_bin("dummy.bin"), relax=3)
+    assert e.value.status == dlg.DLG_INVALID_INPUT
+
+
 def test_wcc_graph_captured_max_iterations(ctx):
     rows = read_bin("data_223001.bin")
     a = ctx.run("wcc", rows, strategy=1, max_iterations=37)
