@@ -251,6 +251,20 @@ __device__ __forceinline__ u32 radix_digit(u64 x, const RadixKey &k, u32 mask) {
     return (u32)(c >> k.shift) & mask;
 }
 
+// The same 8-bit digit in 32-bit operations (the branches are uniform): from
+// the value half when it lies there, from the key half when it lies there,
+// from both when it straddles them.  shift - vb < 32 (the key half has at
+// most 32 bits); shift < vb <= 32 otherwise.
+__device__ __forceinline__ u32 radix_digit8_hl(u32 hi, u32 lo, const RadixKey &k) {
+    if (k.shift >= k.vb) return __builtin_amdgcn_ubfe(hi, k.shift - k.vb, 8);
+    u32 d = lo >> k.shift;
+    if (k.vb - k.shift < 8) d |= hi << (k.vb - k.shift);
+    return d & 0xffu;
+}
+__device__ __forceinline__ u32 radix_digit8(u64 x, const RadixKey &k) {
+    return radix_digit8_hl(tkey(x) - k.min_hi, tval(x) - k.min_lo, k);
+}
+
 // min/max of the key and value halves: out[0]=min_hi out[1]=max_hi out[2]=min_lo out[3]=max_lo
 __global__ void minmax_init_k(u32 *out) {
     if (threadIdx.x == 0) {
@@ -443,12 +457,13 @@ constexpr u64 OS_VALUE = (1ull << 56) - 1;
 // Digits of every pass, counted in one LDS histogram per wave (8 passes x
 // 256 bins x 4 waves = 32 KiB), then added into the global counts.  Lanes
 // that hit one LDS address in one atomic are serialised, and clustered input
-// (sorted runs, hub keys) puts most of a wave's lanes on one digit: so a
-// wave counts runs of equal digits over adjacent lanes, one atomic per run
-// head (one per lane on shuffled input, one per wave on sorted input).  A
-// pass whose digits span fewer than 8 bits (the top one) has few distinct
+// (sorted runs, hub keys) puts a whole wave on one digit: a wave whose lanes
+// share the digit adds its count with one atomic, other waves one per lane.
+// A pass whose digits span fewer than 8 bits (the top one) has few distinct
 // digits: its count goes through ballot matching, one atomic per distinct
-// digit per wave.  Each thread loads OH_I keys of a tile before counting any.
+// digit per wave.  Each thread loads OH_I keys of a tile before counting
+// any.  The kernel is bound by its vector instructions, so the digits come
+// from 32-bit bit-field extracts of the two halves.
 constexpr int OH_T = 256, OH_I = 8, OH_TILE = OH_T * OH_I;
 
 __global__ __launch_bounds__(OH_T) void os_hist_k(const u64 *in, u64 n, RadixKey key, u32 passes, u32 top_bits,
@@ -458,7 +473,6 @@ __global__ __launch_bounds__(OH_T) void os_hist_k(const u64 *in, u64 n, RadixKey
     for (int i = threadIdx.x; i < W * 8 * BINS; i += OH_T) (&h[0][0][0])[i] = 0;
     __syncthreads();
     const int wave = threadIdx.x / WAVE;
-    const u64 lt = lanemask_lt();
     const u32 p = passes - 1;
     for (u64 base = (u64)blockIdx.x * OH_TILE; base < n; base += (u64)gridDim.x * OH_TILE) {   // block-uniform
         u64 x[OH_I];
@@ -470,28 +484,32 @@ __global__ __launch_bounds__(OH_T) void os_hist_k(const u64 *in, u64 n, RadixKey
 #pragma unroll
         for (int r = 0; r < OH_I; r++) {
             const bool valid = base + (u64)r * OH_T + threadIdx.x < n;   // valid lanes: a prefix of the wave
-            const u64 c = ((u64)(tkey(x[r]) - key.min_hi) << key.vb) | (u64)(tval(x[r]) - key.min_lo);
             const u64 vm = __ballot(valid);
-            const int nvalid = __popcll(vm), lane = threadIdx.x & (WAVE - 1);
+            if (vm == 0) continue;                                     // wave-uniform
+            const u32 hi = tkey(x[r]) - key.min_hi, lo = tval(x[r]) - key.min_lo;
+            const u32 nvalid = (u32)__popcll(vm);
+            RadixKey kq = key;
             for (u32 q = 0; q < p; q++) {
-                const u32 dq = (u32)(c >> (8 * q)) & 0xffu;
-                const u32 prev = __shfl_up(dq, 1, WAVE);
-                const bool head = valid && (lane == 0 || prev != dq);
-                const u64 heads = __ballot(head);
-                if (head) {
-                    const u64 after = heads & ~((2ull << lane) - 1ull);   // heads above this lane
-                    const int end = after ? __ffsll((unsigned long long)after) - 1 : nvalid;
-                    atomicAdd(&h[wave][q][dq], (u32)(end - lane));
+                kq.shift = 8 * q;
+                const u32 dq = radix_digit8_hl(hi, lo, kq);
+                const u32 d0 = __builtin_amdgcn_readfirstlane(dq);   // lane 0: valid
+                if ((__ballot(dq == d0) & vm) == vm) {
+                    if (threadIdx.x % WAVE == 0) atomicAdd(&h[wave][q][d0], nvalid);
+                } else if (valid) {
+                    atomicAdd(&h[wave][q][dq], 1u);
                 }
             }
-            const u32 d = (u32)(c >> (8 * p)) & 0xffu;
-            u64 m = vm;
+            kq.shift = 8 * p;
+            const u32 d = radix_digit8_hl(hi, lo, kq), nd = ~d;
+            u32 mlo = (u32)vm, mhi = (u32)(vm >> 32);
             for (u32 b = 0; b < top_bits; b++) {
-                const bool bit = (d >> b) & 1u;
-                const u64 bb = __ballot(bit);
-                m &= bit ? bb : ~bb;
+                const u32 clr = (u32)__builtin_amdgcn_sbfe((int)nd, b, 1);
+                const u64 set = __ballot(clr == 0u);
+                mlo &= (u32)set ^ clr;
+                mhi &= (u32)(set >> 32) ^ clr;
             }
-            if (valid && (m & lt) == 0) atomicAdd(&h[wave][p][d], (u32)__popcll(m));
+            const u32 below = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
+            if (valid && below == 0) atomicAdd(&h[wave][p][d], (u32)__popc(mlo) + (u32)__popc(mhi));
         }
     }
     __syncthreads();
@@ -520,6 +538,23 @@ __device__ __forceinline__ void os_store(u64 *p, u64 v) {
 }
 
 template <int T, int I>
+__device__ __forceinline__ bool os_load_tile(const u64 *in, u64 n, u64 tile0, u64 wbase, int lane, u64 (&keys)[I]) {
+    constexpr int TILE = T * I;
+    const bool full = tile0 + TILE <= n;   // every tile but the last: no per-key bounds checks
+    if (full) {
+#pragma unroll
+        for (int r = 0; r < I; r++) keys[r] = in[wbase + (u64)r * WAVE + lane];
+    } else {
+#pragma unroll
+        for (int r = 0; r < I; r++) {
+            const u64 idx = wbase + (u64)r * WAVE + lane;
+            keys[r] = idx < n ? in[idx] : 0ull;
+        }
+    }
+    return full;
+}
+
+template <int T, int I>
 __global__ __launch_bounds__(T) void os_pass_k(const u64 *in, u64 *out, u64 n, RadixKey key, const u32 *dstart,
                                                u64 *status, u32 *ticket, u32 pass, u32 *err) {
     constexpr int BINS = 256, W = T / WAVE, TILE = T * I;
@@ -530,35 +565,44 @@ __global__ __launch_bounds__(T) void os_pass_k(const u64 *in, u64 *out, u64 n, R
     __shared__ u32 sc[T / WAVE + 1];
     __shared__ u32 s_tile;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    for (int i = threadIdx.x; i < W * BINS; i += T) (&wcnt[0][0])[i] = 0;
-    __syncthreads();
-    const u32 tile = s_tile;
-    const u64 tile0 = (u64)tile * TILE;
-    const u64 wbase = tile0 + (u64)wave * (WAVE * I);
-    const u64 lt = lanemask_lt();
+    // The keys of tile blockIdx.x are loaded while the ticket is in flight (blocks
+    // mostly start in index order); a block whose ticket differs loads its own.
+    u32 tk = 0;
+    if (threadIdx.x == 0) tk = atomicAdd(ticket, 1u);
+    u32 tile = blockIdx.x;
+    u64 tile0 = (u64)tile * TILE;
+    u64 wbase = tile0 + (u64)wave * (WAVE * I);
     u64 keys[I];
     u32 pk[I];   // digit | (rank within the wave's digit run << 8)
-#pragma unroll
-    for (int r = 0; r < I; r++) {
-        const u64 idx = wbase + (u64)r * WAVE + lane;
-        keys[r] = idx < n ? in[idx] : 0ull;
+    bool full = os_load_tile<T, I>(in, n, tile0, wbase, lane, keys);
+    if (threadIdx.x == 0) s_tile = tk;
+    for (int i = threadIdx.x; i < W * BINS; i += T) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+    if (s_tile != tile) {   // block-uniform
+        tile = s_tile;
+        tile0 = (u64)tile * TILE;
+        wbase = tile0 + (u64)wave * (WAVE * I);
+        full = os_load_tile<T, I>(in, n, tile0, wbase, lane, keys);
     }
+    // Lanes of equal digit by ballot matching: per bit, clr = -1 when the lane's
+    // bit is clear, the ballot of clr == 0 is the lanes with it set, and the
+    // lane keeps (ballot ^ clr) -- the lanes that agree with it on that bit.
 #pragma unroll
     for (int r = 0; r < I; r++) {
-        const u64 idx = wbase + (u64)r * WAVE + lane;
-        const bool valid = idx < n;
-        const u32 d = radix_digit(keys[r], key, BINS - 1);
-        u64 m = __ballot(valid);
+        const bool valid = full || wbase + (u64)r * WAVE + lane < n;
+        const u32 d = radix_digit8(keys[r], key), nd = ~d;
+        const u64 vm = full ? ~0ull : __ballot(valid);
+        u32 mlo = (u32)vm, mhi = (u32)(vm >> 32);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            const bool bit = (d >> b) & 1u;
-            const u64 bb = __ballot(bit);
-            m &= bit ? bb : ~bb;
+            const u32 clr = (u32)__builtin_amdgcn_sbfe((int)nd, b, 1);
+            const u64 set = __ballot(clr == 0u);
+            mlo &= (u32)set ^ clr;
+            mhi &= (u32)(set >> 32) ^ clr;
         }
-        const u32 before = __popcll(m & lt);
+        const u32 before = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
         const u32 cnt = wcnt[wave][d];
-        if (valid && before == 0) wcnt[wave][d] = cnt + (u32)__popcll(m);
+        if (valid && before == 0) wcnt[wave][d] = cnt + (u32)__popc(mlo) + (u32)__popc(mhi);
         pk[r] = d | ((cnt + before) << 8);
         __builtin_amdgcn_wave_barrier();
     }
@@ -603,13 +647,12 @@ __global__ __launch_bounds__(T) void os_pass_k(const u64 *in, u64 *out, u64 n, R
             }
             os_store(mine, pre | (excl + tot));
         }
-        gofs[d] = dstart[d] + (u32)excl;
+        gofs[d] = dstart[d] + (u32)excl - ex;   // output position of stage slot i of digit d: gofs[d] + i
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < I; r++) {
-        const u64 idx = wbase + (u64)r * WAVE + lane;
-        if (idx < n) {
+        if (full || wbase + (u64)r * WAVE + lane < n) {
             const u32 d = pk[r] & 0xffu;
             stage[bstart[d] + wcnt[wave][d] + (pk[r] >> 8)] = keys[r];
         }
@@ -619,16 +662,16 @@ __global__ __launch_bounds__(T) void os_pass_k(const u64 *in, u64 *out, u64 n, R
     const u32 tn = rem < (u64)TILE ? (u32)rem : (u32)TILE;
     for (u32 i = threadIdx.x; i < tn; i += T) {
         const u64 x = stage[i];
-        const u32 d = radix_digit(x, key, BINS - 1);
-        out[(u64)gofs[d] + (i - bstart[d])] = x;
+        out[(u64)(gofs[radix_digit8(x, key)] + i)] = x;   // < n < 2^32
     }
 }
 
 // All passes of an 8-bit sort: histograms of every pass in one read, then one
 // kernel per pass (one read, one write of the keys).
 static void radix_onesweep(dlg_ctx *ctx, u64 *&src, u64 *&dst, u64 n, RadixKey key, int total_bits) {
+    constexpr int OT = OS_T, OI = OS_I, OTILE = OS_TILE;
     const u32 passes = (u32)((total_bits + 7) / 8);
-    const u64 tiles = (n + OS_TILE - 1) / OS_TILE;
+    const u64 tiles = (n + OTILE - 1) / OTILE;
     if (tiles > 0xffffffffull) fail(DLG_OVERFLOW, "radix sort: too many tiles");
     DBuf<u32> hist(ctx, 8 * 256 * 2 + 16);   // histograms, digit starts, tickets, error flag
     DBuf<u64> status(ctx, tiles * 256);
@@ -645,7 +688,7 @@ static void radix_onesweep(dlg_ctx *ctx, u64 *&src, u64 *&dst, u64 n, RadixKey k
                starts);
     for (u32 p = 0; p < passes; p++) {
         key.shift = 8 * p;
-        DLG_LAUNCH(ctx, DLG_K_SORT_DOWNSWEEP, 16.0 * n, (os_pass_k<OS_T, OS_I>), (unsigned)tiles, OS_T, 0,
+        DLG_LAUNCH(ctx, DLG_K_SORT_DOWNSWEEP, 16.0 * n, (os_pass_k<OT, OI>), (unsigned)tiles, OT, 0,
                    (const u64 *)src, dst, n, key, (const u32 *)(starts + 256 * p), status.p, tickets + p, p, err);
         std::swap(src, dst);
     }
