@@ -207,7 +207,14 @@ typedef struct dlg_opts {
                                 (each node takes the least label of its neighbours).
                                 1 and 2 also select rounds launched one by one under
                                 strategy 0.  Same labels and iteration counts        */
-    int32_t reserved[2];
+    int32_t queue;           /* TC in the bitmap layout, plain runs: rounds queued
+                                ahead of the host, each reading its sizes from the
+                                device state the round before left and leaving its
+                                totals in a ring of mapped host words (a round
+                                whose output outgrows its buffer does nothing and
+                                is queued again after the buffer grows).  0 auto
+                                (4), 1 one round at a time, at most 8            */
+    int32_t reserved[1];
 } dlg_opts;
 
 /* Phase timers in the reference's CSV order (tc.cu:413-420). */

@@ -43,7 +43,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("load_factor", c_double), ("block_size", c_int32), ("radix_bits", c_int32),
                 ("strategy", c_int32), ("max_iterations", c_int32), ("keep_output", c_int32),
                 ("check_output", c_int32), ("join_method", c_int32), ("reorder", c_int32),
-                ("relax", c_int32), ("reserved", c_int32 * 2)]
+                ("relax", c_int32), ("queue", c_int32), ("reserved", c_int32 * 1)]
 
 
 class Result(ctypes.Structure):

@@ -109,13 +109,18 @@ struct dlg_ctx {
     hipEvent_t cur_a = nullptr;
     // pinned host scalar area for size read-backs
     uint64_t *h_scalars = nullptr;
-    // pinned host words that kernels write directly (mapped): a fixpoint
-    // round's totals, then (after a system-scope fence) the round's sequence
-    // number at MAPPED_SEQ; the host waits for that number, with no copy
+    // pinned host words that kernels write directly (mapped), as a ring of
+    // MAPPED_SLOTS slots of MAPPED_WORDS words: a fixpoint round's totals go
+    // to slot seq % MAPPED_SLOTS, then (after a system-scope fence) the
+    // round's sequence number to that slot's word MAPPED_SEQ; the host waits
+    // for that number, with no copy.  Rounds queued ahead of the host each
+    // have a slot of their own.
     uint64_t *h_mapped = nullptr, *d_mapped = nullptr;
     uint64_t mapped_seq = 0;
-    static constexpr int MAPPED_SEQ = 7;
+    static constexpr int MAPPED_SEQ = 7, MAPPED_WORDS = 8, MAPPED_SLOTS = 8;
     static constexpr int MAPPED_SPIN_US = 2000;   // busy-wait this long, then yield between checks
+    uint64_t *mapped_host(uint64_t seq) const { return h_mapped + MAPPED_WORDS * (seq % MAPPED_SLOTS); }
+    uint64_t *mapped_dev(uint64_t seq) const { return d_mapped + MAPPED_WORDS * (seq % MAPPED_SLOTS); }
 
     void prof_begin();
     void prof_end(int kid, double bytes);
@@ -125,9 +130,9 @@ struct dlg_ctx {
         if (prof) kstat[kid].bytes += bytes;
     }
     void sync();
-    // Wait until a kernel has written `seq` to h_mapped[MAPPED_SEQ] (spinning:
-    // no sleep between the kernel's write and the host's next launch); fails
-    // if the stream runs dry without it.
+    // Wait until a kernel has written `seq` to word MAPPED_SEQ of its slot
+    // (spinning: no sleep between the kernel's write and the host's next
+    // launch); fails if the stream runs dry without it.
     void wait_mapped(uint64_t seq);
     // read `count` u64 scalars from device memory into host (synchronises)
     void read_scalars(const uint64_t *d, int count, uint64_t *h);
@@ -450,6 +455,10 @@ struct TcRows {
 void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr);
 // One round: N = (Δ ⋈ E) \ F into F, Δ = N (grouped).  Returns |N|; *raw += |Δ ⋈ E|.
 u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw);
+// Every round to the fixpoint (or max_it > 0 rounds), with up to `queue`
+// rounds (0: the default) queued ahead of the host; returns the round count,
+// *total += Σ |N|.
+int tc_rows_run(dlg_ctx *ctx, RowMap &F, TcRows &tr, int max_it, int queue, u64 *total, u64 *raw);
 // The current Δ as (z, x) tuples.
 void tc_rows_tuples(dlg_ctx *ctx, const TcRows &tr, DBuf<u64> &out);
 // Locality order of the ids 0 .. W-1 of the edges (src[i], dst[i]) (order.cpp):

@@ -21,6 +21,7 @@ struct Opts {
     int join;        // 0 hash index, 1 sorted index (binary search)
     int reorder;     // TC bitmap layout: 0 auto, 1 never, 2 always renumber into a locality order
     int relax;       // WCC eager rounds: 0 auto, 1 push, 2 pull
+    int queue;       // TC bitmap rounds queued ahead of the host: 0 auto, 1..8
 };
 Opts read_opts(dlg_ctx *ctx, const dlg_opts *o);
 

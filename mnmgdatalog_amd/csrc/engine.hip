@@ -25,6 +25,8 @@ PhaseClock::~PhaseClock() {
 }
 
 void PhaseClock::mark(int ph) {
+    // the same phase again (every round of a loop) adds nothing to the totals: no event
+    if (!marks.empty() && marks.back().first == ph && ph >= 0) return;
     if (ctx->event_free.empty()) ctx->event_free.push_back(timing_event());
     hipEvent_t e = ctx->event_free.back();
     ctx->event_free.pop_back();
@@ -43,7 +45,7 @@ void PhaseClock::finish(double *out) {
 }
 
 Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
-    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0, false, 0, 0, 0};
+    Opts r{ctx->load_factor, ctx->radix_bits, 0, true, 0, 0, false, 0, 0, 0, 0};
     if (o) {
         if (o->load_factor < 0 || o->load_factor > 0.95)
             fail(DLG_INVALID_INPUT, "load_factor out of (0, 0.95]");
@@ -66,6 +68,8 @@ Opts read_opts(dlg_ctx *ctx, const dlg_opts *o) {
         r.reorder = o->reorder;
         if (o->relax < 0 || o->relax > 2) fail(DLG_INVALID_INPUT, "relax out of 0..2");
         r.relax = o->relax;
+        if (o->queue < 0 || o->queue > dlg_ctx::MAPPED_SLOTS) fail(DLG_INVALID_INPUT, "queue out of 0..8");
+        r.queue = o->queue;
     }
     return r;
 }
@@ -345,17 +349,19 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         pc.mark(DLG_PH_DEDUP);
         tc_rows_setup(ctx, ix, F, tr);                  // Δ0 = F = unique(E reversed)
         u64 total = tr.nd, raw = 0;
-        while (true) {
-            pc.mark(DLG_PH_JOIN);
-            u64 nn = tc_rows_round(ctx, F, tr, &raw);
-            total += nn;
-            it++;
-            if (hook) {
+        pc.mark(DLG_PH_JOIN);
+        if (!hook) {
+            it = tc_rows_run(ctx, F, tr, o.max_it, o.queue, &total, &raw);   // to |F| unchanged (tc.cu:294-300)
+        } else {
+            while (true) {
+                u64 nn = tc_rows_round(ctx, F, tr, &raw);
+                total += nn;
+                it++;
                 tc_rows_tuples(ctx, tr, N);
                 hook->on_round(ctx, N.p, nn, it, false);
+                if (nn == 0) break;                     // |F| unchanged (tc.cu:294-300)
+                if (o.max_it > 0 && it >= o.max_it) break;
             }
-            if (nn == 0) break;                     // |F| unchanged (tc.cu:294-300)
-            if (o.max_it > 0 && it >= o.max_it) break;
         }
         pc.mark(DLG_PH_FINAL);
         out.raw += (i64)raw;

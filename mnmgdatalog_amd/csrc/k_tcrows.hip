@@ -22,6 +22,7 @@
 #include "dlg_internal.h"
 
 #include <algorithm>
+#include <deque>
 #include <type_traits>
 #include <vector>
 
@@ -37,7 +38,19 @@ constexpr int TR_BATCH = TR_U * WAVE;        // rows per batch of the load-balan
 constexpr u32 TR_DMAX_BOUND = 16;   // degrees up to this: output regions bounded by new ends * max degree
 constexpr u64 TR_ENTRY_COST = 512;  // an entry's fixed cost (loads, final flush) in join rows
 constexpr int TR_PART = 6;          // totals per workgroup of the round kernel
-static_assert(TR_PART < dlg_ctx::MAPPED_SEQ, "the round's totals precede its number in the mapped words");
+// The round state in device memory (TcRows::counters), left by each round's
+// compaction for the next round, so that rounds can be queued ahead of the
+// host: the round's totals (new ends = the next Δ, Σ bounds = the next output
+// span, join rows, the next list's length, words of F read and written),
+// then whether the last row kernel ran, and the stall flag (an output span
+// above the output buffer: the round did nothing; the host grows the buffer).
+enum { TR_ST_ND = 0, TR_ST_SPAN = 1, TR_ST_RAW = 2, TR_ST_NACT = 3, TR_ST_RUN = TR_PART, TR_ST_STALL, TR_ST_WORDS };
+// A round's mapped slot: the totals, then its status, then its number.
+enum : u64 { TR_RAN = 1, TR_IDLE = 2, TR_STALLED = 3 };
+constexpr int TR_QUEUE = 4;   // rounds queued ahead of the host by default
+static_assert(TR_PART + 1 <= dlg_ctx::MAPPED_SEQ, "the round's totals and status precede its number in a slot");
+// every queued round has a mapped slot of its own (a depth asked for is capped too)
+static_assert(TR_QUEUE <= dlg_ctx::MAPPED_SLOTS, "more queued rounds than mapped slots");
 
 // The word table of a wave: 2^LOG slots, linear probing.  A larger table
 // flushes a long segment fewer times; a smaller one leaves LDS for more waves
@@ -310,10 +323,23 @@ __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32
 // global counter), and its four waves share them through an LDS counter.
 template <int LOG, bool ELL>
 __global__ __launch_bounds__(TR_T, (tr_blocks_per_cu<LOG, ELL>())) void tc_rows_k(
-    const u32 *dy, const u32 *ax, const u64 *abeg, const u32 *alen, const u64 *aoff, u64 nact, u64 span,
+    const u32 *dy, const u32 *ax, const u64 *abeg, const u32 *alen, const u64 *aoff, u64 *st, u64 cap,
     const u64 *csr_off, const u32 *csr_dst, const u32 *deg, const uint4 *ell, u32 dmax, u32 *F, const u64 *row_base,
     const u32 *col_lo, u32 lo, u64 wcap, u32 *ny, u32 *rcnt, u64 *rbound, u64 *bpart, u64 *brange) {
     using C = TrCfg<LOG>;
+    // The round's gate, read from the state the previous round left (every
+    // block reads the same): no ends, no join rows or no entries -- nothing
+    // to do; an output span above the output buffer's capacity -- nothing
+    // done, and the stall flag set for the host.  Block 0 records whether the
+    // round ran, for the compaction.
+    const u64 nact = st[TR_ST_NACT], span = st[TR_ST_SPAN];
+    const bool idle = st[TR_ST_STALL] != 0 || st[TR_ST_ND] == 0 || span == 0 || nact == 0;
+    const bool stall = !idle && span > cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[TR_ST_RUN] = idle || stall ? 0 : 1;
+        if (stall) st[TR_ST_STALL] = 1;
+    }
+    if (idle || stall) return;   // block-uniform
     using SW = std::conditional_t<ELL, TrWave<LOG>, TrWaveLB<LOG>>;
     __shared__ SW sw[TR_W];
     __shared__ u64 s_rng[2];
@@ -537,6 +563,17 @@ __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u
                                                           u64 *hcounters, u64 seq) {
     __shared__ u64 scr[TA_T / WAVE + 1];
     const bool last = blockIdx.x == gridDim.x - 1;
+    if (counters[TR_ST_RUN] == 0) {   // the row kernel did nothing: the state stays, the slot says why
+        if (last && threadIdx.x == 0) {
+            for (int q = 0; q < TR_PART; q++) hcounters[q] = 0;
+            hcounters[TR_ST_SPAN] = counters[TR_ST_SPAN];
+            hcounters[TR_PART] = counters[TR_ST_STALL] ? TR_STALLED : TR_IDLE;
+            __threadfence_system();
+            *(volatile u64 *)(hcounters + dlg_ctx::MAPPED_SEQ) = seq;
+            __threadfence_system();
+        }
+        return;
+    }
     // the blocks before this one (all of them, for the last block's totals)
     u64 pc = 0, ps = 0, pn = 0, pr = 0, pt = 0, pw = 0;
     for (u32 j = threadIdx.x; j < (last ? gridDim.x : blockIdx.x); j += TA_T) {
@@ -596,6 +633,7 @@ __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u
             counters[q] = v[q];
             hcounters[q] = v[q];
         }
+        hcounters[TR_PART] = TR_RAN;
         __threadfence_system();
         *(volatile u64 *)(hcounters + dlg_ctx::MAPPED_SEQ) = seq;
         __threadfence_system();
@@ -647,6 +685,22 @@ __global__ void tc_rows_tuples_k(const u32 *dy, const u32 *ax, const u64 *abeg, 
         for (u32 i = threadIdx.x; i < alen[j]; i += blockDim.x) out[pos[j] + i] = tmake(dy[abeg[j] + i], ax[j]);
 }
 
+// The first round's state: Δ0's ends and entries (its span is already in
+// place), no totals, not stalled.
+__global__ void tc_rows_state_k(u64 *st, u64 nd, u64 nact) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st[TR_ST_ND] = nd;
+        st[TR_ST_RAW] = 0;
+        st[TR_ST_NACT] = nact;
+        st[4] = st[5] = 0;
+        st[TR_ST_RUN] = 0;
+        st[TR_ST_STALL] = 0;
+    }
+}
+__global__ void tc_rows_unstall_k(u64 *st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st[TR_ST_STALL] = 0;
+}
+
 void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
     const u64 W = F.W, n = ix.n;
     // CSR of E over the id range (duplicates kept, as the hash index keeps them)
@@ -689,23 +743,27 @@ void tc_rows_setup(dlg_ctx *ctx, const Index &ix, RowMap &F, TcRows &tr) {
     tr.abeg.ensure(nh ? nh : 1);
     tr.alen.ensure(nh ? nh : 1);
     tr.aoff.ensure(nh ? nh : 1);
-    tr.counters.ensure(TR_PART);
+    tr.counters.ensure(TR_ST_WORDS);
     tr.nd = nu;
     tr.nact = nh;
     tr.rows = 0;
     tr.wcap = 0;
     for (const auto &b : F.blocks) tr.wcap = std::max<u64>(tr.wcap, b.W);
     if (!nu) return;
-    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 20.0 * nu, tc_rows_init_k, sgrid(ctx, nu), ctx->block_size, 0,
+    DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * nu, tc_rows_init_k, sgrid(ctx, nu), ctx->block_size, 0,
                (const u64 *)uniq.p, nu, (const u32 *)heads.p, nh, tr.dy.p, tr.ax.p, tr.abeg.p, tr.alen.p,
                F.words.p, (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo);
     DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * nu, tc_rows_segrows_k, sgrid(ctx, nh), ctx->block_size, 0,
                (const u32 *)tr.dy.p, (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, nh, (const u64 *)tr.csr_off.p,
                F.lo, tr.aoff.p);
-    scan_u64_async(ctx, tr.aoff.p, nh, tr.aoff.p, tr.counters.p + 1);
+    scan_u64_async(ctx, tr.aoff.p, nh, tr.aoff.p, tr.counters.p + TR_ST_SPAN);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 0.0, tc_rows_state_k, 1, 64, 0, tr.counters.p, nu, nh);
     u64 h[2];
     ctx->read_scalars(tr.counters.p, 2, h);
     tr.rows = h[1];
+    // the active list never lengthens: its buffers are sized once
+    for (DBuf<u32> *b : {&tr.rcnt, &tr.nx, &tr.nlen}) b->ensure(nh);
+    for (DBuf<u64> *b : {&tr.rbound, &tr.nbeg, &tr.noff}) b->ensure(nh);
 }
 
 constexpr u64 TR_LONG_SEGMENTS = 64;   // average probes per segment above which the big table wins
@@ -716,12 +774,81 @@ static unsigned launch_rows(dlg_ctx *ctx, RowMap &F, TcRows &tr) {
     tr.bpart.ensure(TR_PART * (u64)g);
     tr.brange.ensure((u64)g + 1);
     DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, (tc_rows_k<LOG, ELL>), g, TR_T, 0, (const u32 *)tr.dy.p,
-               (const u32 *)tr.ax.p, (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, (const u64 *)tr.aoff.p, tr.nact,
-               tr.rows, (const u64 *)tr.csr_off.p, (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p,
+               (const u32 *)tr.ax.p, (const u64 *)tr.abeg.p, (const u32 *)tr.alen.p, (const u64 *)tr.aoff.p,
+               tr.counters.p, tr.ny.cap, (const u64 *)tr.csr_off.p, (const u32 *)tr.csr_dst.p, (const u32 *)tr.deg.p,
                (const uint4 *)tr.ell.p, tr.dmax, F.words.p,
                (const u64 *)F.row_base.p, (const u32 *)F.col_lo.p, F.lo, tr.wcap, tr.ny.p, tr.rcnt.p, tr.rbound.p,
                tr.bpart.p, tr.brange.p);
     return g;
+}
+
+static void tc_rows_swap(TcRows &tr) {
+    tr.dy.swap(tr.ny);
+    tr.ax.swap(tr.nx);
+    tr.abeg.swap(tr.nbeg);
+    tr.alen.swap(tr.nlen);
+    tr.aoff.swap(tr.noff);
+}
+
+// One round queued: the row kernel and the compaction, whose totals go to
+// the mapped slot of the returned number.  Sizes come from the last round
+// the host has read (the rounds queued since can only shorten the list);
+// the round itself reads its own from the device state.  The buffers of Δ
+// and of the list swap for the next round.  The output buffer grows to the
+// last known span only when no round is queued (`may_grow`): behind a queued
+// round it is that round's input, which a stall must find in place again.
+static u64 tc_rows_enqueue(dlg_ctx *ctx, RowMap &F, TcRows &tr, bool may_grow) {
+    const u64 na = tr.nact ? tr.nact : 1;
+    if (may_grow || !tr.ny.p) tr.ny.ensure(tr.rows ? tr.rows : 1);   // a larger span stalls the round
+    const bool big = tr.nd >= TR_LONG_SEGMENTS * na;   // long segments: big tables; short ones: more waves
+    unsigned g;
+    if (tr.ell.p) g = big ? launch_rows<10, true>(ctx, F, tr) : launch_rows<9, true>(ctx, F, tr);
+    else g = big ? launch_rows<10, false>(ctx, F, tr) : launch_rows<9, false>(ctx, F, tr);
+    const u64 seq = ++ctx->mapped_seq;
+    DLG_LAUNCH(ctx, DLG_K_MISC, 48.0 * na, tc_rows_compact_k, g, TA_T, 0, (const u32 *)tr.ax.p,
+               (const u64 *)tr.aoff.p, (const u32 *)tr.rcnt.p, (const u64 *)tr.rbound.p, (const u64 *)tr.bpart.p,
+               (const u64 *)tr.brange.p, tr.nx.p, tr.nbeg.p, tr.nlen.p, tr.noff.p, tr.counters.p,
+               ctx->mapped_dev(seq), seq);
+    tc_rows_swap(tr);
+    return seq;
+}
+
+// The totals of round `seq` (waits for them).  A stalled round did nothing:
+// the rounds queued after it did nothing either (the flag holds), so after
+// they drain the buffers swap back to the stalled round's (no buffer was
+// replaced while they were queued), its output buffer grows, the flag
+// clears and the caller queues again from there.
+// Returns false for a stalled round.
+static bool tc_rows_collect(dlg_ctx *ctx, TcRows &tr, u64 seq, size_t queued_after, u64 *h) {
+    ctx->wait_mapped(seq);
+    const volatile u64 *m = ctx->mapped_host(seq);
+    for (int q = 0; q <= TR_PART; q++) h[q] = m[q];
+    if (h[TR_PART] != TR_STALLED) return true;
+    ctx->sync();
+    if ((queued_after + 1) % 2) tc_rows_swap(tr);
+    tr.ny.ensure(std::max<u64>(h[TR_ST_SPAN] + h[TR_ST_SPAN] / 4, 2 * tr.ny.cap));
+    DLG_LAUNCH(ctx, DLG_K_MISC, 0.0, tc_rows_unstall_k, 1, 64, 0, tr.counters.p);
+    return false;
+}
+
+// Book-keeping of a round the host has read: returns |N|.
+static u64 tc_rows_account(dlg_ctx *ctx, TcRows &tr, const u64 *h, u64 nd_in, u64 *raw) {
+    if (h[TR_PART] == TR_IDLE) {   // no ends or no join rows: nothing new
+        tr.nd = 0;
+        return 0;
+    }
+    tr.nd = h[TR_ST_ND];
+    tr.rows = h[TR_ST_SPAN];
+    tr.nact = h[TR_ST_NACT];
+    if (raw) *raw += h[TR_ST_RAW];
+    // algorithmic bytes, added once the round's join rows are known: 4 B per
+    // join row (its matched end) + 20 B per probe (its end, CSR offset,
+    // degree) + 4 B per new end (written as the next round's Δ) + 4 B per
+    // word of F read and 4 B per word of F written (the set difference and
+    // merge of the bitmap layout)
+    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT,
+                        4.0 * h[TR_ST_RAW] + 20.0 * nd_in + 4.0 * h[TR_ST_ND] + 4.0 * h[4] + 4.0 * h[5]);
+    return h[TR_ST_ND];
 }
 
 u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
@@ -729,49 +856,41 @@ u64 tc_rows_round(dlg_ctx *ctx, RowMap &F, TcRows &tr, u64 *raw) {
         tr.nd = 0;
         return 0;
     }
-    const u64 na = tr.nact;
-    tr.ny.ensure(tr.rows);
-    tr.rcnt.ensure(na);
-    tr.rbound.ensure(na);
-    // algorithmic bytes, added once the round's join rows are known: 4 B per
-    // join row (its matched end) + 20 B per probe (its end, CSR offset,
-    // degree) + 4 B per new end (written as the next round's Δ) + 4 B per
-    // word of F read and 4 B per word of F written (the set difference and
-    // merge of the bitmap layout)
-    const u64 nd0 = tr.nd;
-    const bool big = tr.nd >= TR_LONG_SEGMENTS * na;   // long segments: big tables; short ones: more waves
-    unsigned g;
-    if (tr.ell.p) g = big ? launch_rows<10, true>(ctx, F, tr) : launch_rows<9, true>(ctx, F, tr);
-    else g = big ? launch_rows<10, false>(ctx, F, tr) : launch_rows<9, false>(ctx, F, tr);
-    // the next active list; its ends sit at this round's output offsets
-    tr.nx.ensure(na);
-    tr.nbeg.ensure(na);
-    tr.nlen.ensure(na);
-    tr.noff.ensure(na);
-    const u64 seq = ++ctx->mapped_seq;
-    DLG_LAUNCH(ctx, DLG_K_MISC, 48.0 * na, tc_rows_compact_k, g, TA_T, 0, (const u32 *)tr.ax.p,
-               (const u64 *)tr.aoff.p, (const u32 *)tr.rcnt.p, (const u64 *)tr.rbound.p, (const u64 *)tr.bpart.p,
-               (const u64 *)tr.brange.p, tr.nx.p, tr.nbeg.p, tr.nlen.p, tr.noff.p, tr.counters.p, ctx->d_mapped,
-               seq);
-    // new ends, next bound total, join rows, next list length: written by the
-    // compaction into mapped host memory before the round's number.  The
-    // host goes on while the compaction's other blocks may still run: none
-    // of the buffers they use can grow (the list never lengthens), and the
-    // next launches queue behind them on the stream.
-    ctx->wait_mapped(seq);
-    u64 h[TR_PART];
-    for (int q = 0; q < TR_PART; q++) h[q] = ((volatile u64 *)ctx->h_mapped)[q];
-    tr.dy.swap(tr.ny);
-    tr.ax.swap(tr.nx);
-    tr.abeg.swap(tr.nbeg);
-    tr.alen.swap(tr.nlen);
-    tr.aoff.swap(tr.noff);
-    tr.nd = h[0];
-    tr.rows = h[1];
-    tr.nact = h[3];
-    if (raw) *raw += h[2];
-    ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[2] + 20.0 * nd0 + 4.0 * h[0] + 4.0 * h[4] + 4.0 * h[5]);
-    return h[0];
+    const u64 nd_in = tr.nd;
+    u64 h[TR_PART + 1];
+    while (!tc_rows_collect(ctx, tr, tc_rows_enqueue(ctx, F, tr, true), 0, h)) {
+    }
+    return tc_rows_account(ctx, tr, h, nd_in, raw);
+}
+
+int tc_rows_run(dlg_ctx *ctx, RowMap &F, TcRows &tr, int max_it, int queue, u64 *total, u64 *raw) {
+    int it = 0;
+    const int depth = queue > 0 ? std::min(queue, (int)dlg_ctx::MAPPED_SLOTS) : TR_QUEUE;
+    if (tr.nd == 0 || tr.rows == 0) {   // one round that finds nothing
+        tr.nd = 0;
+        return 1;
+    }
+    std::deque<u64> q;   // numbers of the queued rounds, oldest first
+    u64 nd_in = tr.nd;
+    while (true) {
+        while ((int)q.size() < depth && (max_it <= 0 || it + (int)q.size() < max_it))
+            q.push_back(tc_rows_enqueue(ctx, F, tr, q.empty()));
+        u64 h[TR_PART + 1];
+        if (!tc_rows_collect(ctx, tr, q.front(), q.size() - 1, h)) {
+            q.clear();
+            continue;
+        }
+        q.pop_front();
+        it++;
+        const u64 nn = tc_rows_account(ctx, tr, h, nd_in, raw);
+        nd_in = nn;
+        *total += nn;
+        if (nn == 0 || (max_it > 0 && it >= max_it)) {
+            // the rounds still queued find no ends and do nothing; tr keeps this round's buffers
+            if (q.size() % 2) tc_rows_swap(tr);
+            return it;
+        }
+    }
 }
 
 u64 rowmap_bits(u32 lo, u32 hi, const std::vector<u32> *starts) {

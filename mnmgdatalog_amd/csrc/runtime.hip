@@ -202,7 +202,7 @@ void dlg_ctx::prof_flush() {
 void dlg_ctx::sync() { HIPCHK(hipStreamSynchronize(stream)); }
 
 void dlg_ctx::wait_mapped(uint64_t seq) {
-    volatile uint64_t *w = h_mapped + MAPPED_SEQ;
+    volatile uint64_t *w = mapped_host(seq) + MAPPED_SEQ;
     // Spin for about one long round (the next launch goes out the moment the
     // number lands), then yield the core between checks so that other host
     // threads (the served form's slots and admission) keep running.
@@ -258,9 +258,9 @@ dlg_status dlg_ctx_create(int device, dlg_ctx **out) {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->pool = new Pool(device);
         HIPCHK(hipHostMalloc((void **)&c->h_scalars, 64 * sizeof(uint64_t), hipHostMallocDefault));
-        HIPCHK(hipHostMalloc((void **)&c->h_mapped, (dlg_ctx::MAPPED_SEQ + 1) * sizeof(uint64_t),
-                             hipHostMallocMapped));
-        std::memset(c->h_mapped, 0, (dlg_ctx::MAPPED_SEQ + 1) * sizeof(uint64_t));   // no sequence number yet
+        const size_t mw = (size_t)dlg_ctx::MAPPED_WORDS * dlg_ctx::MAPPED_SLOTS;
+        HIPCHK(hipHostMalloc((void **)&c->h_mapped, mw * sizeof(uint64_t), hipHostMallocMapped));
+        std::memset(c->h_mapped, 0, mw * sizeof(uint64_t));   // no sequence number yet
         HIPCHK(hipHostGetDevicePointer((void **)&c->d_mapped, c->h_mapped, 0));
         noalloc_init(device);
     } catch (const Error &e) {

@@ -362,6 +362,49 @@ def test_wcc_round_forms_truncated(ctx):
             assert np.array_equal(r["rows"], runs[0]["rows"])
 
 
+@pytest.mark.parametrize("name", ["data_7035.bin", "data_23874.bin"])
+def test_tc_queued_rounds_agree(ctx, orc, name):
+    """Rounds queued ahead of the host (depths 2, 4 and 8; growing outputs stall rounds, which
+    are queued again) give the oracle's relation, iteration count and join rows, as one round
+    at a time does."""
+    rows = read_bin(name)
+    want = orc.tc(rows)
+    for q in (1, 2, 0, 8):
+        assert_same(ctx.run("tc", rows, queue=q), want)
+
+
+@pytest.mark.parametrize("name,its,size", [
+    ("data_223001.bin", 287, 80_498_014),   # logs/small-merged-job-output.txt:108 (SF.cedge)
+    ("data_163734.bin", 188, 156_120_489)])  # rough/compares.md:19 (fe_body)
+def test_tc_queued_rounds_published(ctx, name, its, size):
+    """Long closures, where outputs grow over many rounds (and queued rounds stall): every
+    queue depth gives the published size and round count, and the same relation (digest)."""
+    rows = read_bin(name)
+    runs = [ctx.run("tc", rows, keep_output=False, check_output=1, queue=q) for q in (1, 0, 8)]
+    for r in runs:
+        assert (r["iterations"], r["output_size"]) == (its, size)
+        assert (r["output_sum"], r["output_order_breaks"]) == (runs[0]["output_sum"], 0)
+        assert r["raw_join_rows"] == runs[0]["raw_join_rows"]
+
+
+def test_tc_queued_rounds_truncated(ctx):
+    """Stopped after k rounds, no queued round past the k-th changes the relation."""
+    rows = read_bin("data_165435.bin")
+    for k in (1, 2, 5, 9):
+        one = ctx.run("tc", rows, queue=1, max_iterations=k)
+        for q in (0, 8):
+            r = ctx.run("tc", rows, queue=q, max_iterations=k)
+            assert r["iterations"] == one["iterations"] == k
+            assert r["raw_join_rows"] == one["raw_join_rows"]
+            assert np.array_equal(r["rows"], one["rows"])
+
+
+def test_tc_queue_out_of_range(ctx):
+    with pytest.raises(dlg.DlgError) as e:
+        ctx.run("tc", read_bin("dummy.bin"), queue=9)
+    assert e.value.status == dlg.DLG_INVALID_INPUT
+
+
 def test_wcc_relax_out_of_range(ctx):
     with pytest.raises(dlg.DlgError) as e:
         ctx.run("wcc", read_bin("dummy.bin"), relax=3)
