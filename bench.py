@@ -115,7 +115,24 @@ def host_info():
                     flags = line.split("=", 1)[1].strip()
     except OSError:
         pass
-    return {"model": model, "nproc": os.cpu_count(), "cpu_share": cpu_share(), "oracle_flags": flags}
+    lscpu = {}
+    try:   # lscpu's fields of the box's host (a child process; the GPU is not touched)
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Vendor ID", "CPU family", "Model", "Stepping", "Socket(s)", "Core(s) per socket",
+                     "Thread(s) per core", "CPU max MHz", "L3 cache"):
+                lscpu[k] = v
+            elif k == "Flags":
+                fl = set(v.split())
+                lscpu["vector_flags"] = sorted(f for f in fl if f in ("avx", "avx2", "avx512f", "avx512bw",
+                                                                       "avx512vl", "bmi2", "fma"))
+    except (OSError, ValueError, Exception):
+        pass
+    return {"model": model, "lscpu": lscpu, "nproc": os.cpu_count(), "cpu_share": cpu_share(),
+            "oracle_flags": flags}
 
 
 def cpu_leg(prog, rows, threads, k):

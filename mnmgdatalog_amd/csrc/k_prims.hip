@@ -565,25 +565,15 @@ __global__ __launch_bounds__(T) void os_pass_k(const u64 *in, u64 *out, u64 n, R
     __shared__ u32 sc[T / WAVE + 1];
     __shared__ u32 s_tile;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-    // The keys of tile blockIdx.x are loaded while the ticket is in flight (blocks
-    // mostly start in index order); a block whose ticket differs loads its own.
-    u32 tk = 0;
-    if (threadIdx.x == 0) tk = atomicAdd(ticket, 1u);
-    u32 tile = blockIdx.x;
-    u64 tile0 = (u64)tile * TILE;
-    u64 wbase = tile0 + (u64)wave * (WAVE * I);
-    u64 keys[I];
-    u32 pk[I];   // digit | (rank within the wave's digit run << 8)
-    bool full = os_load_tile<T, I>(in, n, tile0, wbase, lane, keys);
-    if (threadIdx.x == 0) s_tile = tk;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     for (int i = threadIdx.x; i < W * BINS; i += T) (&wcnt[0][0])[i] = 0;
     __syncthreads();
-    if (s_tile != tile) {   // block-uniform
-        tile = s_tile;
-        tile0 = (u64)tile * TILE;
-        wbase = tile0 + (u64)wave * (WAVE * I);
-        full = os_load_tile<T, I>(in, n, tile0, wbase, lane, keys);
-    }
+    const u32 tile = s_tile;
+    const u64 tile0 = (u64)tile * TILE;
+    const u64 wbase = tile0 + (u64)wave * (WAVE * I);
+    u64 keys[I];
+    u32 pk[I];   // digit | (rank within the wave's digit run << 8)
+    const bool full = os_load_tile<T, I>(in, n, tile0, wbase, lane, keys);
     // Lanes of equal digit by ballot matching: per bit, clr = -1 when the lane's
     // bit is clear, the ballot of clr == 0 is the lanes with it set, and the
     // lane keeps (ballot ^ clr) -- the lanes that agree with it on that bit.

@@ -799,7 +799,9 @@ static void tc_rows_swap(TcRows &tr) {
 // round it is that round's input, which a stall must find in place again.
 static u64 tc_rows_enqueue(dlg_ctx *ctx, RowMap &F, TcRows &tr, bool may_grow) {
     const u64 na = tr.nact ? tr.nact : 1;
-    if (may_grow || !tr.ny.p) tr.ny.ensure(tr.rows ? tr.rows : 1);   // a larger span stalls the round
+    // half again the last known span: the rounds queued behind it rarely outgrow it (a larger
+    // span stalls the round)
+    if (may_grow || !tr.ny.p) tr.ny.ensure(tr.rows ? tr.rows + tr.rows / 2 : 1);
     const bool big = tr.nd >= TR_LONG_SEGMENTS * na;   // long segments: big tables; short ones: more waves
     unsigned g;
     if (tr.ell.p) g = big ? launch_rows<10, true>(ctx, F, tr) : launch_rows<9, true>(ctx, F, tr);
@@ -816,8 +818,9 @@ static u64 tc_rows_enqueue(dlg_ctx *ctx, RowMap &F, TcRows &tr, bool may_grow) {
 // The totals of round `seq` (waits for them).  A stalled round did nothing:
 // the rounds queued after it did nothing either (the flag holds), so after
 // they drain the buffers swap back to the stalled round's (no buffer was
-// replaced while they were queued), its output buffer grows, the flag
-// clears and the caller queues again from there.
+// replaced while they were queued), its output buffer grows to twice the
+// span, its input buffer (Δ copied over) to the same size, the flag clears
+// and the caller queues again from there.
 // Returns false for a stalled round.
 static bool tc_rows_collect(dlg_ctx *ctx, TcRows &tr, u64 seq, size_t queued_after, u64 *h) {
     ctx->wait_mapped(seq);
@@ -826,7 +829,12 @@ static bool tc_rows_collect(dlg_ctx *ctx, TcRows &tr, u64 seq, size_t queued_aft
     if (h[TR_PART] != TR_STALLED) return true;
     ctx->sync();
     if ((queued_after + 1) % 2) tc_rows_swap(tr);
-    tr.ny.ensure(std::max<u64>(h[TR_ST_SPAN] + h[TR_ST_SPAN] / 4, 2 * tr.ny.cap));
+    tr.ny.ensure(std::max<u64>(2 * h[TR_ST_SPAN], 2 * tr.ny.cap));
+    if (tr.dy.cap < tr.ny.cap) {   // the other buffer of the pair as large: Δ moves into a copy
+        DBuf<u32> big(ctx, tr.ny.cap);
+        HIPCHK(hipMemcpyAsync(big.p, tr.dy.p, sizeof(u32) * tr.dy.cap, hipMemcpyDeviceToDevice, ctx->stream));
+        tr.dy.swap(big);
+    }
     DLG_LAUNCH(ctx, DLG_K_MISC, 0.0, tc_rows_unstall_k, 1, 64, 0, tr.counters.p);
     return false;
 }
