@@ -1,6 +1,6 @@
 """Served form: requests of several programs and sizes in flight together
 (continuous batching, several streams, a resident graph) must each give the
-same result as the request run alone."""
+oracle's result, and the same result as the request run alone."""
 import numpy as np
 import pytest
 
@@ -46,7 +46,7 @@ def mixed_requests(seed):
     return reqs
 
 
-def test_served_results_equal_alone(ctx):
+def test_served_results_equal_alone(ctx, orc):
     reqs = mixed_requests(5)
     resident = dlg.gen_rmat(14, 60000, 0.57, 0.19, 0.19, 77, undirected=True)
     with dlg.Server(0, workers=4, small_rows=5000) as srv:
@@ -56,7 +56,8 @@ def test_served_results_equal_alone(ctx):
         got = [srv.wait(t) for t in tickets]
         ggot = [srv.wait(t) for t in gt]
     for (prog, rows), g in zip(reqs, got):
-        same(g, ctx.run(prog, rows), secondary=prog == "wcc")
+        same(g, getattr(orc, prog)(rows), secondary=prog == "wcc")    # the oracle
+        same(g, ctx.run(prog, rows), secondary=prog == "wcc")         # and the plain run
     alone = ctx.run("wcc", resident)
     for g in ggot:
         same(g, alone, secondary=True)
