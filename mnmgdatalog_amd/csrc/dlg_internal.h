@@ -295,6 +295,37 @@ __device__ __forceinline__ u64 upper_bound_u64(const u64 *a, u64 lo, u64 hi, u64
     return lo;
 }
 
+// The same searches by one whole wave (every lane calls with the same
+// arguments; every lane gets the result): 64 pivots per step, so about
+// log64(n) dependent loads instead of log2(n) -- for a block's range in a
+// large offsets array, found before the block can start (a 10M-entry array:
+// 4 loads instead of 23).  The first index in [lo, hi) whose element
+// satisfies `past` (false, then true, along the array), or hi.
+template <typename T, typename P>
+__device__ __forceinline__ u64 wave_partition_point(const T *a, u64 lo, u64 hi, P past) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    while (hi - lo > (u64)WAVE) {
+        const u64 step = (hi - lo + WAVE - 1) / WAVE;
+        const u64 p = lo + (u64)lane * step;
+        const int c = __popcll(__ballot(p < hi && !past(a[p])));   // pivots before the point: a prefix
+        if (c == 0) return lo;
+        const u64 top = lo + (u64)c * step;
+        lo = lo + (u64)(c - 1) * step + 1;
+        hi = top < hi ? top : hi;
+    }
+    const u64 p = lo + (u64)lane;
+    const u64 m = __ballot(p < hi && past(a[p]));
+    return m ? lo + (u64)(__ffsll((unsigned long long)m) - 1) : hi;
+}
+template <typename T>
+__device__ __forceinline__ u64 wave_upper_bound(const T *a, u64 lo, u64 hi, u64 x) {
+    return wave_partition_point(a, lo, hi, [x](T v) { return (u64)v > x; });
+}
+template <typename T>
+__device__ __forceinline__ u64 wave_lower_bound(const T *a, u64 lo, u64 hi, u64 x) {
+    return wave_partition_point(a, lo, hi, [x](T v) { return (u64)v >= x; });
+}
+
 // ------------------------------------------------------------- primitives (k_prims.hip)
 // Exclusive scan of u32 counts into u64 offsets; returns the total (synchronises).
 u64 scan_u32_u64(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out);

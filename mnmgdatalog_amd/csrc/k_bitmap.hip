@@ -106,12 +106,15 @@ __global__ __launch_bounds__(BJ_T) void bm_join_insert_k(const u64 *probe, u64 n
     for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const u64 base = tile * BJ_TILE;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            u64 last = (base + BJ_TILE < total ? base + BJ_TILE : total) - 1;
-            u64 l = upper_bound_u64(offs, 0, np, base) - 1;
-            s_lo = l;
-            s_hi = upper_bound_u64(offs, l, np, last);
-            s_nused = 0;
+        if (threadIdx.x < WAVE) {   // the first wave finds the tile's probes
+            const u64 last = (base + BJ_TILE < total ? base + BJ_TILE : total) - 1;
+            const u64 l = wave_upper_bound(offs, 0, np, base) - 1;
+            const u64 h = wave_upper_bound(offs, l, np, last);
+            if (threadIdx.x == 0) {
+                s_lo = l;
+                s_hi = h;
+                s_nused = 0;
+            }
         }
         __syncthreads();
         const u64 plo = s_lo, phi = s_hi, span = phi - plo;

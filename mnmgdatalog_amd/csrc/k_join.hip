@@ -219,11 +219,14 @@ __global__ __launch_bounds__(EX_T) void join_expand_k(const u64 *probe, u64 np, 
     __shared__ u32 l_start[EX_LDS];
     __shared__ u32 l_pv[EX_LDS];
     const u64 base = (u64)blockIdx.x * EX_TILE;
-    if (threadIdx.x == 0) {
-        u64 last = (base + EX_TILE < total ? base + EX_TILE : total) - 1;
-        u64 lo = upper_bound_u64(offs, 0, np, base) - 1;
-        s_lo = lo;
-        s_hi = upper_bound_u64(offs, lo, np, last);
+    if (threadIdx.x < WAVE) {   // the first wave finds the block's probes
+        const u64 last = (base + EX_TILE < total ? base + EX_TILE : total) - 1;
+        const u64 lo = wave_upper_bound(offs, 0, np, base) - 1;
+        const u64 hi = wave_upper_bound(offs, lo, np, last);
+        if (threadIdx.x == 0) {
+            s_lo = lo;
+            s_hi = hi;
+        }
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;

@@ -817,11 +817,14 @@ __global__ __launch_bounds__(SEL_T) void select_count_k(const u64 *in, u64 n, co
     const u64 base = (u64)blockIdx.x * SEL_TILE;
     constexpr bool NEED_OTHER = (MODE == SEL_UNIQUE_NOT_IN || MODE == SEL_NOT_IN);
     if (NEED_OTHER) {
-        if (threadIdx.x == 0) {
-            u64 last = base + SEL_TILE - 1 < n ? base + SEL_TILE - 1 : n - 1;
-            u64 lo = lower_bound_u64(other, 0, no, in[base]);
-            s_lo = lo;
-            s_hi = upper_bound_u64(other, lo, no, in[last]);
+        if (threadIdx.x < WAVE) {   // the first wave finds the tile's range of `other`
+            const u64 last = base + SEL_TILE - 1 < n ? base + SEL_TILE - 1 : n - 1;
+            const u64 lo = wave_lower_bound(other, 0, no, in[base]);
+            const u64 hi = wave_upper_bound(other, lo, no, in[last]);
+            if (threadIdx.x == 0) {
+                s_lo = lo;
+                s_hi = hi;
+            }
         }
         __syncthreads();
     }
@@ -1020,18 +1023,20 @@ __global__ __launch_bounds__(MG_T) void merge_side_k(const u64 *src, u64 ns, con
                                                      u64 *out) {
     __shared__ u64 s_lo, s_hi;
     const u64 base = (u64)blockIdx.x * MG_TILE;
-    if (threadIdx.x == 0) {
-        u64 last = base + MG_TILE - 1 < ns ? base + MG_TILE - 1 : ns - 1;
+    if (threadIdx.x < WAVE) {   // the first wave finds the tile's range of the other side
+        const u64 last = base + MG_TILE - 1 < ns ? base + MG_TILE - 1 : ns - 1;
         u64 lo, hi;
         if (SIDE_A) {
-            lo = lower_bound_u64(oth, 0, no, src[base]);
-            hi = lower_bound_u64(oth, lo, no, src[last]);
+            lo = wave_lower_bound(oth, 0, no, src[base]);
+            hi = wave_lower_bound(oth, lo, no, src[last]);
         } else {
-            lo = upper_bound_u64(oth, 0, no, src[base]);
-            hi = upper_bound_u64(oth, lo, no, src[last]);
+            lo = wave_upper_bound(oth, 0, no, src[base]);
+            hi = wave_upper_bound(oth, lo, no, src[last]);
         }
-        s_lo = lo;
-        s_hi = hi;
+        if (threadIdx.x == 0) {
+            s_lo = lo;
+            s_hi = hi;
+        }
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;

@@ -99,11 +99,14 @@ __global__ __launch_bounds__(WR_T) void wcc_relax_k(const u32 *flabel, u64 nf, c
     __shared__ u64 s_lo, s_hi;
     __shared__ u32 s_cnt[WR_I * WR_T / WAVE + 1];
     const u64 base = (u64)blockIdx.x * WR_TILE;
-    if (threadIdx.x == 0) {
-        u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
-        u64 lo = upper_bound_u64(offs, 0, nf, base) - 1;
-        s_lo = lo;
-        s_hi = upper_bound_u64(offs, lo, nf, last);
+    if (threadIdx.x < WAVE) {   // the first wave finds the tile's frontier nodes
+        const u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
+        const u64 lo = wave_upper_bound(offs, 0, nf, base) - 1;
+        const u64 hi = wave_upper_bound(offs, lo, nf, last);
+        if (threadIdx.x == 0) {
+            s_lo = lo;
+            s_hi = hi;
+        }
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
@@ -180,11 +183,14 @@ __global__ __launch_bounds__(WP_T) void wcc_pull_k(const u32 *gstart, u64 V, u64
     __shared__ u64 s_lo, s_hi;
     const u64 base = (u64)blockIdx.x * WP_TILE;
     const int lane = threadIdx.x & (WAVE - 1);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < WAVE) {   // the first wave finds the tile's nodes
         const u64 last = (base + WP_TILE < ne ? base + WP_TILE : ne) - 1;
-        const u64 lo = upper_bound_u32(gstart, 0, V, base) - 1;
-        s_lo = lo;
-        s_hi = upper_bound_u32(gstart, lo, V, last);
+        const u64 lo = wave_upper_bound(gstart, 0, V, base) - 1;
+        const u64 hi = wave_upper_bound(gstart, lo, V, last);
+        if (threadIdx.x == 0) {
+            s_lo = lo;
+            s_hi = hi;
+        }
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
@@ -365,11 +371,14 @@ __global__ __launch_bounds__(WR_T) void wcc_tuple_relax_k(const u64 *delta, u64 
                                                           const u32 *nbr, u64 ncc, u64 *cc, u32 *bad) {
     __shared__ u64 s_lo, s_hi;
     const u64 base = (u64)blockIdx.x * WR_TILE;
-    if (threadIdx.x == 0) {
-        u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
-        u64 lo = upper_bound_u64(offs, 0, nd, base) - 1;
-        s_lo = lo;
-        s_hi = upper_bound_u64(offs, lo, nd, last);
+    if (threadIdx.x < WAVE) {   // the first wave finds the tile's probes
+        const u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
+        const u64 lo = wave_upper_bound(offs, 0, nd, base) - 1;
+        const u64 hi = wave_upper_bound(offs, lo, nd, last);
+        if (threadIdx.x == 0) {
+            s_lo = lo;
+            s_hi = hi;
+        }
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
@@ -548,11 +557,14 @@ __global__ __launch_bounds__(WR_T) void gw_relax_k(const u32 *flabel, const u32 
     const u64 base = (u64)blockIdx.x * WR_TILE;
     if (st->done || base >= total) return;   // block-uniform
     const u32 round = st->it + 1;
-    if (threadIdx.x == 0) {
-        u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
-        u64 lo = upper_bound_u64(offs, 0, nf, base) - 1;
-        s_lo = lo;
-        s_hi = upper_bound_u64(offs, lo, nf, last);
+    if (threadIdx.x < WAVE) {   // the first wave finds the tile's frontier nodes
+        const u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
+        const u64 lo = wave_upper_bound(offs, 0, nf, base) - 1;
+        const u64 hi = wave_upper_bound(offs, lo, nf, last);
+        if (threadIdx.x == 0) {
+            s_lo = lo;
+            s_hi = hi;
+        }
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
