@@ -307,6 +307,17 @@ __device__ __forceinline__ u64 tr_ell_rows(TrWave<LOG> &s, u32 &nused, const u32
     return cnt;
 }
 
+// The cost share a workgroup takes: workgroups go to the XCDs in turn
+// (workgroup b to XCD b % 8), so XCD x gets the x-th eighth of the shares --
+// a contiguous range of the active list, whose sources (and, in a locality
+// order, their ends and those ends' adjacency rows) sit together in that
+// XCD's L2.  A bijection of [0, G); the totals and ranges are kept by share.
+constexpr u32 TR_XCD = 8;
+__device__ __forceinline__ u32 tr_share(u32 b, u32 G) {
+    const u32 x = b % TR_XCD, q = G / TR_XCD, r = G % TR_XCD;
+    return x * q + (x < r ? x : r) + b / TR_XCD;
+}
+
 // One round over the active list (the sources with ends): entry i (source
 // ax[i], ends dy[abeg[i] ..][0, alen[i])) writes its new ends at
 // ny[aoff[i] ..] -- aoff is the exclusive scan of the entries' output
@@ -354,7 +365,7 @@ __global__ __launch_bounds__(TR_T, (tr_blocks_per_cu<LOG, ELL>())) void tc_rows_
     if (lane == 0) s.touched = s.changed = 0u;
     s.pad[lane] = 0u;
     if (threadIdx.x < 2) {   // the first entries of this workgroup's share and of the next one's
-        const u64 b = (u64)blockIdx.x + threadIdx.x, tot = span + TR_ENTRY_COST * nact;
+        const u64 b = (u64)tr_share(blockIdx.x, gridDim.x) + threadIdx.x, tot = span + TR_ENTRY_COST * nact;
         const u64 key = (b * tot + gridDim.x - 1) / gridDim.x;
         u64 l = 0, h = nact;
         while (l < h) {
@@ -524,11 +535,12 @@ __global__ __launch_bounds__(TR_T, (tr_blocks_per_cu<LOG, ELL>())) void tc_rows_
     if (threadIdx.x < TR_PART) {   // the workgroup's totals, for the compaction
         u64 t = 0;
         for (int w = 0; w < TR_W; w++) t += s_tot[threadIdx.x][w];
-        bpart[TR_PART * blockIdx.x + threadIdx.x] = t;
+        bpart[TR_PART * (u64)tr_share(blockIdx.x, gridDim.x) + threadIdx.x] = t;
     }
     if (threadIdx.x == 0) {   // and its range of entries
-        brange[blockIdx.x] = r0;
-        if (blockIdx.x == gridDim.x - 1) brange[gridDim.x] = nact;
+        const u32 sh = tr_share(blockIdx.x, gridDim.x);
+        brange[sh] = r0;
+        if (sh == gridDim.x - 1) brange[gridDim.x] = nact;
     }
 }
 
