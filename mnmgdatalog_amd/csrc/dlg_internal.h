@@ -338,13 +338,18 @@ void scan_u32_u64_with(dlg_ctx *ctx, const u32 *in, u64 n, u64 *out, u64 *bsum, 
 // Exclusive scan of u64 values (out may alias in); total left at d_total.
 void scan_u64_async(dlg_ctx *ctx, const u64 *in, u64 n, u64 *out, u64 *d_total);
 // Stable LSD radix sort of u64 keys; result in `data`.  `tmp` is scratch of >= n.
-void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits = 0);
+// key_only: a stable sort by the key half alone (equal keys keep their order).
+void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits = 0, bool key_only = false);
 // Same on a pool buffer: the sorted keys may end in the other buffer, in which
 // case `data` and `tmp` swap ownership (no copy).  `bounds` (min key, max key,
 // min value, max value of the u64 halves) may be given when known to cover
 // every element; otherwise a pass over the data computes them.
 void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits = 0,
-                const u32 *bounds = nullptr);
+                const u32 *bounds = nullptr, bool key_only = false);
+// Adjacent pairs out of order (d[i] > d[i+1]); 0 for sorted data.  Synchronises.
+u64 count_descents(dlg_ctx *ctx, const u64 *d, u64 n);
+// The same over int32 (a, b) rows in signed lexicographic order.
+u64 count_descents_rows(dlg_ctx *ctx, const int32_t *rows, u64 n);
 // Digest of tuples as output rows in `orient` (dlg_result.output_sum) and the
 // number of i with d[i] >= d[i + 1] (synchronises).
 // sub0 / sub1 are subtracted (mod 2^32) from the rows' two columns first: the
@@ -536,7 +541,8 @@ u64 select_unique_not_in_hash(dlg_ctx *ctx, const u64 *in, u64 n, const HashSet 
 void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out);
 void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out);
 void unpack_rows(dlg_ctx *ctx, const u64 *t, u64 n, int orient, int32_t *rows);
-void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out);  // 2n tuples: kv, vk
+// E into out[0, n) and E⁻¹ into out_rev[0, n) (default: out[n, 2n)), row order kept.
+void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out, u64 *out_rev = nullptr);  // 2n tuples: kv, vk
 void fill_u64(dlg_ctx *ctx, u64 *p, u64 n, u64 v);
 void fill_u32(dlg_ctx *ctx, u32 *p, u64 n, u32 v);
 void copy_u64(dlg_ctx *ctx, u64 *dst, const u64 *src, u64 n);

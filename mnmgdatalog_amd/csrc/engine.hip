@@ -516,14 +516,31 @@ void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
 }
 
 // ------------------------------------------------------------------ WCC
+// Edge lists of at least this many rows are checked for arriving sorted (one read).
+constexpr u64 WCC_PRESORTED_MIN = 1ull << 20;
+
 void wcc_graph_build(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
                      WccGraph &g) {
     pc.mark(DLG_PH_INIT);
-    DBuf<u64> sym(ctx, 2 * n), edge(ctx), tmp(ctx);
-    pack_sym_rows(ctx, d_rows, n, sym.p);                  // E ∪ E⁻¹ (wcc.cu:140-160)
+    const bool presorted = n >= WCC_PRESORTED_MIN && count_descents_rows(ctx, d_rows, n) == 0;
+    DBuf<u64> sym(ctx, presorted ? n : 2 * n), rev(ctx), edge(ctx), tmp(ctx);
+    if (presorted) rev.ensure(n);
+    pack_sym_rows(ctx, d_rows, n, sym.p, presorted ? rev.p : nullptr);   // E ∪ E⁻¹ (wcc.cu:140-160)
     pc.mark(DLG_PH_DEDUP);
-    g.ne = sort_unique_into(ctx, sym, 2 * n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
+    if (presorted) {
+        // E arrives sorted by (src, dst): E⁻¹ in that order lists each dst's sources in
+        // ascending order, so a stable sort by its key alone sorts it by (dst, src) --
+        // the key's passes only -- and one merge gives E ∪ E⁻¹ sorted, as the full sort would
+        radix_sort(ctx, rev, n, tmp, o.bits, nullptr, true);
+        tmp.ensure(2 * n);
+        merge(ctx, sym.p, n, rev.p, n, tmp.p);
+        edge.ensure(2 * n);
+        g.ne = select(ctx, SEL_UNIQUE, tmp.p, 2 * n, nullptr, 0, edge.p);
+    } else {
+        g.ne = sort_unique_into(ctx, sym, 2 * n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
+    }
     sym.release();
+    rev.release();
     tmp.release();
     pc.mark(DLG_PH_HASHTABLE);
     index_build(ctx, g.ix, edge.p, g.ne, o.lf, true, o.join == 1);

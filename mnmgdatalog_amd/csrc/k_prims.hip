@@ -61,14 +61,14 @@ __global__ void pack_k(const int2 *rows, u64 n, int orient, u64 *out) {
         out[i] = orient ? tmake(b, a) : tmake(a, b);
     }
 }
-__global__ void pack_sym_k(const int2 *rows, u64 n, u64 *out) {
+__global__ void pack_sym_k(const int2 *rows, u64 n, u64 *out, u64 *out_rev) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     u64 stride = (u64)gridDim.x * blockDim.x;
     for (; i < n; i += stride) {
         int2 r = rows[i];
         u32 a = (u32)r.x ^ 0x80000000u, b = (u32)r.y ^ 0x80000000u;
         out[i] = tmake(a, b);
-        out[n + i] = tmake(b, a);
+        out_rev[i] = tmake(b, a);
     }
 }
 __global__ void unpack_k(const u64 *t, u64 n, int orient, int2 *rows) {
@@ -86,10 +86,10 @@ void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out) {
     DLG_LAUNCH(ctx, DLG_K_PACK, 16.0 * n, pack_k, sgrid(ctx, n), ctx->block_size, 0,
                (const int2 *)rows, n, orient, out);
 }
-void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out) {
+void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out, u64 *out_rev) {
     if (!n) return;
     DLG_LAUNCH(ctx, DLG_K_PACK, 24.0 * n, pack_sym_k, sgrid(ctx, n), ctx->block_size, 0,
-               (const int2 *)rows, n, out);
+               (const int2 *)rows, n, out, out_rev ? out_rev : out + n);
 }
 void unpack_rows(dlg_ctx *ctx, const u64 *t, u64 n, int orient, int32_t *rows) {
     if (!n) return;
@@ -244,9 +244,11 @@ struct RadixKey {
     u32 shift;  // digit position in the compacted key
 };
 
+// vb = 0: the value half takes no bits -- every value equal (then lo - min_lo
+// is 0 anyway), or a sort by key alone (radix_sort's key_only).
 __device__ __forceinline__ u32 radix_digit(u64 x, const RadixKey &k, u32 mask) {
     u64 hi = (u64)(tkey(x) - k.min_hi);
-    u64 lo = (u64)(tval(x) - k.min_lo);
+    u64 lo = k.vb ? (u64)(tval(x) - k.min_lo) : 0ull;
     u64 c = (hi << k.vb) | lo;  // vb <= 32, (hi bits + vb) <= 64
     return (u32)(c >> k.shift) & mask;
 }
@@ -760,7 +762,8 @@ void rows_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u64 *sum, u64 *b
     *breaks = h[1];
 }
 
-void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits, const u32 *bounds) {
+void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits, const u32 *bounds,
+                bool key_only) {
     if (n < 2) return;
     if (n > 0xffffffffull) fail(DLG_OVERFLOW, "radix_sort: more than 2^32-1 tuples in one buffer");
     if (bits <= 0) bits = ctx->radix_bits;
@@ -772,7 +775,7 @@ void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits, 
     RadixKey key;
     key.min_hi = h[0];
     key.min_lo = h[2];
-    key.vb = (u32)bit_width(h[3] - h[2]);
+    key.vb = key_only ? 0u : (u32)bit_width(h[3] - h[2]);   // key_only: a stable sort by key alone
     key.shift = 0;
     int total_bits = (int)key.vb + bit_width(h[1] - h[0]);
     if (total_bits == 0) return;  // all equal
@@ -788,11 +791,11 @@ void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits, 
     if (src != data.p) data.swap(tmp);   // result is in the other buffer: swap ownership
 }
 
-void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits) {
+void radix_sort(dlg_ctx *ctx, u64 *data, u64 n, DBuf<u64> &tmp, int bits, bool key_only) {
     DBuf<u64> view(ctx);
     view.p = data;          // borrowed: released without returning it to the pool
     view.cap = n;
-    radix_sort(ctx, view, n, tmp, bits, nullptr);
+    radix_sort(ctx, view, n, tmp, bits, nullptr, key_only);
     if (view.p != data) {   // sorted copy ended in tmp's old block; bring it back
         copy_u64(ctx, data, view.p, n);
         tmp.swap(view);     // tmp owns that block again
@@ -1051,6 +1054,64 @@ __global__ __launch_bounds__(MG_T) void merge_side_k(const u64 *src, u64 ns, con
             out[i + rk] = x;
         }
     }
+}
+
+// Adjacent pairs out of order (d[i] > d[i+1]), counted per block.
+__global__ __launch_bounds__(256) void descents_k(const u64 *d, u64 n, unsigned long long *count) {
+    __shared__ u64 s_part[256 / WAVE];
+    u64 c = 0;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n; i += stride) c += d[i] > d[i + 1];
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0) s_part[threadIdx.x / WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 t = 0;
+        for (int w = 0; w < 256 / WAVE; w++) t += s_part[w];
+        if (t) atomicAdd(count, (unsigned long long)t);
+    }
+}
+
+// The same over int32 (a, b) rows in signed lexicographic order (the packed order).
+__global__ __launch_bounds__(256) void descents_rows_k(const int2 *r, u64 n, unsigned long long *count) {
+    __shared__ u64 s_part[256 / WAVE];
+    u64 c = 0;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n; i += stride) {
+        const int2 a = r[i], b = r[i + 1];
+        c += a.x > b.x || (a.x == b.x && a.y > b.y);
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0) s_part[threadIdx.x / WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 t = 0;
+        for (int w = 0; w < 256 / WAVE; w++) t += s_part[w];
+        if (t) atomicAdd(count, (unsigned long long)t);
+    }
+}
+
+u64 count_descents_rows(dlg_ctx *ctx, const int32_t *rows, u64 n) {
+    if (n < 2) return 0;
+    DBuf<u64> c(ctx, 1);
+    fill_u64(ctx, c.p, 1, 0ull);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, descents_rows_k, sgrid(ctx, n), 256, 0, (const int2 *)rows, n,
+               (unsigned long long *)c.p);
+    u64 h = 0;
+    ctx->read_scalars(c.p, 1, &h);
+    return h;
+}
+
+u64 count_descents(dlg_ctx *ctx, const u64 *d, u64 n) {
+    if (n < 2) return 0;
+    DBuf<u64> c(ctx, 1);
+    fill_u64(ctx, c.p, 1, 0ull);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, descents_k, sgrid(ctx, n), 256, 0, d, n, (unsigned long long *)c.p);
+    u64 h = 0;
+    ctx->read_scalars(c.p, 1, &h);
+    return h;
 }
 
 void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out) {

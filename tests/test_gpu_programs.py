@@ -405,6 +405,24 @@ def test_tc_queue_out_of_range(ctx):
     assert e.value.status == dlg.DLG_INVALID_INPUT
 
 
+def test_wcc_presorted_edges(ctx, orc):
+    """Above 2^20 rows an edge list that arrives sorted by (src, dst) builds its symmetric
+    graph by sorting only the reversed half (by key) and merging; shuffled, by the full sort.
+    Both give the oracle's labels, iterations and largest component."""
+    rows = dlg.gen_rmat(17, 1 << 20, 0.57, 0.19, 0.19, 31)            # sorted by (src, dst)
+    key = rows[:, 0].astype(np.int64) * (1 << 32) + rows[:, 1].astype(np.int64)
+    assert rows.shape[0] >= 1 << 20 and np.all(np.diff(key) >= 0)
+    want = orc.wcc(rows)
+    shuffled = rows[np.random.default_rng(3).permutation(rows.shape[0])]
+    for r in (rows, shuffled):
+        for strategy in (1, 0):
+            assert_same(ctx.run("wcc", r, strategy=strategy), want, secondary=True)
+    neg = rows.copy()
+    neg[:, 0] -= 70_000                                                # sorted, with negative ids
+    neg[:, 1] -= 70_000
+    assert_same(ctx.run("wcc", neg, strategy=1), orc.wcc(neg), secondary=True)
+
+
 def test_wcc_relax_out_of_range(ctx):
     with pytest.raises(dlg.DlgError) as e:
         ctx.run("wcc", read_bin("dummy.bin"), relax=3)
