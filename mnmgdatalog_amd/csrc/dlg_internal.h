@@ -301,21 +301,25 @@ __device__ __forceinline__ u64 upper_bound_u64(const u64 *a, u64 lo, u64 hi, u64
 // large offsets array, found before the block can start (a 10M-entry array:
 // 4 loads instead of 23).  The first index in [lo, hi) whose element
 // satisfies `past` (false, then true, along the array), or hi.
-template <typename T, typename P>
-__device__ __forceinline__ u64 wave_partition_point(const T *a, u64 lo, u64 hi, P past) {
+template <typename P>
+__device__ __forceinline__ u64 wave_partition_index(u64 lo, u64 hi, P past) {   // past(i), i in [lo, hi)
     const int lane = threadIdx.x & (WAVE - 1);
     while (hi - lo > (u64)WAVE) {
         const u64 step = (hi - lo + WAVE - 1) / WAVE;
         const u64 p = lo + (u64)lane * step;
-        const int c = __popcll(__ballot(p < hi && !past(a[p])));   // pivots before the point: a prefix
+        const int c = __popcll(__ballot(p < hi && !past(p)));   // pivots before the point: a prefix
         if (c == 0) return lo;
         const u64 top = lo + (u64)c * step;
         lo = lo + (u64)(c - 1) * step + 1;
         hi = top < hi ? top : hi;
     }
     const u64 p = lo + (u64)lane;
-    const u64 m = __ballot(p < hi && past(a[p]));
+    const u64 m = __ballot(p < hi && past(p));
     return m ? lo + (u64)(__ffsll((unsigned long long)m) - 1) : hi;
+}
+template <typename T, typename P>
+__device__ __forceinline__ u64 wave_partition_point(const T *a, u64 lo, u64 hi, P past) {
+    return wave_partition_index(lo, hi, [a, past](u64 i) { return past(a[i]); });
 }
 template <typename T>
 __device__ __forceinline__ u64 wave_upper_bound(const T *a, u64 lo, u64 hi, u64 x) {

@@ -364,15 +364,15 @@ __global__ __launch_bounds__(TR_T, (tr_blocks_per_cu<LOG, ELL>())) void tc_rows_
     }
     if (lane == 0) s.touched = s.changed = 0u;
     s.pad[lane] = 0u;
-    if (threadIdx.x < 2) {   // the first entries of this workgroup's share and of the next one's
-        const u64 b = (u64)tr_share(blockIdx.x, gridDim.x) + threadIdx.x, tot = span + TR_ENTRY_COST * nact;
-        const u64 key = (b * tot + gridDim.x - 1) / gridDim.x;
-        u64 l = 0, h = nact;
-        while (l < h) {
-            const u64 m = (l + h) >> 1;
-            if (aoff[m] + TR_ENTRY_COST * m < key) l = m + 1; else h = m;
+    if (threadIdx.x < WAVE) {   // the first wave finds this share's first entry and the next one's
+        const u64 tot = span + TR_ENTRY_COST * nact, sh = tr_share(blockIdx.x, gridDim.x);
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const u64 b = sh + (u64)e;
+            const u64 key = (b * tot + gridDim.x - 1) / gridDim.x;
+            const u64 l = wave_partition_index(0, nact, [aoff, key](u64 m) { return aoff[m] + TR_ENTRY_COST * m >= key; });
+            if (threadIdx.x == 0) s_rng[e] = b >= gridDim.x ? nact : l;
         }
-        s_rng[threadIdx.x] = b >= gridDim.x ? nact : l;
     }
     if (threadIdx.x == 0) s_next = 0;
     __syncthreads();
