@@ -316,7 +316,8 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
             // tc_nl.cu:182-185 sorts and deduplicates the edge relation it probes
             DBuf<u64> ue(ctx);
             u64 ne = sort_unique_into(ctx, edge, n, tmp, ue, SEL_UNIQUE, nullptr, 0, o.bits, ib);
-            index_build(ctx, ix, ue.p, ne, o.lf, true, true);
+            ix.rows.swap(ue);   // the index keeps the sorted rows in place
+            index_build(ctx, ix, ix.rows.p, ne, o.lf, true, true);
         } else {
             index_build(ctx, ix, edge.p, n, o.lf, false);  // duplicates kept, as tc.cu:210
         }
@@ -504,8 +505,8 @@ void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     a.release();
     pc.mark(DLG_PH_HASHTABLE);
     Index ix(ctx);
-    index_build(ctx, ix, R.p, nr, o.lf, true, o.join == 1);
-    R.release();
+    ix.rows.swap(R);
+    index_build(ctx, ix, ix.rows.p, nr, o.lf, true, o.join == 1);
     pc.mark(DLG_PH_JOIN);
     JoinScratch js(ctx);
     u64 nj = join_into(ctx, ix, Rr.p, nrr, DLG_PROJ_HV_PV, js, J);   // (c, a)
@@ -543,8 +544,8 @@ void wcc_graph_build(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, 
     rev.release();
     tmp.release();
     pc.mark(DLG_PH_HASHTABLE);
-    index_build(ctx, g.ix, edge.p, g.ne, o.lf, true, o.join == 1);
-    edge.release();
+    g.ix.rows.swap(edge);   // the index keeps the sorted edges in place
+    index_build(ctx, g.ix, g.ix.rows.p, g.ne, o.lf, true, o.join == 1);
     g.V = g.ix.groups;
     g.nbr.ensure(g.ne ? g.ne : 1);
     index_lookup_values(ctx, g.ix, g.nbr.p);               // neighbour dense ids

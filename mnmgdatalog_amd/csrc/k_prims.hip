@@ -1017,42 +1017,73 @@ u64 select_unique_not_in_runs(dlg_ctx *ctx, const u64 *in, u64 n, const RunSet &
 }
 
 // =============================================================== merge
-constexpr int MG_T = 256, MG_I = 8, MG_TILE = MG_T * MG_I;
+// Merge path: output tile t of MP_TILE elements takes a[split[t], split[t+1])
+// and the matching stretch of b; the split of diagonal d is the number of
+// a-elements among the first d outputs of the stable merge (a first on ties).
+constexpr int MP_T = 256, MP_I = 8, MP_TILE = MP_T * MP_I, MP_SPLIT_T = 256;
 
-// Scatter one side of a merge: element x of `src` goes to i + rank(x in oth),
-// rank = #oth < x for side a (a first on ties) and #oth <= x for side b.
-template <bool SIDE_A>
-__global__ __launch_bounds__(MG_T) void merge_side_k(const u64 *src, u64 ns, const u64 *oth, u64 no,
-                                                     u64 *out) {
-    __shared__ u64 s_lo, s_hi;
-    const u64 base = (u64)blockIdx.x * MG_TILE;
-    if (threadIdx.x < WAVE) {   // the first wave finds the tile's range of the other side
-        const u64 last = base + MG_TILE - 1 < ns ? base + MG_TILE - 1 : ns - 1;
-        u64 lo, hi;
-        if (SIDE_A) {
-            lo = wave_lower_bound(oth, 0, no, src[base]);
-            hi = wave_lower_bound(oth, lo, no, src[last]);
-        } else {
-            lo = wave_upper_bound(oth, 0, no, src[base]);
-            hi = wave_upper_bound(oth, lo, no, src[last]);
+// One thread per diagonal d = t * MP_TILE, t in [0, tiles]: a plain binary search
+// (a wave's 64 pivots per step would read 64 lines per step for every diagonal).
+__global__ __launch_bounds__(MP_SPLIT_T) void merge_split_k(const u64 *a, u64 na, const u64 *b, u64 nb,
+                                                            u64 tiles, u64 *split) {
+    const u64 t = (u64)blockIdx.x * MP_SPLIT_T + threadIdx.x;
+    if (t > tiles) return;
+    const u64 n = na + nb, d = t * MP_TILE < n ? t * MP_TILE : n;
+    u64 lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+    while (lo < hi) {   // a[m] > b[d-1-m] turns true at the split; d-1-m stays in [0, nb)
+        const u64 m = (lo + hi) >> 1;
+        if (a[m] > b[d - 1 - m]) hi = m; else lo = m + 1;
+    }
+    split[t] = lo;
+}
+
+__global__ __launch_bounds__(MP_T) void merge_path_k(const u64 *a, u64 na, const u64 *b, u64 nb,
+                                                     const u64 *split, u64 *out) {
+    __shared__ u64 s[MP_TILE];
+    const u64 n = na + nb, d0 = (u64)blockIdx.x * MP_TILE;
+    const int cnt = (int)(n - d0 < (u64)MP_TILE ? n - d0 : (u64)MP_TILE);
+    const u64 i0 = split[blockIdx.x], j0 = d0 - i0;
+    const int ca = (int)(split[blockIdx.x + 1] - i0), cb = cnt - ca;
+#pragma unroll
+    for (int r = 0; r < MP_I; r++) {   // a's stretch, then b's, side by side in the tile
+        const int k = r * MP_T + (int)threadIdx.x;
+        if (k < cnt) s[k] = k < ca ? a[i0 + (u64)k] : b[j0 + (u64)(k - ca)];
+    }
+    __syncthreads();
+    const int dt = (int)threadIdx.x * MP_I;
+    u64 v[MP_I];
+    if (dt < cnt) {
+        int lo = dt > cb ? dt - cb : 0, hi = dt < ca ? dt : ca;
+        while (lo < hi) {   // this thread's split inside the tile
+            const int m = (lo + hi) >> 1;
+            if (s[m] <= s[ca + dt - 1 - m]) lo = m + 1; else hi = m;
         }
-        if (threadIdx.x == 0) {
-            s_lo = lo;
-            s_hi = hi;
+        int ia = lo, ib = dt - lo;
+        u64 va = ia < ca ? s[ia] : 0, vb = ib < cb ? s[ca + ib] : 0;
+#pragma unroll
+        for (int r = 0; r < MP_I; r++) {
+            const bool take_a = ia < ca && (ib >= cb || va <= vb);
+            v[r] = take_a ? va : vb;
+            if (take_a) {
+                ia++;
+                va = ia < ca ? s[ia] : 0;
+            } else {
+                ib++;
+                vb = ib < cb ? s[ca + ib] : 0;
+            }
         }
     }
     __syncthreads();
-    const u64 lo = s_lo, hi = s_hi;
+    if (dt < cnt) {
 #pragma unroll
-    for (int r = 0; r < MG_I; r++) {
-        u64 i = base + (u64)r * MG_T + threadIdx.x;
-        if (i < ns) {
-            u64 x = src[i];
-            u64 rk = lo == hi ? lo
-                              : (SIDE_A ? lower_bound_u64(oth, lo, hi, x)
-                                        : upper_bound_u64(oth, lo, hi, x));
-            out[i + rk] = x;
-        }
+        for (int r = 0; r < MP_I; r++)
+            if (dt + r < cnt) s[dt + r] = v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < MP_I; r++) {
+        const int k = r * MP_T + (int)threadIdx.x;
+        if (k < cnt) out[d0 + (u64)k] = s[k];
     }
 }
 
@@ -1115,12 +1146,14 @@ u64 count_descents(dlg_ctx *ctx, const u64 *d, u64 n) {
 }
 
 void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out) {
-    if (na)
-        DLG_LAUNCH(ctx, DLG_K_MERGE, 16.0 * na, merge_side_k<true>, grid_for(na, MG_TILE), MG_T, 0,
-                   a, na, b, nb, out);
-    if (nb)
-        DLG_LAUNCH(ctx, DLG_K_MERGE, 16.0 * nb, merge_side_k<false>, grid_for(nb, MG_TILE), MG_T,
-                   0, b, nb, a, na, out);
+    const u64 n = na + nb;
+    if (n == 0) return;
+    const u64 tiles = (n + MP_TILE - 1) / MP_TILE;
+    DBuf<u64> split(ctx, tiles + 1);
+    DLG_LAUNCH(ctx, DLG_K_MERGE, 16.0 * (tiles + 1), merge_split_k,
+               grid_for(tiles + 1, MP_SPLIT_T), MP_SPLIT_T, 0, a, na, b, nb, tiles, split.p);
+    DLG_LAUNCH(ctx, DLG_K_MERGE, 16.0 * n, merge_path_k, (unsigned)tiles, MP_T, 0, a, na, b, nb,
+               (const u64 *)split.p, out);
 }
 
 }  // namespace dlg

@@ -104,6 +104,19 @@ def test_set_difference_and_merge(ctx, na, nb, overlap):
     assert np.array_equal(m, np.sort(np.concatenate([a, b])))
 
 
+@pytest.mark.parametrize("na,nb,vals", [(2048, 0, 5), (2047, 1, 5), (1, 4095, 2), (3000, 5000, 3),
+                                       (100_000, 70_000, 40), (1_500_000, 600_000, 1 << 40)])
+def test_merge_repeated_values(ctx, na, nb, vals):
+    """Merges of sorted inputs with long runs of equal values crossing tile edges,
+    including the largest u64 value."""
+    rng = np.random.default_rng(na * 7 + nb)
+    top = np.uint64(0xFFFFFFFFFFFFFFFF)
+    a = np.sort(top - rng.integers(0, vals, size=na, dtype=np.uint64))
+    b = np.sort(top - rng.integers(0, vals, size=nb, dtype=np.uint64))
+    m = ctx.merge(a, b)
+    assert np.array_equal(m, np.sort(np.concatenate([a, b])))
+
+
 @pytest.mark.parametrize("n", [1, 100, 2048, 2049, 1_000_000, 5_000_001])
 def test_exclusive_scan(ctx, n):
     rng = np.random.default_rng(n)
