@@ -407,7 +407,16 @@ struct Bitmap {
     bool tr = false;
     explicit Bitmap(dlg_ctx *c) : words(c), count(c) {}
 };
-void bitmap_init(dlg_ctx *ctx, Bitmap &bm, u32 lo, u32 hi, bool transposed);
+// Geometry for ids [lo, hi]; the map is cleared unless `clear` is false (its
+// padding always is).
+void bitmap_init(dlg_ctx *ctx, Bitmap &bm, u32 lo, u32 hi, bool transposed, bool clear = true);
+// The single join R(a, b), R(b, c) straight into a row-major map (row c, bit
+// a), from `rows` = R keyed by its second column, sorted, n rows, with dense
+// offsets over the map's id range (W + 1 entries).  Every row is written.
+constexpr u64 SJ_ROW_LDS_MAX = 64 * 1024;   // bytes of one map row in LDS
+void bitmap_self_join(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, Bitmap &bm);
+// The raw size of that join, sum over b of |{a}| x |{c}| (synchronises).
+u64 self_join_raw(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo);
 // probe ⋈ index, each row inserted; rows that set a clear bit go to `out`
 // (unordered).  Returns their number; *raw += |J|.
 u64 bitmap_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, int proj,

@@ -491,17 +491,56 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
 }
 
 // ------------------------------------------------------------------ single join
+// The map form (strategy 4, or automatic when the id range allows the bitmap
+// layout and the W*W-bit map is no larger than the raw join's tuples): every
+// output row built in LDS and read out of the map in tuple order, so the raw
+// join is never materialised or sorted.
+static bool sj_use_map(const Opts &o, u64 W, u64 raw) {
+    const u64 wpr = (W + 31) / 32;
+    if (o.strategy == 4) {
+        if (W > BITMAP_FORCED_MAX_W || 4 * wpr > SJ_ROW_LDS_MAX)
+            fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
+        return true;
+    }
+    return o.strategy == 0 && o.join == 0 && W <= BITMAP_MAX_W && 4 * W * wpr <= 8 * raw;
+}
+
 void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
                  FixpointOut &out) {
     pc.mark(DLG_PH_INIT);
     DBuf<u64> a(ctx, n), R(ctx), Rr(ctx), J(ctx), tmp(ctx);
+    pack_rows(ctx, d_rows, n, DLG_ORIENT_VK, a.p);   // (b, a) for each (a, b)
+    u32 ib[4];
+    id_bounds(ctx, a.p, n, ib);
+    pc.mark(DLG_PH_DEDUP);
+    u64 nrr = sort_unique_into(ctx, a, n, tmp, Rr, SEL_UNIQUE, nullptr, 0, o.bits, ib);
+    out.iterations = 1;
+    if (nrr == 0) return;
+    pc.mark(DLG_PH_HASHTABLE);
+    const u64 W = (u64)(ib[1] - ib[0]) + 1;
+    if (o.strategy == 4 || (o.strategy == 0 && o.join == 0 && W <= BITMAP_MAX_W)) {
+        Index rx(ctx);   // the (b, a) rows with dense offsets: b's run of a's
+        rx.rows.swap(Rr);
+        rx.n = nrr;
+        if (W <= BITMAP_FORCED_MAX_W) index_dense(ctx, rx, ib[0], W);
+        const u64 raw = rx.dense_w ? self_join_raw(ctx, rx.rows.p, nrr, rx.dense.p, ib[0]) : 0;
+        if (sj_use_map(o, W, raw)) {
+            pc.mark(DLG_PH_JOIN);
+            Bitmap F(ctx);
+            bitmap_init(ctx, F, ib[0], ib[1], false, false);
+            bitmap_self_join(ctx, rx.rows.p, nrr, rx.dense.p, F);   // row c: {a : (a, b), (b, c)}
+            out.raw += (i64)raw;
+            pc.mark(DLG_PH_DEDUP);
+            out.nfull = bitmap_drain(ctx, F, out.full, false, false);   // (c, a) in order
+            return;
+        }
+        if (o.strategy == 4) fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
+        Rr.swap(rx.rows);
+    }
+    pc.mark(DLG_PH_INIT);
     pack_rows(ctx, d_rows, n, DLG_ORIENT_KV, a.p);
     pc.mark(DLG_PH_DEDUP);
-    u64 nr = sort_unique_into(ctx, a, n, tmp, R, SEL_UNIQUE, nullptr, 0, o.bits);
-    pc.mark(DLG_PH_INIT);
-    pack_rows(ctx, d_rows, n, DLG_ORIENT_VK, a.p);
-    pc.mark(DLG_PH_DEDUP);
-    u64 nrr = sort_unique_into(ctx, a, n, tmp, Rr, SEL_UNIQUE, nullptr, 0, o.bits);
+    u64 nr = sort_unique_into(ctx, a, n, tmp, R, SEL_UNIQUE, nullptr, 0, o.bits, ib);
     a.release();
     pc.mark(DLG_PH_HASHTABLE);
     Index ix(ctx);
@@ -513,7 +552,6 @@ void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     out.raw += (i64)nj;
     pc.mark(DLG_PH_DEDUP);
     out.nfull = sort_unique_into(ctx, J, nj, tmp, out.full, SEL_UNIQUE, nullptr, 0, o.bits);
-    out.iterations = 1;
 }
 
 // ------------------------------------------------------------------ WCC

@@ -349,17 +349,20 @@ __global__ __launch_bounds__(BE_T) void bm_count_k(const u32 *words, u64 *bsum) 
 }
 
 // SWAP: write (column, row) instead of (row, column).  CLEAR: zero the words read.
-// Emission of a tile: per-thread popcounts -> block scan.  A sparse tile
-// (fewer set bits than words) is written thread by thread; a dense one gets a
-// position for every word (in LDS), then each wave walks its 512 words two at
-// a time, one lane per bit, so the set bits of a word pair are written to
-// consecutive addresses (coalesced).
+// Emission of a tile: per-thread popcounts -> block scan.  A tile of at most
+// 2 BE_STAGE set bits is expanded thread by thread (each thread's words give
+// a consecutive run of outputs) into an LDS stage of BE_STAGE tuples at a
+// time, which the block then writes out with whole-wave coalesced stores.  A
+// denser tile gets a position for every word (in LDS), then each wave walks
+// its 512 words two at a time, one lane per bit, so the set bits of a word
+// pair are written to consecutive addresses.
+constexpr int BE_STAGE = 4096;
 template <bool SWAP, bool CLEAR>
 __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u32 lo, u64 wpr, u64 *out) {
     __shared__ u64 sc[BE_T / WAVE + 1];
-    __shared__ u32 s_w[BE_TILE];
-    __shared__ u32 s_pos[BE_TILE];   // each word's first output, from the tile's first (at most 2^16 in a tile)
-    __shared__ u32 s_row[BE_TILE];   // each word's row
+    __shared__ u64 s_buf[BE_STAGE];   // the stage, or the dense walk's three word tables
+    static_assert(3 * BE_TILE * 4 <= BE_STAGE * 8, "word tables must fit the stage");
+    u32 *s_w = (u32 *)s_buf, *s_pos = s_w + BE_TILE, *s_row = s_pos + BE_TILE;
     const u64 tile = (u64)blockIdx.x * BE_TILE;
     const u64 base = tile + (u64)threadIdx.x * BE_W;
     u32 w[BE_W];
@@ -368,33 +371,49 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
 #pragma unroll
     for (int i = 0; i < BE_W; i++) c += __popc(w[i]);
     u64 tot;
-    u64 pos = block_exclusive_scan<u64>(c, sc, &tot) + bpre[blockIdx.x];
+    const u64 rel = block_exclusive_scan<u64>(c, sc, &tot);   // this thread's first output in the tile
+    const u64 out0 = bpre[blockIdx.x];
     if (CLEAR && c) {
         *(uint4 *)(words + base) = make_uint4(0u, 0u, 0u, 0u);
         *(uint4 *)(words + base + 4) = make_uint4(0u, 0u, 0u, 0u);
     }
-    if (tot < (u64)BE_TILE) {   // sparse tile (uniform): each thread writes its own bits
+    if (tot == 0) return;
+    if (tot <= (u64)(2 * BE_STAGE)) {   // uniform: staged, one or two rounds
+        const u64 k0 = base / wpr, cw0 = base - k0 * wpr;
+        for (u64 r0 = 0; r0 < tot; r0 += BE_STAGE) {
+            const u64 r1 = r0 + BE_STAGE < tot ? r0 + BE_STAGE : tot;
+            if (c && rel < r1 && rel + c > r0) {
+                u64 p = rel, k = k0, cw = cw0;
 #pragma unroll
-        for (int i = 0; i < BE_W; i++) {
-            u32 x = w[i];
-            if (!x) continue;
-            u64 k = (base + i) / wpr;
-            u32 c0 = (u32)((base + i - k * wpr) * 32);
-            while (x) {
-                u32 v = c0 + (u32)(__ffs(x) - 1);
-                x &= x - 1;
-                out[pos++] = SWAP ? tmake(lo + v, lo + (u32)k) : tmake(lo + (u32)k, lo + v);
+                for (int i = 0; i < BE_W; i++) {
+                    u32 x = w[i];
+                    const u32 c0 = (u32)(cw * 32);
+                    while (x) {
+                        const u32 v = c0 + (u32)(__ffs(x) - 1);
+                        x &= x - 1;
+                        if (p >= r0 && p < r1)
+                            s_buf[p - r0] = SWAP ? tmake(lo + v, lo + (u32)k) : tmake(lo + (u32)k, lo + v);
+                        p++;
+                    }
+                    if (++cw == wpr) {
+                        cw = 0;
+                        k++;
+                    }
+                }
             }
+            __syncthreads();
+            for (u64 i = threadIdx.x; i < r1 - r0; i += BE_T) out[out0 + r0 + i] = s_buf[i];
+            __syncthreads();
         }
         return;
     }
     {
-        const u64 out0 = bpre[blockIdx.x];
+        u64 pos = rel;
         u64 k = base / wpr, cw = base - k * wpr;   // one division per thread; the words are consecutive
 #pragma unroll
         for (int i = 0; i < BE_W; i++) {
             s_w[threadIdx.x * BE_W + i] = w[i];
-            s_pos[threadIdx.x * BE_W + i] = (u32)(pos - out0);
+            s_pos[threadIdx.x * BE_W + i] = (u32)pos;
             s_row[threadIdx.x * BE_W + i] = (u32)k;
             pos += __popc(w[i]);
             if (++cw == wpr) {
@@ -404,7 +423,7 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
         }
     }
     __syncthreads();
-    out += bpre[blockIdx.x];
+    out += out0;
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const u32 half = lane >> 5, bit = lane & 31;
     const u32 below = (1u << bit) - 1u;
@@ -419,7 +438,7 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
     }
 }
 
-void bitmap_init(dlg_ctx *ctx, Bitmap &bm, u32 lo, u32 hi, bool transposed) {
+void bitmap_init(dlg_ctx *ctx, Bitmap &bm, u32 lo, u32 hi, bool transposed, bool clear) {
     bm.lo = lo;
     bm.W = (u64)(hi - lo) + 1;
     bm.wpr = (bm.W + 31) / 32;
@@ -430,7 +449,8 @@ void bitmap_init(dlg_ctx *ctx, Bitmap &bm, u32 lo, u32 hi, bool transposed) {
     bm.tr = transposed;
     u64 padded = (bm.nwords + BE_TILE - 1) / BE_TILE * BE_TILE;
     bm.words.ensure(padded);
-    fill_u32(ctx, bm.words.p, padded, 0u);
+    if (clear) fill_u32(ctx, bm.words.p, padded, 0u);
+    else fill_u32(ctx, bm.words.p + bm.nwords, padded - bm.nwords, 0u);   // the caller writes every row
     bm.count.ensure(1);
 }
 
@@ -730,6 +750,74 @@ u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symme
     }
     if (h != expect) fail(DLG_DEVICE_ERROR, "bitmap emission count mismatch");
     return h;
+}
+
+// ------------------------------------------------- single join into a map
+// The relation comes as `rows` = (y, x) for each tuple (x, y): keyed by its
+// second column, sorted, with dense[k] = the first row of key lo + k for k in
+// [0, W] (dense[W] = the row count).  Row c of the map gets bit a for every
+// (a, b), (b, c) of the relation: b runs over key c's run, a over key b's.
+// One workgroup per row keeps the row's wpr words in LDS; each wave takes one
+// b of the row's run at a time, its lanes OR in the first columns of b's run.
+constexpr int SJR_T = 256;
+__global__ __launch_bounds__(SJR_T) void sj_rows_k(const u64 *rows, const u64 *dense, u32 lo, u64 wpr,
+                                                   u32 *words) {
+    extern __shared__ u32 s_row[];
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const u64 c = blockIdx.x;
+    for (u64 j = threadIdx.x; j < wpr; j += SJR_T) s_row[j] = 0u;
+    __syncthreads();
+    const u64 p0 = dense[c], p1 = dense[c + 1];
+    for (u64 p = p0 + (u64)w; p < p1; p += SJR_T / WAVE) {
+        const u64 b = (u64)(tval(rows[p]) - lo);
+        const u64 q0 = dense[b], q1 = dense[b + 1];
+        for (u64 q = q0 + (u64)lane; q < q1; q += WAVE) {
+            const u32 a = tval(rows[q]) - lo;
+            atomicOr(&s_row[a >> 5], 1u << (a & 31u));
+        }
+    }
+    __syncthreads();
+    u32 *dst = words + c * wpr;
+    for (u64 j = threadIdx.x; j < wpr; j += SJR_T) dst[j] = s_row[j];
+}
+
+// |{(a, b, c)}|: the sum over rows (y, x) of key x's run length.
+__global__ __launch_bounds__(256) void sj_raw_k(const u64 *rows, u64 n, const u64 *dense, u32 lo,
+                                                unsigned long long *raw) {
+    __shared__ u64 s_part[256 / WAVE];
+    u64 t = 0;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u64 b = (u64)(tval(rows[i]) - lo);
+        t += dense[b + 1] - dense[b];
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0) s_part[threadIdx.x / WAVE] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 sum = 0;
+        for (int j = 0; j < 256 / WAVE; j++) sum += s_part[j];
+        if (sum) atomicAdd(raw, (unsigned long long)sum);
+    }
+}
+
+u64 self_join_raw(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo) {
+    if (n == 0) return 0;
+    DBuf<u64> c(ctx, 1);
+    fill_u64(ctx, c.p, 1, 0ull);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 24.0 * n, sj_raw_k, sgrid(ctx, n), 256, 0, rows, n, dense, lo,
+               (unsigned long long *)c.p);
+    u64 h = 0;
+    ctx->read_scalars(c.p, 1, &h);
+    return h;
+}
+
+void bitmap_self_join(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, Bitmap &bm) {
+    if (bm.tr) fail(DLG_INVALID_INPUT, "self join map: row-major maps only");
+    if (4 * bm.wpr > SJ_ROW_LDS_MAX) fail(DLG_INVALID_INPUT, "self join map: rows wider than LDS");
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 4.0 * bm.nwords + 8.0 * n + 8.0 * bm.W, sj_rows_k, (unsigned)bm.W, SJR_T,
+               (unsigned)(4 * bm.wpr), rows, dense, bm.lo, bm.wpr, bm.words.p);
 }
 
 }  // namespace dlg

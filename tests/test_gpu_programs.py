@@ -235,6 +235,40 @@ def test_single_join_published(ctx, rows, rng_, out):
         assert r["raw_join_rows"] == 100_003_467
 
 
+@pytest.mark.parametrize("lo,hi,n,seed", [(1, 400, 6000, 0), (-50, 400, 3000, 1), (0, 33, 500, 2),
+                                         (7, 8, 10, 3), (-2000, 3000, 4000, 4), (0, 64, 20000, 5)])
+def test_single_join_map_and_sorted_agree(ctx, orc, lo, hi, n, seed):
+    """The map form (each output row built in LDS, read out of the map in
+    order) against the sorted raw join (strategy 1) and the oracle: rows,
+    sizes and raw join rows, automatic and forced (strategy 4)."""
+    rng = np.random.default_rng(700 + seed)
+    rows = rng.integers(lo, hi, size=(n, 2)).astype(np.int32)
+    want = orc.single_join(rows)
+    runs = [ctx.run("single_join", rows, strategy=s) for s in (0, 1, 4)]
+    for r in runs:
+        assert_same(r, want)
+    assert len({r["raw_join_rows"] for r in runs}) == 1
+
+
+def test_single_join_map_range_100k_equals_sorted(ctx):
+    """10M rows over [1, 100,000] (950,725,901 tuples, logs/single-join-small-job-output.txt:10):
+    the map form and the sorted raw join give the same count, raw join rows and
+    row digest, both in order."""
+    rows = dlg.generate_rand_rows(10_000_000, 100_000)
+    a = ctx.run("single_join", rows, keep_output=False, check_output=1)
+    b = ctx.run("single_join", rows, keep_output=False, check_output=1, strategy=1)
+    assert a["output_size"] == b["output_size"] == 950_725_901
+    assert a["raw_join_rows"] == b["raw_join_rows"]
+    assert a["output_sum"] == b["output_sum"]
+    assert a["output_order_breaks"] == b["output_order_breaks"] == 0
+
+
+def test_single_join_map_too_wide(ctx):
+    rows = np.array([[0, 1], [1, 400_000]], np.int32)
+    with pytest.raises(dlg.DlgError):
+        ctx.run("single_join", rows, strategy=4)
+
+
 def test_single_join_10m_rows_equal_oracle(ctx, orc):
     rows = dlg.generate_rand_rows(10_000_000, 1_000_000)
     got = ctx.run("single_join", rows)
