@@ -755,30 +755,65 @@ u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symme
 // ------------------------------------------------- single join into a map
 // The relation comes as `rows` = (y, x) for each tuple (x, y): keyed by its
 // second column, sorted, with dense[k] = the first row of key lo + k for k in
-// [0, W] (dense[W] = the row count).  Row c of the map gets bit a for every
-// (a, b), (b, c) of the relation: b runs over key c's run, a over key b's.
-// One workgroup per row keeps the row's wpr words in LDS; each wave takes one
-// b of the row's run at a time, its lanes OR in the first columns of b's run.
-constexpr int SJR_T = 256;
-__global__ __launch_bounds__(SJR_T) void sj_rows_k(const u64 *rows, const u64 *dense, u32 lo, u64 wpr,
-                                                   u32 *words) {
+// [0, W] (dense[W] = the row count); col[i] = row i's first column minus lo.
+// Row c of the map gets bit a for every (a, b), (b, c) of the relation: b
+// runs over key c's run, a over key b's.  One workgroup per row keeps the
+// row's wpr words in LDS.  It takes c's run 256 b's at a time: each thread
+// looks up one b's run, a block scan places the runs end to end, and the
+// threads then stride over that concatenation (SJR_U independent loads in
+// flight each), finding their run by walking the scanned starts forward.
+constexpr int SJR_T = 256, SJR_U = 8;
+__global__ __launch_bounds__(SJR_T) void sj_rows_k(const u32 *col, const u64 *dense, u64 wpr, u32 *words) {
     extern __shared__ u32 s_row[];
-    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    __shared__ u64 sc[SJR_T / WAVE + 1];
+    __shared__ u64 s_pre[SJR_T + 1];
+    __shared__ u64 s_q0[SJR_T];
     const u64 c = blockIdx.x;
     for (u64 j = threadIdx.x; j < wpr; j += SJR_T) s_row[j] = 0u;
-    __syncthreads();
     const u64 p0 = dense[c], p1 = dense[c + 1];
-    for (u64 p = p0 + (u64)w; p < p1; p += SJR_T / WAVE) {
-        const u64 b = (u64)(tval(rows[p]) - lo);
-        const u64 q0 = dense[b], q1 = dense[b + 1];
-        for (u64 q = q0 + (u64)lane; q < q1; q += WAVE) {
-            const u32 a = tval(rows[q]) - lo;
-            atomicOr(&s_row[a >> 5], 1u << (a & 31u));
+    for (u64 pb = p0; pb < p1; pb += SJR_T) {
+        const u64 p = pb + threadIdx.x;
+        u64 q0 = 0, len = 0;
+        if (p < p1) {
+            const u32 b = col[p];
+            q0 = dense[b];
+            len = dense[b + 1] - q0;
         }
+        u64 tot;
+        const u64 ex = block_exclusive_scan<u64>(len, sc, &tot);   // also orders the row's clearing
+        s_pre[threadIdx.x] = ex;
+        s_q0[threadIdx.x] = q0;
+        if (threadIdx.x == 0) s_pre[SJR_T] = tot;
+        __syncthreads();
+        u32 sg = 0;   // s_pre[sg] <= j < s_pre[sg + 1] for this thread's current j
+        for (u64 j0 = threadIdx.x; j0 < tot; j0 += (u64)SJR_U * SJR_T) {
+            u64 q[SJR_U];
+#pragma unroll
+            for (int r = 0; r < SJR_U; r++) {
+                const u64 j = j0 + (u64)r * SJR_T;
+                q[r] = ~0ull;
+                if (j < tot) {
+                    while (s_pre[sg + 1] <= j) sg++;
+                    q[r] = s_q0[sg] + (j - s_pre[sg]);
+                }
+            }
+            u32 a[SJR_U];
+#pragma unroll
+            for (int r = 0; r < SJR_U; r++) a[r] = q[r] != ~0ull ? col[q[r]] : 0u;
+#pragma unroll
+            for (int r = 0; r < SJR_U; r++)
+                if (q[r] != ~0ull) atomicOr(&s_row[a[r] >> 5], 1u << (a[r] & 31u));
+        }
+        __syncthreads();   // s_pre, s_q0 and sc are reused by the next 256 b's
     }
     __syncthreads();
     u32 *dst = words + c * wpr;
     for (u64 j = threadIdx.x; j < wpr; j += SJR_T) dst[j] = s_row[j];
+}
+
+__global__ void sj_col_k(const u64 *rows, u64 n, u32 lo, u32 *col) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) col[i] = tval(rows[i]) - lo;
 }
 
 // |{(a, b, c)}|: the sum over rows (y, x) of key x's run length.
@@ -816,8 +851,10 @@ u64 self_join_raw(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo
 void bitmap_self_join(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, Bitmap &bm) {
     if (bm.tr) fail(DLG_INVALID_INPUT, "self join map: row-major maps only");
     if (4 * bm.wpr > SJ_ROW_LDS_MAX) fail(DLG_INVALID_INPUT, "self join map: rows wider than LDS");
-    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 4.0 * bm.nwords + 8.0 * n + 8.0 * bm.W, sj_rows_k, (unsigned)bm.W, SJR_T,
-               (unsigned)(4 * bm.wpr), rows, dense, bm.lo, bm.wpr, bm.words.p);
+    DBuf<u32> col(ctx, n);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, sj_col_k, sgrid(ctx, n), ctx->block_size, 0, rows, n, bm.lo, col.p);
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 4.0 * bm.nwords + 4.0 * n + 8.0 * bm.W, sj_rows_k, (unsigned)bm.W, SJR_T,
+               (unsigned)(4 * bm.wpr), (const u32 *)col.p, dense, bm.wpr, bm.words.p);
 }
 
 }  // namespace dlg

@@ -126,23 +126,26 @@ __global__ __launch_bounds__(1024) void join_count_dense_k(const u64 *probe, u64
         pstart[i] = st;
     }
 }
-__global__ void dense_hist_k(const u64 *rows, u64 n, u32 lo, u64 W, u32 *cnt) {
-    u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const u32 k = tkey(rows[i]) - lo;
-        if (k < W) atomicAdd(cnt + k, 1u);
+// dense[k] = the first row whose key is at least lo + k, k in [0, W]: one
+// binary search per key over the sorted rows.
+__global__ __launch_bounds__(256) void dense_bounds_k(const u64 *rows, u64 n, u32 lo, u64 W, u64 *dense) {
+    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > W) return;
+    const u64 key = (u64)lo + k;
+    u64 a = 0, b = n;
+    while (a < b) {
+        const u64 m = (a + b) >> 1;
+        if ((u64)tkey(rows[m]) < key) a = m + 1; else b = m;
     }
+    dense[k] = a;
 }
 
 void index_dense(dlg_ctx *ctx, Index &ix, u32 lo, u64 W) {
     ix.dense_w = 0;
     if (ix.n == 0 || W == 0) return;
-    DBuf<u32> cnt(ctx, W + 1);
-    fill_u32(ctx, cnt.p, W + 1, 0u);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * ix.n, dense_hist_k, sgrid(ctx, ix.n), ctx->block_size, 0,
-               (const u64 *)ix.rows.p, ix.n, lo, W, cnt.p);
     ix.dense.ensure(W + 1);
-    scan_u32_u64(ctx, cnt.p, W + 1, ix.dense.p);   // synchronises: cnt may go
+    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * (W + 1), dense_bounds_k, grid_for(W + 1, 256), 256, 0,
+               (const u64 *)ix.rows.p, ix.n, lo, W, ix.dense.p);
     ix.dense_lo = lo;
     ix.dense_w = W;
 }

@@ -114,18 +114,26 @@ __global__ __launch_bounds__(SCAN_T) void scan_reduce_k(const T *in, u64 n, u64 
 }
 
 // One block scans the block sums in place (exclusive); total to *d_total.
+// One workgroup: each thread sums a contiguous segment of ceil(nb / 1024)
+// entries, one block scan of those sums, then each thread rewrites its
+// segment from its prefix (the segments stay in cache between the passes).
 __global__ __launch_bounds__(1024) void scan_blocks_k(u64 *bsum, u64 nb, u64 *d_total) {
     __shared__ u64 sc[1024 / WAVE + 1];
-    u64 carry = 0;
-    for (u64 base = 0; base < nb; base += 1024) {
-        u64 i = base + threadIdx.x;
-        u64 v = i < nb ? bsum[i] : 0;
-        u64 tot;
-        u64 ex = block_exclusive_scan<u64>(v, sc, &tot);
-        if (i < nb) bsum[i] = carry + ex;
-        carry += tot;
+    const u64 seg = (nb + 1023) / 1024;
+    const u64 b0 = (u64)threadIdx.x * seg < nb ? (u64)threadIdx.x * seg : nb;
+    const u64 b1 = b0 + seg < nb ? b0 + seg : nb;
+    u64 s = 0;
+#pragma unroll 8
+    for (u64 i = b0; i < b1; i++) s += bsum[i];
+    u64 tot;
+    u64 run = block_exclusive_scan<u64>(s, sc, &tot);
+#pragma unroll 8
+    for (u64 i = b0; i < b1; i++) {
+        const u64 v = bsum[i];
+        bsum[i] = run;
+        run += v;
     }
-    if (threadIdx.x == 0 && d_total) *d_total = carry;
+    if (threadIdx.x == 0 && d_total) *d_total = tot;
 }
 
 template <typename T>

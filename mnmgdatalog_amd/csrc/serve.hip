@@ -147,6 +147,7 @@ bool pick(dlg_server *s, std::vector<Request *> &batch, size_t &reserved) {
             s->pending.erase(it);
             batch.push_back(r);
             reserved = s->budget;
+            s->in_use += reserved;   // nothing else is admitted until it ends (the worker gives it back)
             return true;
         }
     }

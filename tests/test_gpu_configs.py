@@ -98,7 +98,8 @@ def test_c5_served_mix_matches_oracle(ctx, orc):
     with dlg.Server(0, workers=4) as srv:
         tickets = [srv.submit(p, r) for p, r in reqs]
         got = [srv.wait(t) for t in tickets]
-    with ThreadPoolExecutor(16) as ex:
+    print(f"served {len(got)} requests, {sum(g['attempts'] > 1 for g in got)} re-run; oracle next", flush=True)
+    with ThreadPoolExecutor(16) as ex:   # about a minute: the largest TC requests take most of it
         want = list(ex.map(lambda pr: getattr(orc, pr[0])(pr[1]), reqs))
     assert max(g["batch_size"] for g in got) > 1
     for (prog, rows), g, w in zip(reqs, got, want):

@@ -98,3 +98,20 @@ def test_served_wait_timeout_and_unknown_ticket():
         assert first["status"] == dlg.DLG_OK and first["output_size"] == 481121
         with pytest.raises(dlg.DlgError):
             srv.wait(t)                                             # handed over once
+
+
+def test_requests_re_run_alone_one_after_another(ctx):
+    """Requests that outgrow their reservation are re-run alone, each with the
+    whole budget: with one slot they do so one after the other, and every one
+    completes with the result of its plain run (the second used to wait for
+    good: a request run alone was not counted in the budget in use, so the
+    count never returned to zero when it ended)."""
+    rng = np.random.default_rng(77)
+    reqs = [rng.integers(1, 1000, size=(100_000, 2)).astype(np.int32) for _ in range(3)]
+    with dlg.Server(0, workers=1, hbm_budget=2 << 30, retain_bytes=1, strategy=1) as srv:
+        tickets = [srv.submit("single_join", r) for r in reqs]
+        got = [srv.wait(t, timeout=300.0) for t in tickets]
+    for r, g in zip(reqs, got):
+        assert g is not None, "request still pending"
+        assert g["attempts"] == 2 and g["batch_size"] == 1
+        same(g, ctx.run("single_join", r, strategy=1))
