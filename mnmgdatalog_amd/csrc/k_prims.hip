@@ -113,8 +113,8 @@ __global__ __launch_bounds__(SCAN_T) void scan_reduce_k(const T *in, u64 n, u64 
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// One block scans the block sums in place (exclusive); total to *d_total.
-// One workgroup: each thread sums a contiguous segment of ceil(nb / 1024)
+// Block sums scanned in place (exclusive), total to *d_total, by one
+// workgroup: each thread sums a contiguous segment of ceil(nb / 1024)
 // entries, one block scan of those sums, then each thread rewrites its
 // segment from its prefix (the segments stay in cache between the passes).
 __global__ __launch_bounds__(1024) void scan_blocks_k(u64 *bsum, u64 nb, u64 *d_total) {
@@ -155,6 +155,23 @@ __global__ __launch_bounds__(SCAN_T) void scan_apply_k(const T *in, u64 n, const
     }
 }
 
+// Block sums scanned in place (exclusive), total to *d_total: one workgroup
+// up to SCAN_ONE_WG entries, else reduce -> scan of the partials -> apply.
+constexpr u64 SCAN_ONE_WG = 16384;
+void scan_blocks(dlg_ctx *ctx, u64 *bsum, u64 nb, u64 *d_total) {
+    if (nb <= SCAN_ONE_WG) {
+        DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum, nb, d_total);
+        return;
+    }
+    const u64 np = (nb + SCAN_TILE - 1) / SCAN_TILE;
+    DBuf<u64> part(ctx, np);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 8.0 * nb + 8.0 * np, scan_reduce_k<u64>, (unsigned)np, SCAN_T, 0,
+               (const u64 *)bsum, nb, part.p);
+    scan_blocks(ctx, part.p, np, d_total);
+    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_apply_k<u64>, (unsigned)np, SCAN_T, 0, (const u64 *)bsum, nb,
+               (const u64 *)part.p, bsum);
+}
+
 u64 scan_partials(u64 n) {
     u64 nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     return nb ? nb : 1;
@@ -166,7 +183,7 @@ static void scan_to_u64_with(dlg_ctx *ctx, const T *in, u64 n, u64 *out, u64 *bs
     const u64 nb = scan_partials(n);
     DLG_LAUNCH(ctx, DLG_K_SCAN, (double)sizeof(T) * n + 8.0 * nb, scan_reduce_k<T>, (unsigned)nb, SCAN_T, 0,
                in, n, bsum);
-    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum, nb, d_total);
+    scan_blocks(ctx, bsum, nb, d_total);
     DLG_LAUNCH(ctx, DLG_K_SCAN, (sizeof(T) + 8.0) * n, scan_apply_k<T>, (unsigned)nb, SCAN_T, 0, in, n,
                (const u64 *)bsum, out);
 }
@@ -232,8 +249,7 @@ static void scan_u32_inplace(dlg_ctx *ctx, u32 *data, u64 n, DBuf<u64> &bsum) {
     bsum.ensure(nb);
     DLG_LAUNCH(ctx, DLG_K_SCAN, 4.0 * n, scan_reduce32_k, (unsigned)nb, SCAN_T, 0, data, n,
                bsum.p);
-    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb,
-               (u64 *)nullptr);
+    scan_blocks(ctx, bsum.p, nb, nullptr);
     DLG_LAUNCH(ctx, DLG_K_SCAN, 8.0 * n, scan_apply32_k, (unsigned)nb, SCAN_T, 0, data, n,
                bsum.p);
 }
@@ -940,7 +956,7 @@ u64 select(dlg_ctx *ctx, int mode, const u64 *in, u64 n, const u64 *other, u64 n
             break;
         default: fail(DLG_INVALID_INPUT, "select: bad mode");
     }
-    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb, tot.p);
+    scan_blocks(ctx, bsum.p, nb, tot.p);
     u64 total = 0;
     ctx->read_scalars(tot.p, 1, &total);
     if (total == 0) return 0;
@@ -990,14 +1006,10 @@ __global__ __launch_bounds__(SEL_T) void select_count_runs_k(const u64 *in, u64 
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-void scan_blocks(dlg_ctx *ctx, u64 *bsum, u64 nb, u64 *d_total) {
-    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum, nb, d_total);
-}
-
 u64 select_scatter(dlg_ctx *ctx, const u64 *in, u64 n, const u64 *flags, DBuf<u64> &bsum, DBuf<u64> &tot,
                    u64 *out) {
     u64 nb = (n + SEL_TILE - 1) / SEL_TILE;
-    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb, tot.p);
+    scan_blocks(ctx, bsum.p, nb, tot.p);
     u64 total = 0;
     ctx->read_scalars(tot.p, 1, &total);
     if (total == 0) return 0;
@@ -1015,7 +1027,7 @@ u64 select_unique_not_in_runs(dlg_ctx *ctx, const u64 *in, u64 n, const RunSet &
     DBuf<u64> tot(ctx, 1);
     DLG_LAUNCH(ctx, DLG_K_SELECT, 8.0 * n + 8.0 * ((n + 63) / 64), select_count_runs_k,
                (unsigned)nb, SEL_T, 0, in, n, rs, flags.p, bsum.p);
-    DLG_LAUNCH(ctx, DLG_K_SCAN, 16.0 * nb, scan_blocks_k, 1, 1024, 0, bsum.p, nb, tot.p);
+    scan_blocks(ctx, bsum.p, nb, tot.p);
     u64 total = 0;
     ctx->read_scalars(tot.p, 1, &total);
     if (total == 0) return 0;

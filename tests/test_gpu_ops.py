@@ -117,7 +117,7 @@ def test_merge_repeated_values(ctx, na, nb, vals):
     assert np.array_equal(m, np.sort(np.concatenate([a, b])))
 
 
-@pytest.mark.parametrize("n", [1, 100, 2048, 2049, 1_000_000, 5_000_001])
+@pytest.mark.parametrize("n", [1, 100, 2048, 2049, 1_000_000, 5_000_001, 40_000_003])
 def test_exclusive_scan(ctx, n):
     rng = np.random.default_rng(n)
     c = rng.integers(0, 5000, size=n, dtype=np.uint32)
