@@ -163,6 +163,12 @@ dlg_status dlg_set_difference(dlg_ctx *ctx, const uint64_t *d_a, int64_t na, con
  * (tc.cu:275-279).  Equal elements: a's copy first, as a stable merge does. */
 dlg_status dlg_merge(dlg_ctx *ctx, const uint64_t *d_a, int64_t na, const uint64_t *d_b,
                      int64_t nb, uint64_t *d_out);
+/* The same merge keeping one copy of each value (sorted inputs, duplicates
+ * allowed): thrust::sort + thrust::unique over the concatenation of two
+ * sorted arrays, as WCC dedups E ++ E^-1 (wcc.cu:163-167), in one merge.
+ * d_out holds na + nb slots; *out_n = the count written. */
+dlg_status dlg_merge_unique(dlg_ctx *ctx, const uint64_t *d_a, int64_t na, const uint64_t *d_b,
+                            int64_t nb, uint64_t *d_out, int64_t *out_n);
 /* Exclusive prefix sum of uint32 counts into uint64 offsets; *total = sum.
  * Replaces thrust::reduce + thrust::exclusive_scan in get_join (common/join.cu:72-74). */
 dlg_status dlg_exclusive_scan(dlg_ctx *ctx, const uint32_t *d_in, int64_t n, uint64_t *d_out,

@@ -113,6 +113,7 @@ def lib() -> ctypes.CDLL:
         "dlg_sort": (c_int, [vp, vp, i64]),
         "dlg_set_difference": (c_int, [vp, vp, i64, vp, i64, vp, POINTER(c_int64)]),
         "dlg_merge": (c_int, [vp, vp, i64, vp, i64, vp]),
+        "dlg_merge_unique": (c_int, [vp, vp, i64, vp, i64, vp, POINTER(i64)]),
         "dlg_exclusive_scan": (c_int, [vp, vp, i64, vp, POINTER(c_uint64)]),
         "dlg_result_free": (None, [POINTER(Result)]),
         "dlg_read_rows": (c_int, [c_char_p, POINTER(POINTER(c_int32)), POINTER(c_int64)]),
@@ -365,6 +366,18 @@ class Ctx:
         try:
             self.check(self.L.dlg_merge(self.h, c_void_p(pa), len(a), c_void_p(pb), len(b), c_void_p(po)), "merge")
             return self.download(po, len(a) + len(b), np.uint64)
+        finally:
+            for p in (pa, pb, po):
+                self.free(p)
+
+    def merge_unique(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        pa, pb = self.upload(np.asarray(a, np.uint64)), self.upload(np.asarray(b, np.uint64))
+        po = self.alloc(8 * max(len(a) + len(b), 1))
+        try:
+            k = c_int64()
+            self.check(self.L.dlg_merge_unique(self.h, c_void_p(pa), len(a), c_void_p(pb), len(b), c_void_p(po),
+                                               byref(k)), "merge_unique")
+            return self.download(po, k.value, np.uint64)
         finally:
             for p in (pa, pb, po):
                 self.free(p)

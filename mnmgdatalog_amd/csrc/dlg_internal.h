@@ -552,6 +552,9 @@ void hashset_insert(dlg_ctx *ctx, HashSet &hs, const u64 *x, u64 n);
 u64 select_unique_not_in_hash(dlg_ctx *ctx, const u64 *in, u64 n, const HashSet &hs, u64 *out);
 // Merge of sorted a, b into out (a first on ties).
 void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out);
+// The merge with one copy of each value kept (inputs sorted, duplicates
+// allowed); returns the count written to out (synchronises).
+u64 merge_unique(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out);
 void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out);
 void unpack_rows(dlg_ctx *ctx, const u64 *t, u64 n, int orient, int32_t *rows);
 // E into out[0, n) and E⁻¹ into out_rev[0, n) (default: out[n, 2n)), row order kept.

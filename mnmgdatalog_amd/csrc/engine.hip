@@ -571,10 +571,8 @@ void wcc_graph_build(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, 
         // ascending order, so a stable sort by its key alone sorts it by (dst, src) --
         // the key's passes only -- and one merge gives E ∪ E⁻¹ sorted, as the full sort would
         radix_sort(ctx, rev, n, tmp, o.bits, nullptr, true);
-        tmp.ensure(2 * n);
-        merge(ctx, sym.p, n, rev.p, n, tmp.p);
         edge.ensure(2 * n);
-        g.ne = select(ctx, SEL_UNIQUE, tmp.p, 2 * n, nullptr, 0, edge.p);
+        g.ne = merge_unique(ctx, sym.p, n, rev.p, n, edge.p);
     } else {
         g.ne = sort_unique_into(ctx, sym, 2 * n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
     }
@@ -915,6 +913,15 @@ dlg_status dlg_merge(dlg_ctx *c, const uint64_t *a, int64_t na, const uint64_t *
     API_BEGIN(c)
     merge(c, a, (u64)na, b, (u64)nb, out);
     c->sync();
+    API_END(c)
+}
+
+dlg_status dlg_merge_unique(dlg_ctx *c, const uint64_t *a, int64_t na, const uint64_t *b, int64_t nb,
+                            uint64_t *out, int64_t *out_n) {
+    if (!c || !out_n || na < 0 || nb < 0 || (na && !a) || (nb && !b) || ((na || nb) && !out))
+        return DLG_INVALID_INPUT;
+    API_BEGIN(c)
+    *out_n = (int64_t)merge_unique(c, a, (u64)na, b, (u64)nb, out);
     API_END(c)
 }
 

@@ -1057,21 +1057,23 @@ __global__ __launch_bounds__(MP_SPLIT_T) void merge_split_k(const u64 *a, u64 na
     split[t] = lo;
 }
 
-__global__ __launch_bounds__(MP_T) void merge_path_k(const u64 *a, u64 na, const u64 *b, u64 nb,
-                                                     const u64 *split, u64 *out) {
-    __shared__ u64 s[MP_TILE];
+// Tile blockIdx.x of the merge: a's stretch and b's side by side in s, then
+// this thread's MP_I outputs (tile positions [t MP_I, t MP_I + MP_I), valid
+// below the returned count) merged into v.  s is still being read when it
+// returns: a barrier must come before s is written again.
+__device__ __forceinline__ int mp_tile(const u64 *a, u64 na, const u64 *b, u64 nb, const u64 *split, u64 *s,
+                                       u64 (&v)[MP_I]) {
     const u64 n = na + nb, d0 = (u64)blockIdx.x * MP_TILE;
     const int cnt = (int)(n - d0 < (u64)MP_TILE ? n - d0 : (u64)MP_TILE);
     const u64 i0 = split[blockIdx.x], j0 = d0 - i0;
     const int ca = (int)(split[blockIdx.x + 1] - i0), cb = cnt - ca;
 #pragma unroll
-    for (int r = 0; r < MP_I; r++) {   // a's stretch, then b's, side by side in the tile
+    for (int r = 0; r < MP_I; r++) {
         const int k = r * MP_T + (int)threadIdx.x;
         if (k < cnt) s[k] = k < ca ? a[i0 + (u64)k] : b[j0 + (u64)(k - ca)];
     }
     __syncthreads();
     const int dt = (int)threadIdx.x * MP_I;
-    u64 v[MP_I];
     if (dt < cnt) {
         int lo = dt > cb ? dt - cb : 0, hi = dt < ca ? dt : ca;
         while (lo < hi) {   // this thread's split inside the tile
@@ -1093,6 +1095,15 @@ __global__ __launch_bounds__(MP_T) void merge_path_k(const u64 *a, u64 na, const
             }
         }
     }
+    return cnt;
+}
+
+__global__ __launch_bounds__(MP_T) void merge_path_k(const u64 *a, u64 na, const u64 *b, u64 nb,
+                                                     const u64 *split, u64 *out) {
+    __shared__ u64 s[MP_TILE];
+    u64 v[MP_I];
+    const int cnt = mp_tile(a, na, b, nb, split, s, v);
+    const int dt = (int)threadIdx.x * MP_I;
     __syncthreads();
     if (dt < cnt) {
 #pragma unroll
@@ -1100,11 +1111,63 @@ __global__ __launch_bounds__(MP_T) void merge_path_k(const u64 *a, u64 na, const
             if (dt + r < cnt) s[dt + r] = v[r];
     }
     __syncthreads();
+    const u64 d0 = (u64)blockIdx.x * MP_TILE;
 #pragma unroll
     for (int r = 0; r < MP_I; r++) {
         const int k = r * MP_T + (int)threadIdx.x;
         if (k < cnt) out[d0 + (u64)k] = s[k];
     }
+}
+
+// Merge keeping one copy of each value: an output is kept when it differs
+// from the one before it (the tile's first compares with the element before
+// the tile: the larger of a[i0 - 1] and b[j0 - 1]).  Pass 1 (SCATTER false)
+// writes each tile's count to tcnt; pass 2 reads the tiles' exclusive
+// offsets there, compacts the kept outputs in LDS and stores them coalesced.
+template <bool SCATTER>
+__global__ __launch_bounds__(MP_T) void merge_unique_k(const u64 *a, u64 na, const u64 *b, u64 nb,
+                                                       const u64 *split, u64 *tcnt, u64 *out) {
+    __shared__ u64 s[MP_TILE];
+    __shared__ u64 s_last[MP_T];
+    __shared__ u64 sc[MP_T / WAVE + 1];
+    u64 v[MP_I];
+    const int cnt = mp_tile(a, na, b, nb, split, s, v);
+    const int dt = (int)threadIdx.x * MP_I;
+    const int m = dt < cnt ? (cnt - dt < MP_I ? cnt - dt : MP_I) : 0;   // this thread's valid outputs
+    if (m) s_last[threadIdx.x] = v[m - 1];
+    u64 prev = 0;
+    bool has_prev = threadIdx.x > 0;
+    if (threadIdx.x == 0 && blockIdx.x > 0) {   // the element before the tile
+        const u64 i0 = split[blockIdx.x], j0 = (u64)blockIdx.x * MP_TILE - i0;
+        const u64 pa = i0 ? a[i0 - 1] : 0, pb = j0 ? b[j0 - 1] : 0;
+        prev = pa > pb ? pa : pb;
+        has_prev = true;
+    }
+    __syncthreads();
+    if (threadIdx.x > 0) prev = s_last[threadIdx.x - 1];   // a thread with outputs follows a full one
+    u32 keep = 0, nk = 0;
+#pragma unroll
+    for (int r = 0; r < MP_I; r++) {
+        if (r < m) {
+            const bool k = !has_prev || v[r] != prev;
+            keep |= (u32)k << r;
+            nk += k;
+            prev = v[r];
+            has_prev = true;
+        }
+    }
+    u64 tot;
+    u64 pos = block_exclusive_scan<u64>(nk, sc, &tot);   // its barriers also end the tile's reads of s
+    if (!SCATTER) {
+        if (threadIdx.x == 0) tcnt[blockIdx.x] = tot;
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < MP_I; r++)
+        if ((keep >> r) & 1u) s[pos++] = v[r];
+    __syncthreads();
+    const u64 o0 = tcnt[blockIdx.x];
+    for (u64 k = threadIdx.x; k < tot; k += MP_T) out[o0 + k] = s[k];
 }
 
 // Adjacent pairs out of order (d[i] > d[i+1]), counted per block.
@@ -1165,15 +1228,38 @@ u64 count_descents(dlg_ctx *ctx, const u64 *d, u64 n) {
     return h;
 }
 
+static u64 merge_splits(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, DBuf<u64> &split) {
+    const u64 tiles = (na + nb + MP_TILE - 1) / MP_TILE;
+    split.ensure(tiles + 1);
+    DLG_LAUNCH(ctx, DLG_K_MERGE, 16.0 * (tiles + 1), merge_split_k, grid_for(tiles + 1, MP_SPLIT_T), MP_SPLIT_T, 0,
+               a, na, b, nb, tiles, split.p);
+    return tiles;
+}
+
 void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out) {
     const u64 n = na + nb;
     if (n == 0) return;
-    const u64 tiles = (n + MP_TILE - 1) / MP_TILE;
-    DBuf<u64> split(ctx, tiles + 1);
-    DLG_LAUNCH(ctx, DLG_K_MERGE, 16.0 * (tiles + 1), merge_split_k,
-               grid_for(tiles + 1, MP_SPLIT_T), MP_SPLIT_T, 0, a, na, b, nb, tiles, split.p);
+    DBuf<u64> split(ctx);
+    const u64 tiles = merge_splits(ctx, a, na, b, nb, split);
     DLG_LAUNCH(ctx, DLG_K_MERGE, 16.0 * n, merge_path_k, (unsigned)tiles, MP_T, 0, a, na, b, nb,
                (const u64 *)split.p, out);
+}
+
+u64 merge_unique(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out) {
+    const u64 n = na + nb;
+    if (n == 0) return 0;
+    DBuf<u64> split(ctx), tcnt(ctx), tot(ctx, 1);
+    const u64 tiles = merge_splits(ctx, a, na, b, nb, split);
+    tcnt.ensure(tiles);
+    DLG_LAUNCH(ctx, DLG_K_MERGE, 8.0 * n, merge_unique_k<false>, (unsigned)tiles, MP_T, 0, a, na, b, nb,
+               (const u64 *)split.p, tcnt.p, (u64 *)nullptr);
+    scan_blocks(ctx, tcnt.p, tiles, tot.p);
+    u64 total = 0;
+    ctx->read_scalars(tot.p, 1, &total);
+    if (total == 0) return 0;
+    DLG_LAUNCH(ctx, DLG_K_MERGE, 8.0 * n + 8.0 * total, merge_unique_k<true>, (unsigned)tiles, MP_T, 0, a, na, b,
+               nb, (const u64 *)split.p, tcnt.p, out);
+    return total;
 }
 
 }  // namespace dlg

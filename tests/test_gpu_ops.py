@@ -117,6 +117,20 @@ def test_merge_repeated_values(ctx, na, nb, vals):
     assert np.array_equal(m, np.sort(np.concatenate([a, b])))
 
 
+@pytest.mark.parametrize("na,nb,vals", [(0, 0, 5), (0, 7, 3), (2048, 0, 5), (2047, 1, 5), (1, 4095, 2),
+                                       (3000, 5000, 3), (4096, 4096, 1), (100_000, 70_000, 40),
+                                       (1_500_000, 600_000, 1 << 40), (3_000_000, 3_000_000, 1 << 22)])
+def test_merge_unique(ctx, na, nb, vals):
+    """Merge keeping one copy of each value against numpy's unique of the
+    concatenation: runs of equal values inside and across the inputs and
+    across tile edges, the largest u64 value included."""
+    rng = np.random.default_rng(na * 3 + nb * 5 + vals)
+    top = np.uint64(0xFFFFFFFFFFFFFFFF)
+    a = np.sort(top - rng.integers(0, vals, size=na, dtype=np.uint64))
+    b = np.sort(top - rng.integers(0, vals, size=nb, dtype=np.uint64))
+    assert np.array_equal(ctx.merge_unique(a, b), np.unique(np.concatenate([a, b])))
+
+
 @pytest.mark.parametrize("n", [1, 100, 2048, 2049, 1_000_000, 5_000_001, 40_000_003])
 def test_exclusive_scan(ctx, n):
     rng = np.random.default_rng(n)
