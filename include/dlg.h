@@ -189,7 +189,10 @@ typedef struct dlg_opts {
                                 relation with a device hash set for the set
                                 difference, sorted once at the end; 4 (TC, SG) the full
                                 relation as a bitmap over the id range (at most
-                                370,720 ids wide; automatic up to 2^18)                */
+                                370,720 ids wide; automatic up to 2^18); (single join)
+                                every output row built in LDS and read out of the
+                                map, no raw join or sort (automatic up to 2^18 ids
+                                when the map is no larger than the raw join)          */
     int32_t max_iterations;  /* 0 = run to the fixpoint; >0 stops early (sampling)     */
     int32_t keep_output;     /* 1: copy the output rows to host (dlg_result.rows)      */
     int32_t check_output;    /* 1: fill dlg_result.output_sum / output_order_breaks
