@@ -561,23 +561,35 @@ constexpr u64 WCC_PRESORTED_MIN = 1ull << 20;
 void wcc_graph_build(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, PhaseClock &pc,
                      WccGraph &g) {
     pc.mark(DLG_PH_INIT);
-    const bool presorted = n >= WCC_PRESORTED_MIN && count_descents_rows(ctx, d_rows, n) == 0;
-    DBuf<u64> sym(ctx, presorted ? n : 2 * n), rev(ctx), edge(ctx), tmp(ctx);
-    if (presorted) rev.ensure(n);
-    pack_sym_rows(ctx, d_rows, n, sym.p, presorted ? rev.p : nullptr);   // E ∪ E⁻¹ (wcc.cu:140-160)
+    // E then E⁻¹ in one buffer (wcc.cu:140-160), with the input's order breaks
+    // and column bounds taken in the same read
+    DBuf<u64> sym(ctx, 2 * n), edge(ctx), tmp(ctx);
+    u32 cb[4];
+    const u64 breaks = pack_sym_rows_scan(ctx, d_rows, n, sym.p, sym.p + n, cb);
+    const bool presorted = n >= WCC_PRESORTED_MIN && breaks == 0;
     pc.mark(DLG_PH_DEDUP);
     if (presorted) {
         // E arrives sorted by (src, dst): E⁻¹ in that order lists each dst's sources in
         // ascending order, so a stable sort by its key alone sorts it by (dst, src) --
-        // the key's passes only -- and one merge gives E ∪ E⁻¹ sorted, as the full sort would
-        radix_sort(ctx, rev, n, tmp, o.bits, nullptr, true);
+        // the key's passes only -- and one merge keeping one copy of each edge gives
+        // E ∪ E⁻¹ sorted and deduplicated, as the full sort + unique would
+        const u32 rb[4] = {cb[2], cb[3], cb[0], cb[1]};   // E⁻¹: key b, value a
+        DBuf<u64> view(ctx);   // E⁻¹ in place, borrowed from sym
+        view.p = sym.p + n;
+        view.cap = n;
+        radix_sort(ctx, view, n, tmp, o.bits, rb, true);
+        const u64 *rev = view.p;                  // in place, or in tmp's block
+        if (view.p != sym.p + n) tmp.swap(view);  // tmp owns its block again
+        view.p = nullptr;
+        view.cap = 0;
         edge.ensure(2 * n);
-        g.ne = merge_unique(ctx, sym.p, n, rev.p, n, edge.p);
+        g.ne = merge_unique(ctx, sym.p, n, rev, n, edge.p);
     } else {
-        g.ne = sort_unique_into(ctx, sym, 2 * n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits);
+        const u32 lo = std::min(cb[0], cb[2]), hi = std::max(cb[1], cb[3]);
+        const u32 ub[4] = {lo, hi, lo, hi};
+        g.ne = sort_unique_into(ctx, sym, 2 * n, tmp, edge, SEL_UNIQUE, nullptr, 0, o.bits, ub);
     }
     sym.release();
-    rev.release();
     tmp.release();
     pc.mark(DLG_PH_HASHTABLE);
     g.ix.rows.swap(edge);   // the index keeps the sorted edges in place

@@ -71,6 +71,52 @@ __global__ void pack_sym_k(const int2 *rows, u64 n, u64 *out, u64 *out_rev) {
         out_rev[i] = tmake(b, a);
     }
 }
+// pack_sym_k plus, in the same read, the bounds of both columns (as flipped
+// ids) and the count of adjacent rows out of signed lexicographic order;
+// one set of atomics per workgroup.
+__global__ __launch_bounds__(256) void pack_sym_scan_k(const int2 *rows, u64 n, u64 *out, u64 *out_rev, u32 *mm,
+                                                       unsigned long long *desc) {
+    u32 mna = 0xffffffffu, mxa = 0u, mnb = 0xffffffffu, mxb = 0u;
+    u64 dc = 0;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int2 r = rows[i];
+        const u32 a = (u32)r.x ^ 0x80000000u, b = (u32)r.y ^ 0x80000000u;
+        out[i] = tmake(a, b);
+        out_rev[i] = tmake(b, a);
+        mna = min(mna, a); mxa = max(mxa, a); mnb = min(mnb, b); mxb = max(mxb, b);
+        if (i + 1 < n) {
+            const int2 q = rows[i + 1];
+            dc += r.x > q.x || (r.x == q.x && r.y > q.y);
+        }
+    }
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        mna = min(mna, (u32)__shfl_xor((int)mna, d, WAVE));
+        mxa = max(mxa, (u32)__shfl_xor((int)mxa, d, WAVE));
+        mnb = min(mnb, (u32)__shfl_xor((int)mnb, d, WAVE));
+        mxb = max(mxb, (u32)__shfl_xor((int)mxb, d, WAVE));
+        dc += __shfl_xor(dc, d, WAVE);
+    }
+    __shared__ u32 sm[4][256 / WAVE];
+    __shared__ u64 sd[256 / WAVE];
+    const int wave = threadIdx.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        sm[0][wave] = mna; sm[1][wave] = mxa; sm[2][wave] = mnb; sm[3][wave] = mxb;
+        sd[wave] = dc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / WAVE); w++) {
+            mna = min(mna, sm[0][w]); mxa = max(mxa, sm[1][w]);
+            mnb = min(mnb, sm[2][w]); mxb = max(mxb, sm[3][w]);
+            dc += sd[w];
+        }
+        atomicMin(&mm[0], mna); atomicMax(&mm[1], mxa);
+        atomicMin(&mm[2], mnb); atomicMax(&mm[3], mxb);
+        if (dc) atomicAdd(desc, (unsigned long long)dc);
+    }
+}
 __global__ void unpack_k(const u64 *t, u64 n, int orient, int2 *rows) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     u64 stride = (u64)gridDim.x * blockDim.x;
@@ -90,6 +136,26 @@ void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out, u64 *out_
     if (!n) return;
     DLG_LAUNCH(ctx, DLG_K_PACK, 24.0 * n, pack_sym_k, sgrid(ctx, n), ctx->block_size, 0,
                (const int2 *)rows, n, out, out_rev ? out_rev : out + n);
+}
+__global__ void pack_scan_init_k(u64 *w) {   // bounds words (min a, max a, min b, max b), descents
+    if (threadIdx.x == 0) {
+        u32 *mm = (u32 *)w;
+        mm[0] = 0xffffffffu; mm[1] = 0u; mm[2] = 0xffffffffu; mm[3] = 0u;
+        w[2] = 0ull;
+    }
+}
+u64 pack_sym_rows_scan(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out, u64 *out_rev, u32 *cb) {
+    cb[0] = cb[1] = cb[2] = cb[3] = 0u;
+    if (!n) return 0;
+    DBuf<u64> w(ctx, 3);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 0.0, pack_scan_init_k, 1, 64, 0, w.p);
+    const unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 8);
+    DLG_LAUNCH(ctx, DLG_K_PACK, 24.0 * n, pack_sym_scan_k, g, 256, 0, (const int2 *)rows, n, out, out_rev,
+               (u32 *)w.p, (unsigned long long *)(w.p + 2));
+    u64 h[3];
+    ctx->read_scalars(w.p, 3, h);
+    std::memcpy(cb, h, 16);
+    return h[2];
 }
 void unpack_rows(dlg_ctx *ctx, const u64 *t, u64 n, int orient, int32_t *rows) {
     if (!n) return;
@@ -1168,64 +1234,6 @@ __global__ __launch_bounds__(MP_T) void merge_unique_k(const u64 *a, u64 na, con
     __syncthreads();
     const u64 o0 = tcnt[blockIdx.x];
     for (u64 k = threadIdx.x; k < tot; k += MP_T) out[o0 + k] = s[k];
-}
-
-// Adjacent pairs out of order (d[i] > d[i+1]), counted per block.
-__global__ __launch_bounds__(256) void descents_k(const u64 *d, u64 n, unsigned long long *count) {
-    __shared__ u64 s_part[256 / WAVE];
-    u64 c = 0;
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n; i += stride) c += d[i] > d[i + 1];
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
-    if ((threadIdx.x & (WAVE - 1)) == 0) s_part[threadIdx.x / WAVE] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        u64 t = 0;
-        for (int w = 0; w < 256 / WAVE; w++) t += s_part[w];
-        if (t) atomicAdd(count, (unsigned long long)t);
-    }
-}
-
-// The same over int32 (a, b) rows in signed lexicographic order (the packed order).
-__global__ __launch_bounds__(256) void descents_rows_k(const int2 *r, u64 n, unsigned long long *count) {
-    __shared__ u64 s_part[256 / WAVE];
-    u64 c = 0;
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n; i += stride) {
-        const int2 a = r[i], b = r[i + 1];
-        c += a.x > b.x || (a.x == b.x && a.y > b.y);
-    }
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
-    if ((threadIdx.x & (WAVE - 1)) == 0) s_part[threadIdx.x / WAVE] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        u64 t = 0;
-        for (int w = 0; w < 256 / WAVE; w++) t += s_part[w];
-        if (t) atomicAdd(count, (unsigned long long)t);
-    }
-}
-
-u64 count_descents_rows(dlg_ctx *ctx, const int32_t *rows, u64 n) {
-    if (n < 2) return 0;
-    DBuf<u64> c(ctx, 1);
-    fill_u64(ctx, c.p, 1, 0ull);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, descents_rows_k, sgrid(ctx, n), 256, 0, (const int2 *)rows, n,
-               (unsigned long long *)c.p);
-    u64 h = 0;
-    ctx->read_scalars(c.p, 1, &h);
-    return h;
-}
-
-u64 count_descents(dlg_ctx *ctx, const u64 *d, u64 n) {
-    if (n < 2) return 0;
-    DBuf<u64> c(ctx, 1);
-    fill_u64(ctx, c.p, 1, 0ull);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * n, descents_k, sgrid(ctx, n), 256, 0, d, n, (unsigned long long *)c.p);
-    u64 h = 0;
-    ctx->read_scalars(c.p, 1, &h);
-    return h;
 }
 
 static u64 merge_splits(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, DBuf<u64> &split) {
