@@ -87,6 +87,9 @@ double now_s(const dlg_server *s) { return secs(s->t0, Clock::now()); }
 // condensation (dlg_predict_tc_size, estimates padded by a quarter), or the
 // bound nodes^2 (SG; TC when even the bound fits or the request has more than
 // PREDICT_MAX_ROWS rows), with nodes bounded by the id range and twice the rows.
+// A single join reserves 384 B per row for its relations, plus (when the id
+// range allows degree counts) its map and output in the map form, or 24 B
+// per raw join row sorted, the raw join bounded by in-degree x out-degree.
 constexpr int64_t PREDICT_MAX_ROWS = 4000000;
 
 size_t predict(const dlg_server *s, int prog, const int32_t *rows, int64_t n, int graph) {
@@ -121,9 +124,29 @@ size_t predict(const dlg_server *s, int prog, const int32_t *rows, int64_t n, in
         case DLG_PROG_WCC:
             b = 200.0 * (double)n + 64.0 * nodes;
             break;
-        default:            // single join: R, R^-1, index, J and sort scratch
-            b = 48.0 * (double)n * 8.0;
+        default: {          // single join
+            b = 48.0 * (double)n * 8.0;   // R, R^-1 and their index
+            const int st = s->opts.strategy;
+            if (n && W <= (double)BITMAP_FORCED_MAX_W && (st == 4 || (st == 0 && s->opts.join == 0))) {
+                // the raw join's size with duplicates (a bound of the engine's count over
+                // distinct rows): the sum over ids b of in-degree x out-degree
+                std::vector<uint32_t> in((size_t)W, 0u), out((size_t)W, 0u);
+                for (int64_t i = 0; i < n; i++) {
+                    out[(size_t)(rows[2 * i] - lo)]++;
+                    in[(size_t)(rows[2 * i + 1] - lo)]++;
+                }
+                double raw = 0.0;
+                for (size_t k = 0; k < (size_t)W; k++) raw += (double)in[k] * (double)out[k];
+                const double mapb = W * std::ceil(W / 32.0) * 4.0;
+                if (st == 4 || (W <= (double)BITMAP_MAX_W && mapb <= 8.0 * raw))
+                    b += mapb + 8.0 * std::min(raw, W * W);   // the map form: map + output
+                else
+                    b += 24.0 * raw;                          // J, its sort scratch, the output
+            } else {
+                b += 24.0 * 8.0 * (double)n;   // no degree counts over a wide id range
+            }
             break;
+        }
     }
     if (b > 1e18) b = 1e18;
     return (size_t)b + slack;
