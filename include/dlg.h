@@ -192,7 +192,11 @@ typedef struct dlg_opts {
                                 370,720 ids wide; automatic up to 2^18); (single join)
                                 every output row built in LDS and read out of the
                                 map, no raw join or sort (automatic up to 2^18 ids
-                                when the map is no larger than the raw join)          */
+                                when the map is no larger than the raw join).
+                                Single join, 2: each output row's candidates sorted
+                                and deduplicated in LDS by one wave (automatic when
+                                the map is not taken and no row has more than 2,048
+                                candidates); otherwise the sorted raw join            */
     int32_t max_iterations;  /* 0 = run to the fixpoint; >0 stops early (sampling)     */
     int32_t keep_output;     /* 1: copy the output rows to host (dlg_result.rows)      */
     int32_t check_output;    /* 1: fill dlg_result.output_sum / output_order_breaks

@@ -413,6 +413,13 @@ constexpr u64 SJ_ROW_LDS_MAX = 64 * 1024;   // bytes of one map row in LDS
 void bitmap_self_join(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, Bitmap &bm);
 // The raw size of that join, sum over b of |{a}| x |{c}| (synchronises).
 u64 self_join_raw(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo);
+// The same single join by rows: each output row's candidates sorted and
+// deduplicated in LDS by one wave, the rows placed by a scan, (c, a) tuples
+// in order into out (resized), *nout of them.  Returns false, with nothing
+// written, when a row has more than SJW_CAP candidates.  Synchronises.
+constexpr u32 SJW_CAP = 2048;
+bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo, u64 W, DBuf<u64> &out,
+                    u64 *nout);
 // probe ⋈ index, each row inserted; rows that set a clear bit go to `out`
 // (unordered).  Returns their number; *raw += |J|.
 u64 bitmap_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, int proj,

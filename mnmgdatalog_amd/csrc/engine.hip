@@ -495,6 +495,13 @@ void sg_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
 // layout and the W*W-bit map is no larger than the raw join's tuples): every
 // output row built in LDS and read out of the map in tuple order, so the raw
 // join is never materialised or sorted.
+// The row form (strategy 2, or automatic when the map form is not taken):
+// each output row sorted and deduplicated in LDS, for id ranges up to
+// SJ_ROWS_MAX_W wide with dense offsets no more than a few per row, and rows
+// of at most SJW_CAP candidates (else the sorted raw join).
+constexpr u64 SJ_ROWS_MAX_W = 1ull << 27;
+constexpr u64 SJ_ROWS_W_PER_ROW = 4, SJ_ROWS_W_SLACK = 1ull << 20;
+
 static bool sj_use_map(const Opts &o, u64 W, u64 raw) {
     const u64 wpr = (W + 31) / 32;
     if (o.strategy == 4) {
@@ -518,13 +525,16 @@ void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
     if (nrr == 0) return;
     pc.mark(DLG_PH_HASHTABLE);
     const u64 W = (u64)(ib[1] - ib[0]) + 1;
-    if (o.strategy == 4 || (o.strategy == 0 && o.join == 0 && W <= BITMAP_MAX_W)) {
+    const bool try_map = o.strategy == 4 || (o.strategy == 0 && o.join == 0 && W <= BITMAP_MAX_W);
+    const bool try_rows = (o.strategy == 2 || (o.strategy == 0 && o.join == 0)) && W <= SJ_ROWS_MAX_W &&
+                          W <= SJ_ROWS_W_PER_ROW * nrr + SJ_ROWS_W_SLACK;
+    if (try_map || try_rows) {
         Index rx(ctx);   // the (b, a) rows with dense offsets: b's run of a's
         rx.rows.swap(Rr);
         rx.n = nrr;
-        if (W <= BITMAP_FORCED_MAX_W) index_dense(ctx, rx, ib[0], W);
+        if (try_rows || W <= BITMAP_FORCED_MAX_W) index_dense(ctx, rx, ib[0], W);
         const u64 raw = rx.dense_w ? self_join_raw(ctx, rx.rows.p, nrr, rx.dense.p, ib[0]) : 0;
-        if (sj_use_map(o, W, raw)) {
+        if (try_map && sj_use_map(o, W, raw)) {
             pc.mark(DLG_PH_JOIN);
             Bitmap F(ctx);
             bitmap_init(ctx, F, ib[0], ib[1], false, false);
@@ -535,6 +545,15 @@ void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
             return;
         }
         if (o.strategy == 4) fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
+        if (try_rows && rx.dense_w) {
+            pc.mark(DLG_PH_JOIN);
+            u64 nout = 0;
+            if (self_join_rows(ctx, rx.rows.p, nrr, rx.dense.p, ib[0], W, out.full, &nout)) {   // (c, a) in order
+                out.raw += (i64)raw;
+                out.nfull = nout;
+                return;
+            }
+        }
         Rr.swap(rx.rows);
     }
     pc.mark(DLG_PH_INIT);

@@ -857,4 +857,327 @@ void bitmap_self_join(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, Bi
                (unsigned)(4 * bm.wpr), (const u32 *)col.p, dense, bm.wpr, bm.words.p);
 }
 
+// ------------------------------------------------- single join by rows
+// For id ranges too wide for the map, or outputs too sparse for it: output
+// row c is the sorted distinct first columns gathered from c's runs (the
+// a's of (a, b) for each b of (b, c)).  One wave per row gathers its
+// candidates into LDS (64 b's at a time, their runs laid end to end by a
+// wave scan), sorts them there (bitonic), keeps the first of each value and
+// writes them at the row's place in a candidate array; a scan of the kept
+// counts then places each row in the output.  Rows with more than SJW_CAP
+// candidates are left to the sorted raw join (the host checks the maximum).
+constexpr int SJW_T = 256, SJW_W = SJW_T / WAVE;
+
+__device__ __forceinline__ void sjw_sync() {   // LDS written by some lanes of a wave, read by others
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// rraw[c] = c's candidates: the sum over c's run of the run lengths of its b's;
+// the largest to mx[0], the longest run of b's to mx[1].
+__global__ __launch_bounds__(256) void sj_row_raw_k(const u32 *col, const u64 *dense, u64 W, u64 *rraw,
+                                                    unsigned long long *mx) {
+    __shared__ u64 s_m[2][256 / WAVE];
+    u64 m = 0, mb = 0;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 c = (u64)blockIdx.x * blockDim.x + threadIdx.x; c < W; c += stride) {
+        u64 t = 0;
+        const u64 p0 = dense[c], p1 = dense[c + 1];
+        for (u64 p = p0; p < p1; p++) {
+            const u32 b = col[p];
+            t += dense[b + 1] - dense[b];
+        }
+        rraw[c] = t;
+        m = t > m ? t : m;
+        mb = p1 - p0 > mb ? p1 - p0 : mb;
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const u64 x = __shfl_xor(m, o, WAVE), y = __shfl_xor(mb, o, WAVE);
+        m = x > m ? x : m;
+        mb = y > mb ? y : mb;
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        s_m[0][threadIdx.x / WAVE] = m;
+        s_m[1][threadIdx.x / WAVE] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / WAVE; w++) {
+            m = s_m[0][w] > m ? s_m[0][w] : m;
+            mb = s_m[1][w] > mb ? s_m[1][w] : mb;
+        }
+        if (m) atomicMax(mx, (unsigned long long)m);
+        if (mb) atomicMax(mx + 1, (unsigned long long)mb);
+    }
+}
+
+// Bitonic sort, ascending, of SJW_REG * WAVE values held as v[r] = element
+// r WAVE + lane: strides below WAVE exchange with lane ^ j by shuffle, larger
+// ones compare registers of the same lane.
+constexpr int SJW_REG = 4;
+template <int R>
+__device__ __forceinline__ void sjw_sort_regs(u32 (&v)[R], int lane) {
+    constexpr u32 P = R * WAVE;
+#pragma unroll
+    for (u32 k = 2; k <= P; k <<= 1) {
+#pragma unroll
+        for (u32 j = k >> 1; j > 0; j >>= 1) {
+            if (j >= (u32)WAVE) {
+                const u32 m = j / WAVE;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int q = r ^ (int)m;
+                    if (q > r) {
+                        const u32 e = (u32)r * WAVE + (u32)lane;
+                        const bool asc = (e & k) == 0;
+                        const u32 x = v[r], y = v[q];
+                        if ((x > y) == asc) {
+                            v[r] = y;
+                            v[q] = x;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const u32 e = (u32)r * WAVE + (u32)lane;
+                    const bool asc = (e & k) == 0, low = ((u32)lane & j) == 0;
+                    const u32 y = (u32)__shfl_xor((int)v[r], (int)j, WAVE);
+                    const u32 mn = v[r] < y ? v[r] : y, mx = v[r] < y ? y : v[r];
+                    v[r] = low == asc ? mn : mx;
+                }
+            }
+        }
+    }
+}
+
+// Rows the register kernel takes: at most SJW_REG * WAVE candidates from at
+// most WAVE b's (empty rows too).
+__device__ __forceinline__ bool sjw_small(u64 raw, u64 nb) { return raw <= (u64)SJW_REG * WAVE && nb <= (u64)WAVE; }
+
+// A small row's candidates: j = r WAVE + lane found among the lanes' runs
+// (q0, pre: this lane's run start and first candidate) by a binary search
+// over the lanes' starts, loaded into v[r] (R loads in flight), sorted in
+// registers; the first of each value goes to out in order.  Returns their count.
+template <int R>
+__device__ __forceinline__ u32 sjw_small_row(const u32 *col, u32 q0, u32 pre, u64 nb, u64 raw, int lane, u32 *out) {
+    u32 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const u32 j = (u32)r * WAVE + (u32)lane;
+        int s = 0;   // the last lane whose start is <= j (starts rise with the lane; empty runs share one)
+#pragma unroll
+        for (int st = WAVE / 2; st > 0; st >>= 1) {
+            const u32 ps = __shfl(pre, s + st, WAVE);
+            if (s + st < (int)nb && ps <= j) s += st;
+        }
+        const u32 qs = __shfl(q0, s, WAVE), ps = __shfl(pre, s, WAVE);
+        v[r] = (u64)j < raw ? col[(u64)qs + (j - ps)] : 0xffffffffu;
+    }
+    sjw_sort_regs<R>(v, lane);
+    u32 kept = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const u64 e = (u64)r * WAVE + (u64)lane;
+        // element e - 1: lane - 1's v[r], or for lane 0 the last lane's v[r - 1]
+        const u32 up = __shfl(v[r], (lane + WAVE - 1) & (WAVE - 1), WAVE);
+        const u32 wrap = __shfl(v[r > 0 ? r - 1 : 0], WAVE - 1, WAVE);
+        const u32 prev = lane > 0 ? up : wrap;
+        const bool keep = e < raw && (e == 0 || v[r] != prev);
+        const u64 bal = __ballot(keep);
+        if (keep) out[kept + __builtin_amdgcn_mbcnt_hi((u32)(bal >> 32), __builtin_amdgcn_mbcnt_lo((u32)bal, 0u))] = v[r];
+        kept += (u32)__popcll(bal);
+    }
+    return kept;
+}
+
+// Small rows with no LDS: lane l takes the row's l-th b, a wave scan lays the
+// b's runs end to end, and candidate j = r WAVE + lane is found by a binary
+// search over the lanes' starts (by shuffle) and loaded straight into v[r]
+// (SJW_REG loads in flight), then sorted in registers and deduplicated.
+__global__ __launch_bounds__(256) void sj_row_small_k(const u32 *col, const u64 *dense, const u64 *roff, u64 W,
+                                                      u32 *cand, u32 *ucnt) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (u64 c = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; c < W; c += (u64)gridDim.x * blockDim.x / WAVE) {
+        const u64 r0 = roff[c], raw = roff[c + 1] - r0;
+        const u64 p0 = dense[c], nb = dense[c + 1] - p0;
+        if (!sjw_small(raw, nb)) continue;   // sj_row_sort_k's
+        if (raw == 0) {
+            if (lane == 0) ucnt[c] = 0u;
+            continue;
+        }
+        u32 q0 = 0, len = 0;   // this lane's b: its run (a row's b's and runs fit 32 bits here)
+        if ((u64)lane < nb) {
+            const u32 b = col[p0 + (u64)lane];
+            const u64 d0 = dense[b];
+            q0 = (u32)d0;
+            len = (u32)(dense[b + 1] - d0);
+        }
+        u32 inc = len;
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const u32 t = __shfl_up(inc, d, WAVE);
+            if (lane >= d) inc += t;
+        }
+        const u32 pre = inc - len;   // this lane's first candidate
+        u32 kept;
+        if (raw <= (u64)WAVE) kept = sjw_small_row<1>(col, q0, pre, nb, raw, lane, cand + r0);
+        else if (raw <= 2 * (u64)WAVE) kept = sjw_small_row<2>(col, q0, pre, nb, raw, lane, cand + r0);
+        else kept = sjw_small_row<SJW_REG>(col, q0, pre, nb, raw, lane, cand + r0);
+        if (lane == 0) ucnt[c] = kept;
+    }
+}
+
+__global__ __launch_bounds__(SJW_T) void sj_row_sort_k(const u32 *col, const u64 *dense, const u64 *roff, u64 W,
+                                                       u32 *cand, u32 *ucnt) {
+    __shared__ u32 s_buf[SJW_W][SJW_CAP];
+    __shared__ u64 s_pre[SJW_W][WAVE + 1];
+    __shared__ u64 s_q0[SJW_W][WAVE];
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    u32 *buf = s_buf[wv];
+    u64 *pre = s_pre[wv], *q0s = s_q0[wv];
+    for (u64 c = (u64)blockIdx.x * SJW_W + wv; c < W; c += (u64)gridDim.x * SJW_W) {   // wave-uniform
+        const u64 r0 = roff[c], raw = roff[c + 1] - r0;
+        const u64 p1 = dense[c + 1];
+        if (sjw_small(raw, p1 - dense[c])) continue;   // sj_row_small_k's
+        // gather the row's candidates into buf[0, raw)
+        u64 filled = 0;
+        for (u64 pb = dense[c]; pb < p1; pb += WAVE) {
+            u64 q0 = 0, len = 0;
+            if (pb + (u64)lane < p1) {
+                const u32 b = col[pb + (u64)lane];
+                q0 = dense[b];
+                len = dense[b + 1] - q0;
+            }
+            u64 inc = len;   // wave scan of the run lengths
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const u64 t = __shfl_up(inc, d, WAVE);
+                if (lane >= d) inc += t;
+            }
+            const u64 tot = __shfl(inc, WAVE - 1, WAVE);
+            pre[lane] = inc - len;
+            q0s[lane] = q0;
+            if (lane == 0) pre[WAVE] = tot;
+            sjw_sync();
+            u32 sg = 0;
+            for (u64 j = lane; j < tot; j += WAVE) {
+                while (pre[sg + 1] <= j) sg++;
+                buf[filled + j] = col[q0s[sg] + (j - pre[sg])];
+            }
+            filled += tot;
+            sjw_sync();   // pre and q0s are rewritten for the next 64 b's
+        }
+        if (raw <= (u64)SJW_REG * WAVE) {   // sorted in registers: element r WAVE + lane in v[r]
+            sjw_sync();
+            u32 v[SJW_REG];
+#pragma unroll
+            for (int r = 0; r < SJW_REG; r++) {
+                const u64 e = (u64)r * WAVE + (u64)lane;
+                v[r] = e < raw ? buf[e] : 0xffffffffu;
+            }
+            sjw_sort_regs<SJW_REG>(v, lane);
+            u32 kept = 0;
+#pragma unroll
+            for (int r = 0; r < SJW_REG; r++) {
+                const u64 e = (u64)r * WAVE + (u64)lane;
+                // element e - 1: lane - 1's v[r], or for lane 0 the last lane's v[r - 1]
+                const u32 up = __shfl(v[r], (lane + WAVE - 1) & (WAVE - 1), WAVE);
+                const u32 wrap = __shfl(v[r > 0 ? r - 1 : 0], WAVE - 1, WAVE);
+                const u32 prev = lane > 0 ? up : wrap;
+                const bool keep = e < raw && (e == 0 || v[r] != prev);
+                const u64 bal = __ballot(keep);
+                if (keep)
+                    cand[r0 + kept + __builtin_amdgcn_mbcnt_hi((u32)(bal >> 32), __builtin_amdgcn_mbcnt_lo((u32)bal, 0u))] =
+                        v[r];
+                kept += (u32)__popcll(bal);
+            }
+            if (lane == 0) ucnt[c] = kept;
+            sjw_sync();   // buf is rewritten for the next row
+            continue;
+        }
+        // bitonic sort of buf[0, P), P the next power of two, padded with the largest value
+        u32 P = 2;
+        while ((u64)P < raw) P <<= 1;
+        for (u32 j = (u32)raw + lane; j < P; j += WAVE) buf[j] = 0xffffffffu;
+        sjw_sync();
+        for (u32 k = 2; k <= P; k <<= 1) {
+            for (u32 jj = k >> 1; jj > 0; jj >>= 1) {
+                for (u32 i = lane; i < P; i += WAVE) {
+                    const u32 l = i ^ jj;
+                    if (l > i) {
+                        const u32 x = buf[i], y = buf[l];
+                        if ((x > y) == ((i & k) == 0)) {
+                            buf[i] = y;
+                            buf[l] = x;
+                        }
+                    }
+                }
+                sjw_sync();
+            }
+        }
+        // the first of each value, in order, at the row's place in cand
+        u32 kept = 0;
+        for (u64 j0 = 0; j0 < raw; j0 += WAVE) {
+            const u64 j = j0 + (u64)lane;
+            const u32 v = j < raw ? buf[j] : 0u;
+            const bool keep = j < raw && (j == 0 || buf[j - 1] != v);
+            const u64 bal = __ballot(keep);
+            if (keep)
+                cand[r0 + kept + __builtin_amdgcn_mbcnt_hi((u32)(bal >> 32), __builtin_amdgcn_mbcnt_lo((u32)bal, 0u))] = v;
+            kept += (u32)__popcll(bal);
+        }
+        if (lane == 0) ucnt[c] = kept;
+        sjw_sync();   // buf is rewritten for the next row
+    }
+}
+
+// out[uoff[c] ..] = (c, a) for the kept a's of row c: one wave per row.
+__global__ __launch_bounds__(256) void sj_row_emit_k(const u32 *cand, const u64 *roff, const u64 *uoff, u64 W, u32 lo,
+                                                     u64 *out) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (u64 c = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; c < W; c += (u64)gridDim.x * blockDim.x / WAVE) {
+        const u64 r0 = roff[c], o0 = uoff[c], m = uoff[c + 1] - o0;
+        for (u64 k = lane; k < m; k += WAVE) out[o0 + k] = tmake(lo + (u32)c, lo + cand[r0 + k]);
+    }
+}
+
+bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo, u64 W, DBuf<u64> &out,
+                    u64 *nout) {
+    *nout = 0;
+    if (n == 0 || W == 0) return true;
+    DBuf<u32> col(ctx, n);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, sj_col_k, sgrid(ctx, n), ctx->block_size, 0, rows, n, lo, col.p);
+    DBuf<u64> roff(ctx, W + 1), mx(ctx, 2);
+    fill_u64(ctx, mx.p, 2, 0ull);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * W + 12.0 * n, sj_row_raw_k, sgrid(ctx, W), 256, 0, (const u32 *)col.p, dense,
+               W, roff.p, (unsigned long long *)mx.p);
+    u64 hm[2];
+    ctx->read_scalars(mx.p, 2, hm);
+    const u64 m = hm[0];
+    const bool big_b = hm[1] > (u64)WAVE;   // a row with more b's than the register kernel takes
+    if (m > SJW_CAP) return false;
+    scan_u64_async(ctx, roff.p, W, roff.p, roff.p + W);   // roff[W] = all candidates
+    u64 total = 0;
+    ctx->read_scalars(roff.p + W, 1, &total);
+    DBuf<u32> cand(ctx, total ? total : 1), ucnt(ctx, W);
+    // the wave-per-row kernels: a wave per row, up to eight 4-wave workgroups per CU
+    const unsigned g = (unsigned)std::min<u64>((W + SJW_W - 1) / SJW_W, (u64)ctx->num_cus * 8);
+    DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 8.0 * total + 8.0 * n + 16.0 * W, sj_row_small_k, g, 256, 0,
+               (const u32 *)col.p, dense, (const u64 *)roff.p, W, cand.p, ucnt.p);
+    if (m > (u64)SJW_REG * WAVE || big_b)
+        DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 16.0 * W, sj_row_sort_k, g, SJW_T, 0, (const u32 *)col.p, dense,
+                   (const u64 *)roff.p, W, cand.p, ucnt.p);
+    DBuf<u64> uoff(ctx, W + 1);
+    const u64 h = scan_u32_u64(ctx, ucnt.p, W, uoff.p);
+    fill_u64(ctx, uoff.p + W, 1, h);
+    out.ensure(h ? h : 1);
+    if (h)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * h + 24.0 * W, sj_row_emit_k, g, 256, 0,
+                   (const u32 *)cand.p, (const u64 *)roff.p, (const u64 *)uoff.p, W, lo, out.p);
+    *nout = h;
+    return true;
+}
+
 }  // namespace dlg

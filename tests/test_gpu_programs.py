@@ -239,15 +239,65 @@ def test_single_join_published(ctx, rows, rng_, out):
                                          (7, 8, 10, 3), (-2000, 3000, 4000, 4), (0, 64, 20000, 5)])
 def test_single_join_map_and_sorted_agree(ctx, orc, lo, hi, n, seed):
     """The map form (each output row built in LDS, read out of the map in
-    order) against the sorted raw join (strategy 1) and the oracle: rows,
-    sizes and raw join rows, automatic and forced (strategy 4)."""
+    order) and the row form (each row's candidates sorted in LDS) against the
+    sorted raw join (strategy 1) and the oracle: rows, sizes and raw join
+    rows, automatic and forced (strategies 4 and 2)."""
     rng = np.random.default_rng(700 + seed)
     rows = rng.integers(lo, hi, size=(n, 2)).astype(np.int32)
     want = orc.single_join(rows)
-    runs = [ctx.run("single_join", rows, strategy=s) for s in (0, 1, 4)]
+    runs = [ctx.run("single_join", rows, strategy=s) for s in (0, 1, 2, 4)]
     for r in runs:
         assert_same(r, want)
     assert len({r["raw_join_rows"] for r in runs}) == 1
+
+
+@pytest.mark.parametrize("lo,hi,n,seed", [(0, 1_000_000, 60_000, 0), (-500_000, 500_000, 200_000, 1),
+                                         (0, 300_000, 30_000, 2), (5, 70_000, 400_000, 3)])
+def test_single_join_row_form_wide_ids(ctx, orc, lo, hi, n, seed):
+    """Id ranges too wide for the map: the row form (automatic) against the
+    oracle and the sorted raw join."""
+    rng = np.random.default_rng(900 + seed)
+    rows = rng.integers(lo, hi, size=(n, 2)).astype(np.int32)
+    want = orc.single_join(rows)
+    auto, srt = ctx.run("single_join", rows), ctx.run("single_join", rows, strategy=1)
+    assert_same(auto, want)
+    assert_same(srt, want)
+    assert auto["raw_join_rows"] == srt["raw_join_rows"]
+
+
+def test_single_join_row_form_row_shapes(ctx, orc):
+    """Rows of every shape the row form sorts: more than 64 b's with more
+    than 256 candidates (bitonic sort in LDS), more than 64 b's with few
+    candidates, and small rows (sorted in registers), over an id range too
+    wide for the map."""
+    rng = np.random.default_rng(21)
+    parts = []
+    for c in range(60):                      # 100 b's of 3 a's each: 300 candidates
+        bs = 10_000 + c * 100 + np.arange(100)
+        parts.append(np.stack([bs, np.full(100, c)], 1))
+        parts.append(np.stack([rng.integers(200_000, 300_000, 300), np.repeat(bs, 3)], 1))
+    for c in range(60, 90):                  # 70 b's of 2 a's each: 140 candidates
+        bs = 20_000 + c * 70 + np.arange(70)
+        parts.append(np.stack([bs, np.full(70, c)], 1))
+        parts.append(np.stack([rng.integers(200_000, 300_000, 140), np.repeat(bs, 2)], 1))
+    parts.append(rng.integers(0, 300_000, size=(30_000, 2)))   # small rows
+    rows = np.concatenate(parts).astype(np.int32)
+    want = orc.single_join(rows)
+    for st in (0, 2, 1):
+        assert_same(ctx.run("single_join", rows, strategy=st), want)
+
+
+def test_single_join_row_form_heavy_row_falls_back(ctx, orc):
+    """A hub whose rows have 3,000 candidates each (above the 2,048 a wave
+    sorts in LDS): the join falls back to the sorted raw join, same result."""
+    rng = np.random.default_rng(11)
+    hub = np.stack([np.arange(1, 3001), np.zeros(3000, np.int64)], 1)          # (a, 0)
+    out = np.stack([np.zeros(5, np.int64), np.arange(3001, 3006)], 1)          # (0, c)
+    noise = rng.integers(0, 400_000, size=(20_000, 2))
+    rows = np.concatenate([hub, out, noise]).astype(np.int32)
+    want = orc.single_join(rows)
+    for st in (0, 2):
+        assert_same(ctx.run("single_join", rows, strategy=st), want)
 
 
 def test_single_join_map_range_100k_equals_sorted(ctx):
