@@ -415,11 +415,12 @@ void bitmap_self_join(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, Bi
 u64 self_join_raw(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo);
 // The same single join by rows: each output row's candidates sorted and
 // deduplicated in LDS by one wave, the rows placed by a scan, (c, a) tuples
-// in order into out (resized), *nout of them.  Returns false, with nothing
-// written, when a row has more than SJW_CAP candidates.  Synchronises.
+// in order into out (resized), *nout of them, and the raw join's size into
+// *raw.  Returns false, with nothing written, when a row has more than
+// SJW_CAP candidates.  Synchronises.
 constexpr u32 SJW_CAP = 2048;
 bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo, u64 W, DBuf<u64> &out,
-                    u64 *nout);
+                    u64 *nout, u64 *raw);
 // probe ⋈ index, each row inserted; rows that set a clear bit go to `out`
 // (unordered).  Returns their number; *raw += |J|.
 u64 bitmap_join_insert(dlg_ctx *ctx, const Index &ix, const u64 *probe, u64 n, JoinScratch &js, int proj,

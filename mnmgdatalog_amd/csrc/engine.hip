@@ -533,7 +533,7 @@ void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         rx.rows.swap(Rr);
         rx.n = nrr;
         if (try_rows || W <= BITMAP_FORCED_MAX_W) index_dense(ctx, rx, ib[0], W);
-        const u64 raw = rx.dense_w ? self_join_raw(ctx, rx.rows.p, nrr, rx.dense.p, ib[0]) : 0;
+        const u64 raw = rx.dense_w && try_map ? self_join_raw(ctx, rx.rows.p, nrr, rx.dense.p, ib[0]) : 0;
         if (try_map && sj_use_map(o, W, raw)) {
             pc.mark(DLG_PH_JOIN);
             Bitmap F(ctx);
@@ -547,9 +547,9 @@ void sj_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
         if (o.strategy == 4) fail(DLG_INVALID_INPUT, "bitmap layout: id range too wide");
         if (try_rows && rx.dense_w) {
             pc.mark(DLG_PH_JOIN);
-            u64 nout = 0;
-            if (self_join_rows(ctx, rx.rows.p, nrr, rx.dense.p, ib[0], W, out.full, &nout)) {   // (c, a) in order
-                out.raw += (i64)raw;
+            u64 nout = 0, rraw = 0;
+            if (self_join_rows(ctx, rx.rows.p, nrr, rx.dense.p, ib[0], W, out.full, &nout, &rraw)) {   // (c, a)
+                out.raw += (i64)rraw;
                 out.nfull = nout;
                 return;
             }

@@ -1144,8 +1144,9 @@ __global__ __launch_bounds__(256) void sj_row_emit_k(const u32 *cand, const u64 
 }
 
 bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo, u64 W, DBuf<u64> &out,
-                    u64 *nout) {
+                    u64 *nout, u64 *raw) {
     *nout = 0;
+    *raw = 0;
     if (n == 0 || W == 0) return true;
     DBuf<u32> col(ctx, n);
     DLG_LAUNCH(ctx, DLG_K_MISC, 12.0 * n, sj_col_k, sgrid(ctx, n), ctx->block_size, 0, rows, n, lo, col.p);
@@ -1161,6 +1162,7 @@ bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 
     scan_u64_async(ctx, roff.p, W, roff.p, roff.p + W);   // roff[W] = all candidates
     u64 total = 0;
     ctx->read_scalars(roff.p + W, 1, &total);
+    *raw = total;
     DBuf<u32> cand(ctx, total ? total : 1), ucnt(ctx, W);
     // the wave-per-row kernels: a wave per row, up to eight 4-wave workgroups per CU
     const unsigned g = (unsigned)std::min<u64>((W + SJW_W - 1) / SJW_W, (u64)ctx->num_cus * 8);
