@@ -194,9 +194,10 @@ typedef struct dlg_opts {
                                 map, no raw join or sort (automatic up to 2^18 ids
                                 when the map is no larger than the raw join).
                                 Single join, 2: each output row's candidates sorted
-                                and deduplicated in LDS by one wave (automatic when
-                                the map is not taken and no row has more than 2,048
-                                candidates); otherwise the sorted raw join            */
+                                and deduplicated by one wave, rows of more than 2,048
+                                candidates sorted together (automatic when the map
+                                is not taken and the id range allows dense offsets);
+                                otherwise the sorted raw join                          */
     int32_t max_iterations;  /* 0 = run to the fixpoint; >0 stops early (sampling)     */
     int32_t keep_output;     /* 1: copy the output rows to host (dlg_result.rows)      */
     int32_t check_output;    /* 1: fill dlg_result.output_sum / output_order_breaks

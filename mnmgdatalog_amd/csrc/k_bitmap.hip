@@ -1040,7 +1040,7 @@ __global__ __launch_bounds__(SJW_T) void sj_row_sort_k(const u32 *col, const u64
     for (u64 c = (u64)blockIdx.x * SJW_W + wv; c < W; c += (u64)gridDim.x * SJW_W) {   // wave-uniform
         const u64 r0 = roff[c], raw = roff[c + 1] - r0;
         const u64 p1 = dense[c + 1];
-        if (sjw_small(raw, p1 - dense[c])) continue;   // sj_row_small_k's
+        if (sjw_small(raw, p1 - dense[c]) || raw > SJW_CAP) continue;   // sj_row_small_k's, or a heavy row
         // gather the row's candidates into buf[0, raw)
         u64 filled = 0;
         for (u64 pb = dense[c]; pb < p1; pb += WAVE) {
@@ -1143,6 +1143,78 @@ __global__ __launch_bounds__(256) void sj_row_emit_k(const u32 *cand, const u64 
     }
 }
 
+// Heavy rows (more than SJW_CAP candidates) go through a sort: hraw[c] =
+// the row's candidates if it is heavy, else 0 (scanned into the heavy rows'
+// offsets in J).
+__global__ void sj_heavy_raw_k(const u64 *roff, u64 W, u64 *hraw) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 c = (u64)blockIdx.x * blockDim.x + threadIdx.x; c < W; c += stride) {
+        const u64 r = roff[c + 1] - roff[c];
+        hraw[c] = r > SJW_CAP ? r : 0;
+    }
+}
+
+// A heavy row's candidates as (c, a) tuples at J[hoff[c] ..]: one wave per
+// row, 64 b's at a time, candidate j found among the lanes' runs by a binary
+// search over their starts.
+__global__ __launch_bounds__(256) void sj_heavy_gather_k(const u32 *col, const u64 *dense, const u64 *hoff, u64 W,
+                                                         u32 lo, u64 *J) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (u64 c = ((u64)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; c < W; c += (u64)gridDim.x * blockDim.x / WAVE) {
+        const u64 h0 = hoff[c];
+        if (hoff[c + 1] == h0) continue;   // a light row
+        const u64 p1 = dense[c + 1];
+        u64 filled = 0;
+        for (u64 pb = dense[c]; pb < p1; pb += WAVE) {
+            const u64 nb = p1 - pb < (u64)WAVE ? p1 - pb : (u64)WAVE;
+            u64 q0 = 0, len = 0;
+            if ((u64)lane < nb) {
+                const u32 b = col[pb + (u64)lane];
+                q0 = dense[b];
+                len = dense[b + 1] - q0;
+            }
+            u64 inc = len;
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const u64 t = __shfl_up(inc, d, WAVE);
+                if (lane >= d) inc += t;
+            }
+            const u64 tot = __shfl(inc, WAVE - 1, WAVE), pre = inc - len;
+            for (u64 j0 = 0; j0 < tot; j0 += WAVE) {
+                const u64 j = j0 + (u64)lane;
+                int sidx = 0;   // the last lane whose start is <= j
+#pragma unroll
+                for (int st = WAVE / 2; st > 0; st >>= 1) {
+                    const u64 ps = __shfl(pre, sidx + st, WAVE);
+                    if (sidx + st < (int)nb && ps <= j) sidx += st;
+                }
+                const u64 qs = __shfl(q0, sidx, WAVE), ps = __shfl(pre, sidx, WAVE);
+                if (j < tot) J[h0 + filled + j] = tmake(lo + (u32)c, lo + col[qs + (j - ps)]);
+            }
+            filled += tot;
+        }
+    }
+}
+
+// H = the heavy rows' distinct (c, a), sorted.  First the head of each row's
+// run to head[c], then each a to the row's place in cand and the run's length
+// to ucnt[c] (from its last element).
+__global__ void sj_heavy_heads_k(const u64 *H, u64 nh, u32 lo, u32 *head) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += stride)
+        if (i == 0 || tkey(H[i]) != tkey(H[i - 1])) head[tkey(H[i]) - lo] = (u32)i;
+}
+__global__ void sj_heavy_place_k(const u64 *H, u64 nh, u32 lo, const u32 *head, const u64 *roff, u32 *cand,
+                                 u32 *ucnt) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += stride) {
+        const u64 x = H[i];
+        const u32 c = tkey(x) - lo, k = (u32)i - head[c];
+        cand[roff[c] + k] = tval(x) - lo;
+        if (i + 1 == nh || tkey(H[i + 1]) != tkey(x)) ucnt[c] = k + 1;
+    }
+}
+
 bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo, u64 W, DBuf<u64> &out,
                     u64 *nout, u64 *raw) {
     *nout = 0;
@@ -1158,7 +1230,6 @@ bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 
     ctx->read_scalars(mx.p, 2, hm);
     const u64 m = hm[0];
     const bool big_b = hm[1] > (u64)WAVE;   // a row with more b's than the register kernel takes
-    if (m > SJW_CAP) return false;
     scan_u64_async(ctx, roff.p, W, roff.p, roff.p + W);   // roff[W] = all candidates
     u64 total = 0;
     ctx->read_scalars(roff.p + W, 1, &total);
@@ -1171,6 +1242,26 @@ bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 
     if (m > (u64)SJW_REG * WAVE || big_b)
         DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 16.0 * W, sj_row_sort_k, g, SJW_T, 0, (const u32 *)col.p, dense,
                    (const u64 *)roff.p, W, cand.p, ucnt.p);
+    if (m > SJW_CAP) {   // heavy rows: their candidates materialised, sorted and deduplicated
+        DBuf<u64> hoff(ctx, W + 1);
+        DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * W, sj_heavy_raw_k, sgrid(ctx, W), ctx->block_size, 0, (const u64 *)roff.p,
+                   W, hoff.p);
+        scan_u64_async(ctx, hoff.p, W, hoff.p, hoff.p + W);
+        u64 nj = 0;
+        ctx->read_scalars(hoff.p + W, 1, &nj);
+        if (nj > 0xffffffffull) return false;   // beyond one sorted buffer: the caller's sorted join
+        DBuf<u64> J(ctx, nj), tmp(ctx), H(ctx, nj);
+        DLG_LAUNCH(ctx, DLG_K_JOIN_EXPAND, 8.0 * nj + 16.0 * W, sj_heavy_gather_k, g, 256, 0, (const u32 *)col.p,
+                   dense, (const u64 *)hoff.p, W, lo, J.p);
+        const u32 hb[4] = {lo, (u32)(lo + W - 1), lo, (u32)(lo + W - 1)};
+        radix_sort(ctx, J, nj, tmp, 0, hb);
+        const u64 nh = select(ctx, SEL_UNIQUE, J.p, nj, nullptr, 0, H.p);
+        DBuf<u32> head(ctx, W);
+        DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * nh, sj_heavy_heads_k, sgrid(ctx, nh), ctx->block_size, 0,
+                   (const u64 *)H.p, nh, lo, head.p);
+        DLG_LAUNCH(ctx, DLG_K_MISC, 24.0 * nh, sj_heavy_place_k, sgrid(ctx, nh), ctx->block_size, 0,
+                   (const u64 *)H.p, nh, lo, (const u32 *)head.p, (const u64 *)roff.p, cand.p, ucnt.p);
+    }
     DBuf<u64> uoff(ctx, W + 1);
     const u64 h = scan_u32_u64(ctx, ucnt.p, W, uoff.p);
     fill_u64(ctx, uoff.p + W, 1, h);

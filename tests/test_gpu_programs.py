@@ -287,9 +287,10 @@ def test_single_join_row_form_row_shapes(ctx, orc):
         assert_same(ctx.run("single_join", rows, strategy=st), want)
 
 
-def test_single_join_row_form_heavy_row_falls_back(ctx, orc):
+def test_single_join_row_form_heavy_rows(ctx, orc):
     """A hub whose rows have 3,000 candidates each (above the 2,048 a wave
-    sorts in LDS): the join falls back to the sorted raw join, same result."""
+    sorts in LDS): those rows are materialised, sorted and deduplicated
+    together, the others sorted by their waves; same result as the oracle."""
     rng = np.random.default_rng(11)
     hub = np.stack([np.arange(1, 3001), np.zeros(3000, np.int64)], 1)          # (a, 0)
     out = np.stack([np.zeros(5, np.int64), np.arange(3001, 3006)], 1)          # (0, c)
@@ -298,6 +299,21 @@ def test_single_join_row_form_heavy_row_falls_back(ctx, orc):
     want = orc.single_join(rows)
     for st in (0, 2):
         assert_same(ctx.run("single_join", rows, strategy=st), want)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_single_join_row_form_power_law(ctx, orc, seed):
+    """Power-law degrees over a wide id range: many rows above the LDS cap
+    beside light ones, against the oracle and the sorted raw join."""
+    rng = np.random.default_rng(1300 + seed)
+    n = 20_000
+    hubs = rng.zipf(1.6, size=(n, 2)).clip(max=400_000) - 1      # heavy-tailed ids, many repeats
+    rows = np.where(rng.random((n, 2)) < 0.5, hubs, rng.integers(0, 400_000, size=(n, 2))).astype(np.int32)
+    want = orc.single_join(rows)
+    auto, srt = ctx.run("single_join", rows), ctx.run("single_join", rows, strategy=1)
+    assert_same(auto, want)
+    assert_same(srt, want)
+    assert auto["raw_join_rows"] == srt["raw_join_rows"]
 
 
 def test_single_join_map_range_100k_equals_sorted(ctx):
