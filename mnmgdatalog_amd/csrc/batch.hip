@@ -235,6 +235,36 @@ __global__ void key_hist_k(const u64 *a, u64 n, const u32 *ub, u32 B, u32 *cnt) 
         if (s_cnt[j]) atomicAdd(&cnt[j], s_cnt[j]);
 }
 
+// The same counts from a round's sources: source ax[i] adds len[i].
+__global__ void source_hist_k(const u32 *ax, const u32 *len, u64 n, const u32 *ub, u32 B, u32 *cnt) {
+    __shared__ u32 s_ub[MAX_LDS_INST + 1];
+    __shared__ u32 s_cnt[MAX_LDS_INST];
+    const bool lds = B <= MAX_LDS_INST;
+    if (lds)
+        for (u32 i = threadIdx.x; i <= B; i += blockDim.x) {
+            s_ub[i] = ub[i];
+            if (i < B) s_cnt[i] = 0u;
+        }
+    __syncthreads();
+    const u32 *tab = lds ? s_ub : ub;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32 k = ax[i], w = len[i];
+        if (!w) continue;
+        u32 lo = 0, hi = B;   // tab[0] <= every key; the last range is open-ended
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) >> 1;
+            if (tab[mid] <= k) lo = mid; else hi = mid;
+        }
+        if (lds) atomicAdd(&s_cnt[lo], w);
+        else atomicAdd(&cnt[lo], w);
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (u32 j = threadIdx.x; j < B; j += blockDim.x)
+        if (s_cnt[j]) atomicAdd(&cnt[j], s_cnt[j]);
+}
+
 struct BatchHook : RoundHook {
     const DBuf<u32> &ub;
     int B;
@@ -258,6 +288,24 @@ struct BatchHook : RoundHook {
             ctx->sync();
             for (int i = 0; i < B; i++) c[i] = h[i];
         }
+        record(c, round);
+    }
+    bool by_source() const override { return true; }
+    void on_round_sources(dlg_ctx *ctx, const u32 *ax, const u32 *len, u64 nact, u64 nn, int round) override {
+        std::vector<u64> c(B, 0);
+        if (nn && nact) {
+            fill_u32(ctx, cnt.p, B, 0u);
+            DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nact, source_hist_k, sgrid(ctx, nact), ctx->block_size, 0, ax, len,
+                       nact, (const u32 *)ub.p, (u32)B, cnt.p);
+            std::vector<u32> h(B);
+            HIPCHK(hipMemcpyAsync(h.data(), cnt.p, 4 * B, hipMemcpyDeviceToHost, ctx->stream));
+            ctx->sync();
+            for (int i = 0; i < B; i++) c[i] = h[i];
+        }
+        record(c, round);
+    }
+    // an instance's iteration count is the first round that gave it nothing new
+    void record(const std::vector<u64> &c, int round) {
         for (int i = 0; i < B; i++) {
             if (done[i]) continue;
             iters[i] = round;

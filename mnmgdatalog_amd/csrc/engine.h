@@ -56,6 +56,12 @@ inline double secs(Clock::time_point a, Clock::time_point b) {
 struct RoundHook {
     virtual ~RoundHook() {}
     virtual void on_round(dlg_ctx *ctx, const u64 *N, u64 nn, int round, bool sorted) = 0;
+    // TC in the bitmap layout keeps a round's new tuples grouped by source:
+    // a hook that only counts them may take the sources instead (nact of
+    // them, ids in the tuples' packed form, source i with len[i] new tuples;
+    // nn in all), so the tuples are never built.
+    virtual bool by_source() const { return false; }
+    virtual void on_round_sources(dlg_ctx *, const u32 *, const u32 *, u64, u64, int) {}
 };
 
 // Id-range width up to which TC / SG keep the full relation as a bitmap

@@ -358,8 +358,12 @@ void tc_fixpoint(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, Phas
                 u64 nn = tc_rows_round(ctx, F, tr, &raw);
                 total += nn;
                 it++;
-                tc_rows_tuples(ctx, tr, N);
-                hook->on_round(ctx, N.p, nn, it, false);
+                if (hook->by_source()) {   // the new ends by source, no tuples built
+                    hook->on_round_sources(ctx, tr.ax.p, tr.alen.p, nn ? tr.nact : 0, nn, it);
+                } else {
+                    tc_rows_tuples(ctx, tr, N);
+                    hook->on_round(ctx, N.p, nn, it, false);
+                }
                 if (nn == 0) break;                     // |F| unchanged (tc.cu:294-300)
                 if (o.max_it > 0 && it >= o.max_it) break;
             }
