@@ -445,6 +445,16 @@ void bitmap_clear_list(dlg_ctx *ctx, const u64 *in, u64 n, Bitmap &bm);
 // lies.
 u64 bitmap_emit(dlg_ctx *ctx, Bitmap &bm, DBuf<u64> &out, u64 expect, bool symmetric);
 
+// Several maps emitted at once, in the order given, each in bit order (ids
+// from lo, W rows of wpr words at words + base; transposed back when
+// `transposed`): one count, one scan and one emission for all of them.
+// `expect` = their set bits (checked).  Returns the count.
+struct MapRef {
+    u32 lo;
+    u64 W, wpr, base;
+};
+u64 bitmap_emit_maps(dlg_ctx *ctx, const u32 *words, const std::vector<MapRef> &maps, bool transposed, u64 *out,
+                     u64 expect);
 // A W-row block of a map, emitted in tuple order at out through a padded
 // scratch copy (transposed back when `transposed`).  Returns its set bits.
 u64 bitmap_emit_block(dlg_ctx *ctx, const u32 *words, u32 lo, u64 W, u64 wpr, bool transposed, DBuf<u32> &scratch,

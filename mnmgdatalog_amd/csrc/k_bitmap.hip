@@ -356,15 +356,32 @@ __global__ __launch_bounds__(BE_T) void bm_count_k(const u32 *words, u64 *bsum) 
 // denser tile gets a position for every word (in LDS), then each wave walks
 // its 512 words two at a time, one lane per bit, so the set bits of a word
 // pair are written to consecutive addresses.
+// MULTI: the words are several maps, each starting at a whole tile: map m
+// covers tiles [mt[m], mt[m + 1]) with ids from mlo[m] and mwpr[m] words per
+// row (nm maps); each workgroup finds its tile's map by a binary search.
 constexpr int BE_STAGE = 4096;
-template <bool SWAP, bool CLEAR>
-__global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u32 lo, u64 wpr, u64 *out) {
+template <bool SWAP, bool CLEAR, bool MULTI = false>
+__global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u32 lo, u64 wpr, u64 *out,
+                                                  const u64 *mt = nullptr, const u32 *mlo = nullptr,
+                                                  const u64 *mwpr = nullptr, u32 nm = 0) {
     __shared__ u64 sc[BE_T / WAVE + 1];
     __shared__ u64 s_buf[BE_STAGE];   // the stage, or the dense walk's three word tables
     static_assert(3 * BE_TILE * 4 <= BE_STAGE * 8, "word tables must fit the stage");
     u32 *s_w = (u32 *)s_buf, *s_pos = s_w + BE_TILE, *s_row = s_pos + BE_TILE;
     const u64 tile = (u64)blockIdx.x * BE_TILE;
     const u64 base = tile + (u64)threadIdx.x * BE_W;
+    u64 mbase = 0;   // the tile's map's first word
+    if (MULTI) {
+        u32 a = 0, b = nm;   // the last map starting at or before this tile
+        while (b - a > 1) {
+            const u32 m = (a + b) >> 1;
+            if (mt[m] <= (u64)blockIdx.x) a = m; else b = m;
+        }
+        mbase = mt[a] * BE_TILE;
+        lo = mlo[a];
+        wpr = mwpr[a];
+    }
+    const u64 rtile = tile - mbase, rbase = base - mbase;   // word indexes within the map
     u32 w[BE_W];
     load8(words, base, w);
     u64 c = 0;
@@ -379,7 +396,7 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
     }
     if (tot == 0) return;
     if (tot <= (u64)(2 * BE_STAGE)) {   // uniform: staged, one or two rounds
-        const u64 k0 = base / wpr, cw0 = base - k0 * wpr;
+        const u64 k0 = rbase / wpr, cw0 = rbase - k0 * wpr;
         for (u64 r0 = 0; r0 < tot; r0 += BE_STAGE) {
             const u64 r1 = r0 + BE_STAGE < tot ? r0 + BE_STAGE : tot;
             if (c && rel < r1 && rel + c > r0) {
@@ -409,7 +426,7 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
     }
     {
         u64 pos = rel;
-        u64 k = base / wpr, cw = base - k * wpr;   // one division per thread; the words are consecutive
+        u64 k = rbase / wpr, cw = rbase - k * wpr;   // one division per thread; the words are consecutive
 #pragma unroll
         for (int i = 0; i < BE_W; i++) {
             s_w[threadIdx.x * BE_W + i] = w[i];
@@ -433,7 +450,7 @@ __global__ __launch_bounds__(BE_T) void bm_emit_k(u32 *words, const u64 *bpre, u
         const u32 x = half ? x1 : x0;
         if (!((x >> bit) & 1u)) continue;
         const u32 k = s_row[j + half];                                       // row
-        const u32 v = (u32)(tile + j + half - (u64)k * wpr) * 32u + bit;      // column
+        const u32 v = (u32)(rtile + j + half - (u64)k * wpr) * 32u + bit;     // column
         out[s_pos[j + half] + __popc(x & below)] = SWAP ? tmake(lo + v, lo + k) : tmake(lo + k, lo + v);
     }
 }
@@ -695,6 +712,41 @@ u64 bitmap_emit_block(dlg_ctx *ctx, const u32 *words, u32 lo, u64 W, u64 wpr, bo
     u64 h = words_count(ctx, scratch.p, nwords, bsum, nb);
     if (h) emit_pass<false, false>(ctx, lo, wpr, scratch.p, bsum, nb, h, out);
     return h;
+}
+
+// Several maps at once.  Map m: W[m] rows of wpr[m] words at src + sb[m],
+// written transposed to dst + db[m]; 32 x 32-bit block q of all the maps'
+// blocks (cum[m] of them before map m) is map m's block q - cum[m], one half
+// wave each, as bm_transpose_small_k does for one map.
+__global__ void bm_transpose_small_multi_k(const u32 *src, u32 *dst, const u64 *cum, const u64 *W, const u64 *wpr,
+                                           const u64 *sb, const u64 *db, u32 nm) {
+    const int lane = threadIdx.x & 31;
+    const u64 q = ((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 5;   // one block per half wave
+    if (q >= cum[nm]) return;                                             // whole half waves leave together
+    u32 a = 0, b = nm;   // the last map whose blocks start at or before q
+    while (b - a > 1) {
+        const u32 m = (a + b) >> 1;
+        if (cum[m] <= q) a = m; else b = m;
+    }
+    const u64 blk = q - cum[a], wp = wpr[a], Wm = W[a];
+    const u64 R = blk / wp, C = blk - R * wp;   // source block: rows 32R.., word column C
+    const u64 r = R * 32 + lane;
+    u32 x = r < Wm ? src[sb[a] + r * wp + C] : 0u;
+    const u32 m5[5] = {0x0000ffffu, 0x00ff00ffu, 0x0f0f0f0fu, 0x33333333u, 0x55555555u};
+#pragma unroll
+    for (int st = 0; st < 5; st++) {
+        const int j = 16 >> st;
+        const u32 y = __shfl_xor(x, j, 32);
+        x = (lane & j) ? ((x & ~m5[st]) | ((y & ~m5[st]) >> j)) : ((x & m5[st]) | ((y & m5[st]) << j));
+    }
+    const u64 c = C * 32 + lane;   // destination row = source column
+    if (c < Wm) dst[db[a] + c * wp + R] = x;
+}
+
+// Zero map m's words past its own in its last tile: [db[m] + nw[m], db[m + 1]).
+__global__ void bm_zero_tails_k(u32 *words, const u64 *db, const u64 *nw, u32 nm) {
+    for (u32 m = blockIdx.x; m < nm; m += gridDim.x)
+        for (u64 i = db[m] + nw[m] + threadIdx.x; i < db[m + 1]; i += blockDim.x) words[i] = 0u;
 }
 
 // dst row r = src row sigma[r] (rows of wpr words): one workgroup per 8 rows, coalesced.
@@ -1271,6 +1323,84 @@ bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 
                    (const u32 *)cand.p, (const u64 *)roff.p, (const u64 *)uoff.p, W, lo, out.p);
     *nout = h;
     return true;
+}
+
+// ------------------------------------------------- many maps, one emission
+// The maps each at a whole tile of one scratch (transposed back when
+// `transposed`), one count pass and one scan over all the tiles, and one
+// emission launch whose workgroups find their tile's map: no launch or
+// read-back per map.  Maps in the order given, each in bit order, so the
+// output is sorted when the maps' id ranges rise.
+u64 bitmap_emit_maps(dlg_ctx *ctx, const u32 *words, const std::vector<MapRef> &maps, bool transposed, u64 *out,
+                     u64 expect) {
+    const u32 nm = (u32)maps.size();
+    if (nm == 0) return 0;
+    std::vector<u64> db(nm + 1), nw(nm), W(nm), wpr(nm), sb(nm), cum(nm + 1), mt(nm + 1);
+    std::vector<u32> lo(nm);
+    u64 t = 0, q = 0;
+    for (u32 m = 0; m < nm; m++) {
+        const MapRef &r = maps[m];
+        mt[m] = t;
+        db[m] = t * BE_TILE;
+        nw[m] = r.W * r.wpr;
+        W[m] = r.W;
+        wpr[m] = r.wpr;
+        sb[m] = r.base;
+        lo[m] = r.lo;
+        cum[m] = q;
+        if (r.wpr <= BT_SMALL_WPR) q += r.wpr * r.wpr;
+        t += (nw[m] + BE_TILE - 1) / BE_TILE;
+    }
+    mt[nm] = t;
+    db[nm] = t * BE_TILE;
+    cum[nm] = q;
+    if (t == 0) return 0;
+    // the tables: db, nw, W, wpr, sb, cum, mt (u64) and lo (u32)
+    const size_t L = nm + 1;
+    DBuf<u64> tab(ctx, 7 * L + (L + 1) / 2);
+    u64 *d_db = tab.p, *d_nw = d_db + L, *d_W = d_nw + L, *d_wpr = d_W + L, *d_sb = d_wpr + L, *d_cum = d_sb + L,
+        *d_mt = d_cum + L;
+    u32 *d_lo = (u32 *)(d_mt + L);
+    std::vector<u64> h(7 * L + (L + 1) / 2, 0);
+    std::copy(db.begin(), db.end(), h.begin());
+    std::copy(nw.begin(), nw.end(), h.begin() + L);
+    std::copy(W.begin(), W.end(), h.begin() + 2 * L);
+    std::copy(wpr.begin(), wpr.end(), h.begin() + 3 * L);
+    std::copy(sb.begin(), sb.end(), h.begin() + 4 * L);
+    std::copy(cum.begin(), cum.end(), h.begin() + 5 * L);
+    std::copy(mt.begin(), mt.end(), h.begin() + 6 * L);
+    std::memcpy(h.data() + 7 * L, lo.data(), 4 * nm);
+    HIPCHK(hipMemcpyAsync(tab.p, h.data(), 8 * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    DBuf<u32> sc(ctx, t * BE_TILE);
+    if (transposed) {
+        if (q)
+            DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * q * 32, bm_transpose_small_multi_k, grid_for(q * 32, 256), 256, 0,
+                       words, sc.p, (const u64 *)d_cum, (const u64 *)d_W, (const u64 *)d_wpr, (const u64 *)d_sb,
+                       (const u64 *)d_db, nm);
+        for (u32 m = 0; m < nm; m++) {
+            if (wpr[m] <= BT_SMALL_WPR) continue;
+            const u64 nt = (wpr[m] + BT - 1) / BT;
+            DLG_LAUNCH(ctx, DLG_K_MISC, 8.0 * nw[m], bm_transpose_k, (unsigned)(nt * nt), 256, 0, words + sb[m],
+                       sc.p + db[m], W[m], wpr[m]);
+        }
+    } else {
+        for (u32 m = 0; m < nm; m++)
+            if (nw[m])
+                HIPCHK(hipMemcpyAsync(sc.p + db[m], words + sb[m], 4 * nw[m], hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * (db[nm] - 0), bm_zero_tails_k, std::min<u32>(nm, 4096u), 256, 0, sc.p,
+               (const u64 *)d_db, (const u64 *)d_nw, nm);
+    DBuf<u64> bsum(ctx, t), tot(ctx, 1);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * t * BE_TILE, bm_count_k, (unsigned)t, BE_T, 0, (const u32 *)sc.p, bsum.p);
+    scan_blocks(ctx, bsum.p, t, tot.p);
+    u64 total = 0;
+    ctx->read_scalars(tot.p, 1, &total);
+    if (total != expect) fail(DLG_DEVICE_ERROR, "bitmap emission count mismatch");
+    if (total)
+        DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * t * BE_TILE + 8.0 * total, (bm_emit_k<false, false, true>), (unsigned)t,
+                   BE_T, 0, sc.p, (const u64 *)bsum.p, 0u, (u64)1, out, (const u64 *)d_mt, (const u32 *)d_lo,
+                   (const u64 *)d_wpr, nm);
+    return total;
 }
 
 }  // namespace dlg

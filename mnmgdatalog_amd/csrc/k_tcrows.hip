@@ -962,6 +962,12 @@ void rowmap_init(dlg_ctx *ctx, RowMap &m, u32 lo, u32 hi, const std::vector<u32>
 
 u64 rowmap_emit(dlg_ctx *ctx, RowMap &m, DBuf<u64> &out, u64 expect) {
     out.ensure(expect ? expect : 1);
+    if (m.blocks.size() > 1) {   // many blocks (a batch's instances): one emission for all
+        std::vector<MapRef> maps;
+        maps.reserve(m.blocks.size());
+        for (const auto &b : m.blocks) maps.push_back({b.lo, b.W, b.wpr, b.base});
+        return bitmap_emit_maps(ctx, m.words.p, maps, true, out.p, expect);
+    }
     DBuf<u32> scratch(ctx);
     u64 pos = 0;
     for (const auto &b : m.blocks)
