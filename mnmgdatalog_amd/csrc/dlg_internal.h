@@ -572,12 +572,11 @@ void merge(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out);
 u64 merge_unique(dlg_ctx *ctx, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out);
 void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out);
 void unpack_rows(dlg_ctx *ctx, const u64 *t, u64 n, int orient, int32_t *rows);
-// E into out[0, n) and E⁻¹ into out_rev[0, n) (default: out[n, 2n)), row order kept.
-void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out, u64 *out_rev = nullptr);
-// The same into out (E) and out_rev (E reversed), also returning the count of
-// adjacent input rows out of order and, in cb, the flipped bounds of both
-// columns (min a, max a, min b, max b) -- one read (synchronises).
-u64 pack_sym_rows_scan(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out, u64 *out_rev, u32 *cb);  // 2n tuples: kv, vk
+// E into out[0, n) and E⁻¹ into out_rev[0, n), row order kept, also returning
+// the count of adjacent input rows out of order and, in cb, the flipped
+// bounds of both columns (min a, max a, min b, max b) -- one read
+// (synchronises).
+u64 pack_sym_rows_scan(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out, u64 *out_rev, u32 *cb);
 void fill_u64(dlg_ctx *ctx, u64 *p, u64 n, u64 v);
 void fill_u32(dlg_ctx *ctx, u32 *p, u64 n, u32 v);
 void copy_u64(dlg_ctx *ctx, u64 *dst, const u64 *src, u64 n);

@@ -61,19 +61,10 @@ __global__ void pack_k(const int2 *rows, u64 n, int orient, u64 *out) {
         out[i] = orient ? tmake(b, a) : tmake(a, b);
     }
 }
-__global__ void pack_sym_k(const int2 *rows, u64 n, u64 *out, u64 *out_rev) {
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    u64 stride = (u64)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) {
-        int2 r = rows[i];
-        u32 a = (u32)r.x ^ 0x80000000u, b = (u32)r.y ^ 0x80000000u;
-        out[i] = tmake(a, b);
-        out_rev[i] = tmake(b, a);
-    }
-}
-// pack_sym_k plus, in the same read, the bounds of both columns (as flipped
-// ids) and the count of adjacent rows out of signed lexicographic order;
-// one set of atomics per workgroup.
+// Each (a, b) row packed as (a, b) into out and as (b, a) into out_rev and,
+// in the same read, the bounds of both columns (as flipped ids) and the
+// count of adjacent rows out of signed lexicographic order; one set of
+// atomics per workgroup.
 __global__ __launch_bounds__(256) void pack_sym_scan_k(const int2 *rows, u64 n, u64 *out, u64 *out_rev, u32 *mm,
                                                        unsigned long long *desc) {
     u32 mna = 0xffffffffu, mxa = 0u, mnb = 0xffffffffu, mxb = 0u;
@@ -131,11 +122,6 @@ void pack_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, int orient, u64 *out) {
     if (!n) return;
     DLG_LAUNCH(ctx, DLG_K_PACK, 16.0 * n, pack_k, sgrid(ctx, n), ctx->block_size, 0,
                (const int2 *)rows, n, orient, out);
-}
-void pack_sym_rows(dlg_ctx *ctx, const int32_t *rows, u64 n, u64 *out, u64 *out_rev) {
-    if (!n) return;
-    DLG_LAUNCH(ctx, DLG_K_PACK, 24.0 * n, pack_sym_k, sgrid(ctx, n), ctx->block_size, 0,
-               (const int2 *)rows, n, out, out_rev ? out_rev : out + n);
 }
 __global__ void pack_scan_init_k(u64 *w) {   // bounds words (min a, max a, min b, max b), descents
     if (threadIdx.x == 0) {
