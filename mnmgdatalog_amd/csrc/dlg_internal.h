@@ -589,7 +589,8 @@ struct Index {
     DBuf<u64> dense;     // optional: first row of each key of [dense_lo, dense_lo + dense_w), plus n
     u64 n = 0, groups = 0, nslots = 0, dense_w = 0;
     u32 dense_lo = 0;
-    bool sorted = false; // probes binary-search `rows` instead of the slots (no slots built)
+    bool sorted = false; // probes binary-search `rows` instead of the slots (no slots built); a WCC
+                         // graph's index may also have neither (value lookups by rank directory)
     explicit Index(dlg_ctx *c) : rows(c), gstart(c), slots(c), dense(c) {}
 };
 // Build from a device relation (copied); `presorted` skips the sort;
@@ -599,6 +600,11 @@ void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double load_fac
 // Dense row offsets by key over [lo, lo + W) (index_build drops them); the
 // hash-index count phase then reads an offset pair per probe instead of
 // probing the slots.
+// The hash slots of an index built with sorted_lookup (index_build's last
+// step), for key lookups by hash probe.
+void index_hash(dlg_ctx *ctx, Index &ix, double lf);
+// Whether index_lookup_values takes the rank directory (no slots needed).
+bool index_rank_lookup_applies(dlg_ctx *ctx, const Index &ix);
 void index_dense(dlg_ctx *ctx, Index &ix, u32 lo, u64 W);
 // Count phase: per probe, the matching group's row start and count.  Returns |J|.
 struct JoinScratch {

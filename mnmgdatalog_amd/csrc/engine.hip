@@ -616,7 +616,13 @@ void wcc_graph_build(dlg_ctx *ctx, const int32_t *d_rows, u64 n, const Opts &o, 
     tmp.release();
     pc.mark(DLG_PH_HASHTABLE);
     g.ix.rows.swap(edge);   // the index keeps the sorted edges in place
-    index_build(ctx, g.ix, g.ix.rows.p, g.ne, o.lf, true, o.join == 1);
+    index_build(ctx, g.ix, g.ix.rows.p, g.ne, o.lf, true, true);
+    // the rounds read the groups' runs; the neighbour lookup takes the rank
+    // directory when it applies, else hash probes (slots built only then)
+    if (o.join != 1) {
+        if (index_rank_lookup_applies(ctx, g.ix)) g.ix.sorted = false;
+        else index_hash(ctx, g.ix, o.lf);
+    }
     g.V = g.ix.groups;
     g.nbr.ensure(g.ne ? g.ne : 1);
     index_lookup_values(ctx, g.ix, g.nbr.p);               // neighbour dense ids

@@ -82,6 +82,15 @@ void index_build(dlg_ctx *ctx, Index &ix, const u64 *rel, u64 n, double lf, bool
     DLG_LAUNCH(ctx, DLG_K_MISC, 4.0, set_last_k, 1, 64, 0, ix.gstart.p, G, (u32)n);
     ix.groups = G;
     if (sorted_lookup) return;
+    index_hash(ctx, ix, lf);
+}
+
+void index_hash(dlg_ctx *ctx, Index &ix, double lf) {
+    const u64 G = ix.groups;
+    if (G == 0) return;
+    if (!(lf > 0.0)) lf = ctx->load_factor;
+    lf = std::min(0.95, std::max(0.05, lf));
+    ix.sorted = false;
     u64 want = (u64)std::ceil((double)G / lf);
     u64 S = 64;
     while (S < want || S <= G) S <<= 1;
@@ -400,6 +409,18 @@ __global__ void lookup_values_rank_k(const u64 *rows, u64 n, const u64 *dir, u32
 // the directory then takes at most 2 bytes per key over the hash table's slots.
 constexpr u64 RANK_IDS_PER_KEY = 64, RANK_SLACK = 1ull << 20;
 
+bool index_rank_lookup_applies(dlg_ctx *ctx, const Index &ix) {
+    if (ix.n == 0) return true;
+    u64 ends[2];
+    HIPCHK(hipMemcpyAsync(ctx->h_scalars, ix.rows.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync((char *)ctx->h_scalars + 8, ix.rows.p + (ix.n - 1), 8, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    ctx->sync();
+    std::memcpy(ends, ctx->h_scalars, 16);
+    const u64 W = (u64)(u32)(ends[1] >> 32) - (u32)(ends[0] >> 32) + 1;   // rows are sorted
+    return W <= RANK_IDS_PER_KEY * ix.groups + RANK_SLACK;
+}
+
 void index_lookup_values(dlg_ctx *ctx, const Index &ix, u32 *out) {
     if (ix.n == 0) return;
     if (ix.sorted) {
@@ -432,6 +453,7 @@ void index_lookup_values(dlg_ctx *ctx, const Index &ix, u32 *out) {
                    (const u64 *)ix.rows.p, ix.n, (const u64 *)dir.p, lo, W, out);
         return;
     }
+    if (ix.nslots == 0) fail(DLG_DEVICE_ERROR, "index: value lookup by hash probe without slots");
     DLG_LAUNCH(ctx, DLG_K_MISC, 20.0 * ix.n, lookup_values_k, sgrid(ctx, ix.n), ctx->block_size, 0,
                (const u64 *)ix.rows.p,
                ix.n, (const u64 *)ix.slots.p, ix.nslots - 1, out);
