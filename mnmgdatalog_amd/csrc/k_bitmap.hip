@@ -1388,7 +1388,9 @@ u64 bitmap_emit_maps(dlg_ctx *ctx, const u32 *words, const std::vector<MapRef> &
             if (nw[m])
                 HIPCHK(hipMemcpyAsync(sc.p + db[m], words + sb[m], 4 * nw[m], hipMemcpyDeviceToDevice, ctx->stream));
     }
-    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * (db[nm] - 0), bm_zero_tails_k, std::min<u32>(nm, 4096u), 256, 0, sc.p,
+    u64 tails = db[nm];   // the padding words past each map
+    for (u32 m = 0; m < nm; m++) tails -= nw[m];
+    DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * tails, bm_zero_tails_k, std::min<u32>(nm, 4096u), 256, 0, sc.p,
                (const u64 *)d_db, (const u64 *)d_nw, nm);
     DBuf<u64> bsum(ctx, t), tot(ctx, 1);
     DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * t * BE_TILE, bm_count_k, (unsigned)t, BE_T, 0, (const u32 *)sc.p, bsum.p);
