@@ -35,7 +35,8 @@ typedef enum dlg_status {
     DLG_OK = 0,
     DLG_INVALID_INPUT = 1,   /* bad argument, malformed file, reserved id            */
     DLG_OUT_OF_MEMORY = 2,   /* device allocation failed or HBM budget exceeded      */
-    DLG_OVERFLOW = 3,        /* a count exceeds what the engine represents (2^32-1 rows per buffer) */
+    DLG_OVERFLOW = 3,        /* a count exceeds what the engine represents (2^32-2 rows per hash
+                                index, 2^32-1 tuples per sort with 4..7-bit digits) */
     DLG_DEVICE_ERROR = 4,    /* HIP runtime error                                    */
     DLG_IO_ERROR = 5,        /* file open/read/write failure                         */
     DLG_REFUSED = 6,         /* serving: admission refused (predicted need > budget) */

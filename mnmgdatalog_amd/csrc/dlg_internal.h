@@ -418,7 +418,8 @@ u64 self_join_raw(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo
 // in order into out (resized), *nout of them, and the raw join's size into
 // *raw.  Rows with more than SJW_CAP candidates are materialised, sorted and
 // deduplicated together.  Returns false, with nothing written, when those
-// exceed one sorted buffer (2^32 - 1 tuples).  Synchronises.
+// exceed 2^32 - 1 tuples: the caller's sorted join takes such inputs.
+// Synchronises.
 constexpr u32 SJW_CAP = 2048;
 bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 lo, u64 W, DBuf<u64> &out,
                     u64 *nout, u64 *raw);

@@ -1301,7 +1301,7 @@ bool self_join_rows(dlg_ctx *ctx, const u64 *rows, u64 n, const u64 *dense, u32 
         scan_u64_async(ctx, hoff.p, W, hoff.p, hoff.p + W);
         u64 nj = 0;
         ctx->read_scalars(hoff.p + W, 1, &nj);
-        if (nj > 0xffffffffull) return false;   // beyond one sorted buffer: the caller's sorted join
+        if (nj > 0xffffffffull) return false;   // so many heavy candidates: the caller's sorted join
         DBuf<u64> J(ctx, nj), tmp(ctx), H(ctx, nj);
         DLG_LAUNCH(ctx, DLG_K_JOIN_EXPAND, 8.0 * nj + 16.0 * W, sj_heavy_gather_k, g, 256, 0, (const u32 *)col.p,
                    dense, (const u64 *)hoff.p, W, lo, J.p);

@@ -545,7 +545,7 @@ constexpr u64 OS_VALUE = (1ull << 56) - 1;
 constexpr int OH_T = 256, OH_I = 8, OH_TILE = OH_T * OH_I;
 
 __global__ __launch_bounds__(OH_T) void os_hist_k(const u64 *in, u64 n, RadixKey key, u32 passes, u32 top_bits,
-                                                  u32 *hist) {
+                                                  unsigned long long *hist) {
     constexpr int BINS = 256, W = OH_T / WAVE;
     __shared__ u32 h[W][8][BINS];
     for (int i = threadIdx.x; i < W * 8 * BINS; i += OH_T) (&h[0][0][0])[i] = 0;
@@ -592,19 +592,21 @@ __global__ __launch_bounds__(OH_T) void os_hist_k(const u64 *in, u64 n, RadixKey
     }
     __syncthreads();
     for (u32 i = threadIdx.x; i < passes * BINS; i += OH_T) {
-        u32 v = 0;
+        u32 v = 0;   // a workgroup's share of one digit fits 32 bits
 #pragma unroll
         for (int w = 0; w < W; w++) v += (&h[w][0][0])[i];
-        if (v) atomicAdd(&hist[i], v);
+        if (v) atomicAdd(&hist[i], (unsigned long long)v);
     }
 }
 
-// Exclusive scan of each pass's digit counts into its digit starts.
-__global__ __launch_bounds__(256) void os_starts_k(const u32 *hist, u32 passes, u32 *start) {
-    __shared__ u32 sc[256 / WAVE + 1];
+// Exclusive scan of each pass's digit counts into its digit starts (64-bit:
+// one buffer may hold more than 2^32 keys).
+__global__ __launch_bounds__(256) void os_starts_k(const u64 *hist, u32 passes, u64 *start) {
+    __shared__ u64 sc[256 / WAVE + 1];
     for (u32 p = 0; p < passes; p++) {
-        const u32 v = hist[p * 256 + threadIdx.x];
-        start[p * 256 + threadIdx.x] = block_exclusive_scan<u32>(v, sc, nullptr);
+        const u64 v = hist[p * 256 + threadIdx.x];
+        start[p * 256 + threadIdx.x] = block_exclusive_scan<u64>(v, sc, nullptr);
+        __syncthreads();   // sc is reused by the next pass
     }
 }
 
@@ -633,13 +635,13 @@ __device__ __forceinline__ bool os_load_tile(const u64 *in, u64 n, u64 tile0, u6
 }
 
 template <int T, int I>
-__global__ __launch_bounds__(T) void os_pass_k(const u64 *in, u64 *out, u64 n, RadixKey key, const u32 *dstart,
+__global__ __launch_bounds__(T) void os_pass_k(const u64 *in, u64 *out, u64 n, RadixKey key, const u64 *dstart,
                                                u64 *status, u32 *ticket, u32 pass, u32 *err) {
     constexpr int BINS = 256, W = T / WAVE, TILE = T * I;
     __shared__ u64 stage[TILE];
     __shared__ u32 wcnt[W][BINS];
     __shared__ u32 bstart[BINS];
-    __shared__ u32 gofs[BINS];
+    __shared__ u64 gofs[BINS];   // output position of stage slot i of digit d: gofs[d] + i (mod 2^64)
     __shared__ u32 sc[T / WAVE + 1];
     __shared__ u32 s_tile;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
@@ -715,7 +717,7 @@ __global__ __launch_bounds__(T) void os_pass_k(const u64 *in, u64 *out, u64 n, R
             }
             os_store(mine, pre | (excl + tot));
         }
-        gofs[d] = dstart[d] + (u32)excl - ex;   // output position of stage slot i of digit d: gofs[d] + i
+        gofs[d] = dstart[d] + excl - (u64)ex;
     }
     __syncthreads();
 #pragma unroll
@@ -730,7 +732,7 @@ __global__ __launch_bounds__(T) void os_pass_k(const u64 *in, u64 *out, u64 n, R
     const u32 tn = rem < (u64)TILE ? (u32)rem : (u32)TILE;
     for (u32 i = threadIdx.x; i < tn; i += T) {
         const u64 x = stage[i];
-        out[(u64)(gofs[radix_digit8(x, key)] + i)] = x;   // < n < 2^32
+        out[gofs[radix_digit8(x, key)] + (u64)i] = x;
     }
 }
 
@@ -740,24 +742,27 @@ static void radix_onesweep(dlg_ctx *ctx, u64 *&src, u64 *&dst, u64 n, RadixKey k
     constexpr int OT = OS_T, OI = OS_I, OTILE = OS_TILE;
     const u32 passes = (u32)((total_bits + 7) / 8);
     const u64 tiles = (n + OTILE - 1) / OTILE;
-    if (tiles > 0xffffffffull) fail(DLG_OVERFLOW, "radix sort: too many tiles");
-    DBuf<u32> hist(ctx, 8 * 256 * 2 + 16);   // histograms, digit starts, tickets, error flag
+    if (tiles > 0x7fffffffull) fail(DLG_OVERFLOW, "radix sort: too many tiles");
+    DBuf<u64> hist(ctx, 8 * 256 * 2);   // histograms, digit starts (64-bit)
+    DBuf<u32> tick(ctx, 16);             // tickets, error flag
     DBuf<u64> status(ctx, tiles * 256);
-    u32 *starts = hist.p + 8 * 256, *tickets = hist.p + 2 * 8 * 256, *err = tickets + 8;
-    fill_u32(ctx, hist.p, 8 * 256 * 2 + 16, 0u);
+    u64 *starts = hist.p + 8 * 256;
+    u32 *tickets = tick.p, *err = tickets + 8;
+    fill_u64(ctx, hist.p, 8 * 256 * 2, 0ull);
+    fill_u32(ctx, tick.p, 16, 0u);
     fill_u64(ctx, status.p, tiles * 256, 0ull);
     key.shift = 0;
     const u32 top_bits = (u32)(total_bits - 8 * (int)(passes - 1));
     // 4 blocks per CU (32 KiB of LDS each), each looping over tiles
     const unsigned hg = (unsigned)std::min<u64>((n + OH_TILE - 1) / OH_TILE, (u64)ctx->num_cus * 4);
     DLG_LAUNCH(ctx, DLG_K_SORT_UPSWEEP, 8.0 * n, os_hist_k, hg, OH_T, 0, (const u64 *)src, n, key, passes, top_bits,
-               hist.p);
-    DLG_LAUNCH(ctx, DLG_K_SORT_UPSWEEP, 8.0 * 256 * passes, os_starts_k, 1, 256, 0, (const u32 *)hist.p, passes,
+               (unsigned long long *)hist.p);
+    DLG_LAUNCH(ctx, DLG_K_SORT_UPSWEEP, 8.0 * 256 * passes, os_starts_k, 1, 256, 0, (const u64 *)hist.p, passes,
                starts);
     for (u32 p = 0; p < passes; p++) {
         key.shift = 8 * p;
         DLG_LAUNCH(ctx, DLG_K_SORT_DOWNSWEEP, 16.0 * n, (os_pass_k<OT, OI>), (unsigned)tiles, OT, 0,
-                   (const u64 *)src, dst, n, key, (const u32 *)(starts + 256 * p), status.p, tickets + p, p, err);
+                   (const u64 *)src, dst, n, key, (const u64 *)(starts + 256 * p), status.p, tickets + p, p, err);
         std::swap(src, dst);
     }
     u32 e = 0;
@@ -841,10 +846,13 @@ void rows_digest(dlg_ctx *ctx, const u64 *d, u64 n, int orient, u64 *sum, u64 *b
 void radix_sort(dlg_ctx *ctx, DBuf<u64> &data, u64 n, DBuf<u64> &tmp, int bits, const u32 *bounds,
                 bool key_only) {
     if (n < 2) return;
-    if (n > 0xffffffffull) fail(DLG_OVERFLOW, "radix_sort: more than 2^32-1 tuples in one buffer");
     if (bits <= 0) bits = ctx->radix_bits;
     if (bits < 4) bits = 4;
     if (bits > 8) bits = 8;
+    // The 8-bit onesweep counts digits in 64 bits; the narrower digit passes
+    // keep 32-bit digit offsets.
+    if (n > 0xffffffffull && bits != 8)
+        fail(DLG_OVERFLOW, "radix_sort: more than 2^32-1 tuples need 8-bit digits");
     u32 h[4];
     if (bounds) std::memcpy(h, bounds, sizeof(h));
     else tuple_bounds(ctx, data.p, n, h);

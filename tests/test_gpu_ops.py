@@ -3,6 +3,8 @@
 Covers the roles of the reference's Thrust calls and join:
 sort / sort+unique / set_difference / merge / exclusive_scan, and
 get_hash_table + get_join (multiset of join rows, both projections)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -59,6 +61,39 @@ def test_sort_onesweep_sizes(ctx, n, kb, vb):
         assert np.array_equal(ctx.sort(s), s)
         assert np.array_equal(ctx.sort(s[::-1].copy()), s)
         assert np.array_equal(ctx.sort_unique(t), np.unique(t))
+
+
+def test_sort_more_than_2_32_keys(ctx):
+    """One buffer of 2^32 + 2^20 distinct keys (the one-sweep passes count digits in 64
+    bits): 256 shuffled copies of a block b_i = (i * A mod 2^56) << 8 (A odd, so the b_i
+    are distinct), copy j with its low byte set to j.  The sorted buffer is sorted(b)
+    with each entry repeated as b | 0 .. b | 255, checked chunk by chunk on the host.
+    Uses about 70 GB of device memory."""
+    if ctx.device_info()["hbm_bytes"] < (120 << 30):
+        pytest.skip("needs about 70 GB of device memory")
+    m, copies = (1 << 24) + (1 << 12), 256
+    n = m * copies
+    assert n > 0xffffffff
+    A = np.uint64(0x9E3779B97F4A7C15)
+    b = ((np.arange(m, dtype=np.uint64) * A) & np.uint64((1 << 56) - 1)) << np.uint64(8)
+    order = np.random.default_rng(11).permutation(copies)
+    p = ctx.alloc(n * 8)
+    try:
+        for slot, j in enumerate(order):   # the copies in shuffled order
+            blk = b | np.uint64(j)
+            ctx.check(ctx.L.dlg_copy_h2d(ctx.h, ctypes.c_void_p(p + slot * m * 8), blk.ctypes.data, blk.nbytes),
+                      "h2d")
+        del blk
+        ctx.check(ctx.L.dlg_sort(ctx.h, ctypes.c_void_p(p), n), "sort")
+        sb = np.sort(b)
+        low = np.arange(copies, dtype=np.uint64)[None, :]
+        rows = 1 << 20   # sorted entries per checked chunk (2^28 keys, 2 GiB)
+        for k0 in range(0, m, rows):
+            k1 = min(m, k0 + rows)
+            got = ctx.download(p + k0 * copies * 8, (k1 - k0) * copies, np.uint64).reshape(-1, copies)
+            assert np.array_equal(got, sb[k0:k1, None] | low), f"sorted entries {k0}..{k1}"
+    finally:
+        ctx.free(p)
 
 
 @pytest.mark.parametrize("n,kb,vb", SORT_CASES)
@@ -184,7 +219,6 @@ def test_hash_join_extreme_ids(ctx):
 
 
 def test_invalid_arguments_report_status(ctx):
-    import ctypes
     k = ctypes.c_int64()
     s = ctx.L.dlg_sort_unique(ctx.h, None, 5, ctypes.byref(k))
     assert s == dlg.DLG_INVALID_INPUT
