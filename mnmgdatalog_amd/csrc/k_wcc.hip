@@ -14,7 +14,7 @@
 //             a node whose label dropped is appended once per round
 //             (mark[m] == round), giving Δ' = {(m, L'(m)) : L'(m) < L(m)}
 // Edge traversal is load balanced: degree counts -> scan -> one thread per
-// edge with a binary search in the frontier offsets.
+// edge, its node found from the tile's node starts set as bits in LDS.
 //
 // Two other forms of the same round (same labels, same Δ'):
 //   round one  every node is in Δ with its own id as label, so L'(m) is the
@@ -90,7 +90,46 @@ __device__ __forceinline__ void block_append(const bool (&app)[I], u32 *count, u
     __syncthreads();   // s_cnt may be reused
 }
 
+// The owners of a tile's TS_TILE positions (edges, or join rows) base ..
+// base + TS_TILE - 1 under offsets that rise strictly (every node has an
+// edge): offs[lo] <= base and nodes lo+1 .. hi-1 start inside the tile.  Their
+// starts are set as bits in LDS; the owner of tile position p is lo plus the
+// starts at or before p -- what a binary search of offs gives, without global
+// loads per position.
+constexpr int TS_TILE = 2048, TS_WORDS = TS_TILE / 32;
+static_assert(TS_WORDS == WAVE, "one wave scans the start words");
+struct TileStarts {
+    u32 bits[TS_WORDS];     // bit p: a node starts at base + p
+    u32 before[TS_WORDS];   // starts in the words before
+};
+
+// Before the caller's barrier that publishes lo and hi.
+__device__ __forceinline__ void tile_starts_clear(TileStarts &ts) {
+    if (threadIdx.x < TS_WORDS) ts.bits[threadIdx.x] = 0u;
+}
+
+// Every thread of the block calls it (two barriers).
+template <int T, typename O>
+__device__ __forceinline__ void tile_starts_set(TileStarts &ts, const O *offs, u64 lo, u64 hi, u64 base) {
+    for (u64 j = lo + 1 + threadIdx.x; j < hi; j += T) {   // base < offs[j] < base + TS_TILE
+        const u32 p = (u32)((u64)offs[j] - base);
+        atomicOr(&ts.bits[p >> 5], 1u << (p & 31u));
+    }
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+        u32 tot;
+        ts.before[threadIdx.x] = wave_excl_scan32((u32)__popc(ts.bits[threadIdx.x]), &tot);
+    }
+    __syncthreads();
+}
+
+// Owner of tile position p (< TS_TILE), relative to lo.
+__device__ __forceinline__ u32 tile_owner(const TileStarts &ts, u32 p) {
+    return ts.before[p >> 5] + (u32)__popc(ts.bits[p >> 5] & (0xffffffffu >> (31u - (p & 31u))));
+}
+
 constexpr int WR_T = 256, WR_I = 8, WR_TILE = WR_T * WR_I;
+static_assert(WR_TILE == TS_TILE, "relax tiles");
 
 __global__ __launch_bounds__(WR_T) void wcc_relax_k(const u32 *flabel, u64 nf, const u32 *pstart,
                                                     const u64 *offs, u64 total, const u32 *nbr,
@@ -98,7 +137,9 @@ __global__ __launch_bounds__(WR_T) void wcc_relax_k(const u32 *flabel, u64 nf, c
                                                     u32 *next_count) {
     __shared__ u64 s_lo, s_hi;
     __shared__ u32 s_cnt[WR_I * WR_T / WAVE + 1];
+    __shared__ TileStarts ts;
     const u64 base = (u64)blockIdx.x * WR_TILE;
+    tile_starts_clear(ts);
     if (threadIdx.x < WAVE) {   // the first wave finds the tile's frontier nodes
         const u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
         const u64 lo = wave_upper_bound(offs, 0, nf, base) - 1;
@@ -110,6 +151,7 @@ __global__ __launch_bounds__(WR_T) void wcc_relax_k(const u32 *flabel, u64 nf, c
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
+    tile_starts_set<WR_T>(ts, offs, lo, hi, base);   // frontier nodes have edges: offsets rise strictly
     bool app[WR_I];
     u32 m[WR_I], slot[WR_I];
 #pragma unroll
@@ -118,7 +160,7 @@ __global__ __launch_bounds__(WR_T) void wcc_relax_k(const u32 *flabel, u64 nf, c
         app[r] = false;
         m[r] = 0;
         if (o < total) {
-            u64 i = upper_bound_u64(offs, lo, hi, o) - 1;
+            const u64 i = lo + tile_owner(ts, (u32)(r * WR_T) + threadIdx.x);
             m[r] = nbr[(u64)pstart[i] + (o - offs[i])];
             u32 l = flabel[i];
             if (L[m[r]] > l) {
@@ -134,14 +176,6 @@ __global__ __launch_bounds__(WR_T) void wcc_relax_k(const u32 *flabel, u64 nf, c
 }
 
 constexpr int WA_T = 256, WA_I = 16, WA_TILE = WA_T * WA_I;   // node-parallel kernels: one tile per block
-
-__device__ __forceinline__ u64 upper_bound_u32(const u32 *a, u64 lo, u64 hi, u64 x) {
-    while (lo < hi) {
-        const u64 mid = (lo + hi) >> 1;
-        if ((u64)a[mid] <= x) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
 
 // Round one: L(m) = min(m, first neighbour of m); lowered nodes appended with
 // their label.  Every node has a neighbour (the nodes are the edge keys).
@@ -175,14 +209,18 @@ __global__ __launch_bounds__(WA_T) void wcc_first_k(const u32 *gstart, const u32
 // Pull round over all ne edges (CSR gstart/nbr): Ln(m) = min(Ln(m), L(v)) for
 // every edge (m, v), with Ln == L on entry.  Lanes of one node are adjacent
 // (edges in node order), so a segmented minimum over the wave leaves one
-// candidate per node and wave.
+// candidate per node and wave.  An edge's node comes from the tile's node
+// starts (TileStarts).
 constexpr int WP_T = 256, WP_I = 8, WP_TILE = WP_T * WP_I;
+static_assert(WP_TILE == TS_TILE, "pull tiles");
 
 __global__ __launch_bounds__(WP_T) void wcc_pull_k(const u32 *gstart, u64 V, u64 ne, const u32 *nbr,
                                                    const u32 *L, u32 *Ln) {
     __shared__ u64 s_lo, s_hi;
+    __shared__ TileStarts ts;
     const u64 base = (u64)blockIdx.x * WP_TILE;
     const int lane = threadIdx.x & (WAVE - 1);
+    tile_starts_clear(ts);
     if (threadIdx.x < WAVE) {   // the first wave finds the tile's nodes
         const u64 last = (base + WP_TILE < ne ? base + WP_TILE : ne) - 1;
         const u64 lo = wave_upper_bound(gstart, 0, V, base) - 1;
@@ -194,13 +232,21 @@ __global__ __launch_bounds__(WP_T) void wcc_pull_k(const u32 *gstart, u64 V, u64
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
+    tile_starts_set<WP_T>(ts, gstart, lo, hi, base);
+    u32 v[WP_I];
+#pragma unroll
+    for (int r = 0; r < WP_I; r++) {
+        const u64 o = base + (u64)r * WP_T + threadIdx.x;
+        v[r] = o < ne ? nbr[o] : 0u;
+    }
 #pragma unroll
     for (int r = 0; r < WP_I; r++) {
         const u64 o = base + (u64)r * WP_T + threadIdx.x;
         u32 m = 0xffffffffu, val = 0xffffffffu;
         if (o < ne) {
-            m = (u32)(upper_bound_u32(gstart, lo, hi, o) - 1);
-            val = L[nbr[o]];
+            const u32 p = (u32)(r * WP_T) + threadIdx.x;
+            m = (u32)lo + tile_owner(ts, p);
+            val = L[v[r]];
         }
 #pragma unroll
         for (int d = 1; d < WAVE; d <<= 1) {
@@ -553,10 +599,12 @@ __global__ __launch_bounds__(WR_T) void gw_relax_k(const u32 *flabel, const u32 
                                                    WccDev *st) {
     __shared__ u64 s_lo, s_hi;
     __shared__ u32 s_cnt[WR_I * WR_T / WAVE + 1];
+    __shared__ TileStarts ts;
     const u64 total = st->total, nf = st->nf;
     const u64 base = (u64)blockIdx.x * WR_TILE;
     if (st->done || base >= total) return;   // block-uniform
     const u32 round = st->it + 1;
+    tile_starts_clear(ts);
     if (threadIdx.x < WAVE) {   // the first wave finds the tile's frontier nodes
         const u64 last = (base + WR_TILE < total ? base + WR_TILE : total) - 1;
         const u64 lo = wave_upper_bound(offs, 0, nf, base) - 1;
@@ -568,6 +616,7 @@ __global__ __launch_bounds__(WR_T) void gw_relax_k(const u32 *flabel, const u32 
     }
     __syncthreads();
     const u64 lo = s_lo, hi = s_hi;
+    tile_starts_set<WR_T>(ts, offs, lo, hi, base);   // frontier nodes have edges: offsets rise strictly
     bool app[WR_I];
     u32 m[WR_I], slot[WR_I];
 #pragma unroll
@@ -576,7 +625,7 @@ __global__ __launch_bounds__(WR_T) void gw_relax_k(const u32 *flabel, const u32 
         app[r] = false;
         m[r] = 0;
         if (o < total) {
-            u64 i = upper_bound_u64(offs, lo, hi, o) - 1;
+            const u64 i = lo + tile_owner(ts, (u32)(r * WR_T) + threadIdx.x);
             m[r] = nbr[(u64)pstart[i] + (o - offs[i])];
             u32 l = flabel[i];
             if (L[m[r]] > l) {
