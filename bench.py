@@ -524,7 +524,7 @@ def main():
     ap.add_argument("--relax", type=int, default=0, help="WCC round form (0 auto, 1 push, 2 pull)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU per process: every GPU box here has one GPU, so "
+        print(json.dumps({"error": "this engine runs on one GPU per process and is measured on one GPU, so "
                                    "--gpus must be 1; sharding across GPUs (hash partition of the join key + "
                                    "all-to-all) is a design, not built"}))
         raise SystemExit(2)
