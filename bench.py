@@ -89,7 +89,7 @@ CPU_SAMPLE_ITERS = {"tc_grid": (16, 30), "sg_mesh": (2, 3), "tc_powerlaw": (0, 0
 
 def cpu_share():
     """Threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS when set
-    (the box's per-GPU share of the host)."""
+    (a node's per-GPU share of the host)."""
     n = len(os.sched_getaffinity(0))
     omp = os.environ.get("OMP_NUM_THREADS")
     if omp and omp.isdigit() and int(omp) > 0:
@@ -116,7 +116,7 @@ def host_info():
     except OSError:
         pass
     lscpu = {}
-    try:   # lscpu's fields of the box's host (a child process; the GPU is not touched)
+    try:   # lscpu's fields of this host (a child process; the GPU is not touched)
         import subprocess
         txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
         for line in txt.splitlines():
