@@ -518,7 +518,8 @@ u64 wcc_finalize(dlg_ctx *ctx, WccState &s, const Index &ix, DBuf<u64> &out, i64
 // the stop rule runs on the device, and KR rounds (frontier buffers in
 // ping-pong) are captured once into a hipGraph that the host replays,
 // reading back one flag per replay.  No memset nodes: counters are cleared
-// by kernels.
+// by kernels.  Four kernels a round: the frontier's degrees and label
+// snapshot, their scan, the relax, the stop rule.
 namespace dlg {
 
 struct WccDev {
@@ -533,12 +534,12 @@ struct WccDev {
 
 constexpr int GW_T = 256, GW_I = 8, GW_TILE = GW_T * GW_I;
 
-__global__ void gw_begin_k(WccDev *st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && !st->done) st->count = 0;
-}
-
-__global__ __launch_bounds__(GW_T) void gw_degree_scan_k(const u32 *fnode, const u32 *gstart, u32 *cnt,
-                                                         u32 *pstart, u64 *bsum, const WccDev *st) {
+// Degrees of the frontier's nodes and their block sums; each node's label is
+// read here too, after the previous round's relax: the snapshot its edges
+// carry this round (Jacobi).
+__global__ __launch_bounds__(GW_T) void gw_degree_scan_k(const u32 *fnode, const u32 *gstart, const u32 *L,
+                                                         u32 *flabel, u32 *cnt, u32 *pstart, u64 *bsum,
+                                                         const WccDev *st) {
     __shared__ u64 sc[GW_T / WAVE + 1];
     const u64 nf = st->done ? 0 : st->nf;
     const u64 base = (u64)blockIdx.x * GW_TILE + (u64)threadIdx.x * GW_I;
@@ -552,6 +553,7 @@ __global__ __launch_bounds__(GW_T) void gw_degree_scan_k(const u32 *fnode, const
             u32 c = gstart[v + 1] - a;
             cnt[j] = c;
             pstart[j] = a;
+            flabel[j] = L[v];
             s += c;
         }
     }
@@ -560,25 +562,21 @@ __global__ __launch_bounds__(GW_T) void gw_degree_scan_k(const u32 *fnode, const
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(1024) void gw_scan_blocks_k(u64 *bsum, u64 nb, WccDev *st) {
-    __shared__ u64 sc[1024 / WAVE + 1];
-    u64 carry = 0;
-    for (u64 base = 0; base < nb; base += 1024) {
-        u64 i = base + threadIdx.x;
-        u64 v = i < nb ? bsum[i] : 0;
-        u64 tot;
-        u64 ex = block_exclusive_scan<u64>(v, sc, &tot);
-        if (i < nb) bsum[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) st->total = st->done ? 0 : carry;
-}
-
-__global__ __launch_bounds__(GW_T) void gw_scan_apply_k(const u32 *cnt, const u64 *bpre, u64 *offs,
-                                                        const WccDev *st) {
+// Offsets of the frontier's edges: each workgroup adds the block sums before
+// its own (read and summed by the whole workgroup) to a scan of its degrees;
+// the last workgroup also writes the round's edge total.
+__global__ __launch_bounds__(GW_T) void gw_scan_apply_k(const u32 *cnt, const u64 *bsum, u64 *offs, WccDev *st) {
     __shared__ u64 sc[GW_T / WAVE + 1];
     const u64 nf = st->done ? 0 : st->nf;
+    const bool last = blockIdx.x == gridDim.x - 1;
     const u64 base = (u64)blockIdx.x * GW_TILE + (u64)threadIdx.x * GW_I;
+    if ((u64)blockIdx.x * GW_TILE >= nf && !last) return;   // block-uniform: no frontier nodes here
+    u64 p = 0;
+    for (u64 i = threadIdx.x; i < blockIdx.x; i += GW_T) p += bsum[i];
+    u64 pre;
+    block_exclusive_scan<u64>(p, sc, &pre);
+    __syncthreads();   // sc is reused
+    if (last && threadIdx.x == 0) st->total = pre + bsum[blockIdx.x];   // 0 when the rule has fired
     u32 v[GW_I];
     u64 s = 0;
 #pragma unroll
@@ -586,7 +584,7 @@ __global__ __launch_bounds__(GW_T) void gw_scan_apply_k(const u32 *cnt, const u6
         v[i] = base + i < nf ? cnt[base + i] : 0;
         s += v[i];
     }
-    u64 ex = block_exclusive_scan<u64>(s, sc, nullptr) + bpre[blockIdx.x];
+    u64 ex = block_exclusive_scan<u64>(s, sc, nullptr) + pre;
 #pragma unroll
     for (int i = 0; i < GW_I; i++) {
         if (base + i < nf) offs[base + i] = ex;
@@ -640,19 +638,13 @@ __global__ __launch_bounds__(WR_T) void gw_relax_k(const u32 *flabel, const u32 
         if (app[r]) next_node[slot[r]] = m[r];
 }
 
-__global__ void gw_gather_k(const u32 *node, const u32 *L, u32 *label, const WccDev *st) {
-    if (st->done) return;
-    const u64 n = st->count;
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    u64 stride = (u64)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) label[i] = L[node[i]];
-}
-
 // The reference's stop rule, on the device: count the round, stop when two
-// consecutive delta sizes are equal.
+// consecutive delta sizes are equal.  The append counter is cleared for the
+// next round.
 __global__ void gw_end_k(WccDev *st) {
     if (threadIdx.x != 0 || blockIdx.x != 0 || st->done) return;
     u64 cur = st->count;
+    st->count = 0;
     st->it += 1;
     st->nf = cur;
     if (cur == st->prev || (st->max_it && st->it >= st->max_it)) st->done = 1;
@@ -660,20 +652,16 @@ __global__ void gw_end_k(WccDev *st) {
 }
 
 static void gw_capture_round(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr, u64 V, u64 E,
-                             WccDev *st, u32 *fnode, u32 *flabel, u32 *nnode, u32 *nlabel, u64 *bsum) {
-    const u64 nbv = (V + GW_TILE - 1) / GW_TILE;
-    const unsigned gb = grid_for(V, GW_TILE);
-    hipLaunchKernelGGL(gw_begin_k, dim3(1), dim3(64), 0, ctx->stream, st);
+                             WccDev *st, u32 *fnode, u32 *flabel, u32 *nnode, u64 *bsum) {
+    const unsigned gb = grid_for(V, GW_TILE);   // bsum holds gb block sums
     hipLaunchKernelGGL(gw_degree_scan_k, dim3(gb), dim3(GW_T), 0, ctx->stream, (const u32 *)fnode,
-                       (const u32 *)ix.gstart.p, s.cnt.p, s.pstart.p, bsum, (const WccDev *)st);
-    hipLaunchKernelGGL(gw_scan_blocks_k, dim3(1), dim3(1024), 0, ctx->stream, bsum, nbv, st);
+                       (const u32 *)ix.gstart.p, (const u32 *)s.L.p, flabel, s.cnt.p, s.pstart.p, bsum,
+                       (const WccDev *)st);
     hipLaunchKernelGGL(gw_scan_apply_k, dim3(gb), dim3(GW_T), 0, ctx->stream, (const u32 *)s.cnt.p,
-                       (const u64 *)bsum, s.offs.p, (const WccDev *)st);
+                       (const u64 *)bsum, s.offs.p, st);
     hipLaunchKernelGGL(gw_relax_k, dim3(grid_for(E, WR_TILE)), dim3(WR_T), 0, ctx->stream,
                        (const u32 *)flabel, (const u32 *)s.pstart.p, (const u64 *)s.offs.p, nbr, s.L.p,
                        s.mark.p, nnode, st);
-    hipLaunchKernelGGL(gw_gather_k, dim3(sgrid(ctx, V)), dim3(256), 0, ctx->stream, (const u32 *)nnode,
-                       (const u32 *)s.L.p, nlabel, (const WccDev *)st);
     hipLaunchKernelGGL(gw_end_k, dim3(1), dim3(64), 0, ctx->stream, st);
 }
 
@@ -706,9 +694,9 @@ int wcc_rounds_graph(dlg_ctx *ctx, WccState &s, const Index &ix, const u32 *nbr,
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     for (int r = 0; r < kr; r++) {
         if (r % 2 == 0)
-            gw_capture_round(ctx, s, ix, nbr, V, ne, st, s.fnode.p, s.flabel.p, s.nnode.p, nlabel.p, bsum.p);
+            gw_capture_round(ctx, s, ix, nbr, V, ne, st, s.fnode.p, s.flabel.p, s.nnode.p, bsum.p);
         else
-            gw_capture_round(ctx, s, ix, nbr, V, ne, st, s.nnode.p, nlabel.p, s.fnode.p, s.flabel.p, bsum.p);
+            gw_capture_round(ctx, s, ix, nbr, V, ne, st, s.nnode.p, nlabel.p, s.fnode.p, bsum.p);
     }
     hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
     ctx->prof = prof;

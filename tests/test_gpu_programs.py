@@ -529,6 +529,15 @@ def test_wcc_relax_out_of_range(ctx):
     assert e.value.status == dlg.DLG_INVALID_INPUT
 
 
+def test_wcc_graph_captured_many_blocks(ctx, orc):
+    """Captured rounds on ~1.5M nodes: over 700 scan workgroups, each summing the block
+    sums before its own in several strides, against eager rounds and the oracle."""
+    rows = dlg.gen_erdos_renyi(1_500_000, 3_000_000, 41)
+    want = orc.wcc(rows)
+    for strategy in (2, 1):
+        assert_same(ctx.run("wcc", rows, strategy=strategy), want, secondary=True)
+
+
 def test_wcc_graph_captured_max_iterations(ctx):
     rows = read_bin("data_223001.bin")
     a = ctx.run("wcc", rows, strategy=1, max_iterations=37)
