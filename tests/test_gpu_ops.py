@@ -96,6 +96,27 @@ def test_sort_more_than_2_32_keys(ctx):
         ctx.free(p)
 
 
+def test_sort_unique_more_than_2_32_keys(ctx):
+    """Sort + unique over one buffer of 2^32 + 2^20 keys: 256 copies of a block of distinct
+    scattered keys (i * A mod 2^64, A odd), so the result is the sorted block.  Uses about
+    105 GB of device memory (the keys, the sort's second buffer, the unique output)."""
+    if ctx.device_info()["hbm_bytes"] < (160 << 30):
+        pytest.skip("needs about 105 GB of device memory")
+    m, copies = (1 << 24) + (1 << 12), 256
+    n = m * copies
+    b = np.arange(m, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    p = ctx.alloc(n * 8)
+    try:
+        for slot in range(copies):
+            ctx.check(ctx.L.dlg_copy_h2d(ctx.h, ctypes.c_void_p(p + slot * m * 8), b.ctypes.data, b.nbytes), "h2d")
+        k = ctypes.c_int64()
+        ctx.check(ctx.L.dlg_sort_unique(ctx.h, ctypes.c_void_p(p), n, ctypes.byref(k)), "sort_unique")
+        assert k.value == m
+        assert np.array_equal(ctx.download(p, m, np.uint64), np.sort(b))
+    finally:
+        ctx.free(p)
+
+
 @pytest.mark.parametrize("n,kb,vb", SORT_CASES)
 def test_sort_unique(ctx, n, kb, vb):
     rng = np.random.default_rng(7 * n + vb)
