@@ -343,21 +343,12 @@ __device__ __forceinline__ u32 radix_digit8(u64 x, const RadixKey &k) {
     return radix_digit8_hl(tkey(x) - k.min_hi, tval(x) - k.min_lo, k);
 }
 
-// min/max of the key and value halves: out[0]=min_hi out[1]=max_hi out[2]=min_lo out[3]=max_lo
-__global__ void minmax_init_k(u32 *out) {
-    if (threadIdx.x == 0) {
-        out[0] = 0xffffffffu; out[1] = 0u; out[2] = 0xffffffffu; out[3] = 0u;
-    }
-}
-__global__ __launch_bounds__(256) void minmax_k(const u64 *in, u64 n, u32 *out) {
-    u32 mnh = 0xffffffffu, mxh = 0, mnl = 0xffffffffu, mxl = 0;
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    u64 stride = (u64)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) {
-        u64 x = in[i];
-        u32 h = tkey(x), l = tval(x);
-        mnh = min(mnh, h); mxh = max(mxh, h); mnl = min(mnl, l); mxl = max(mxl, l);
-    }
+// min/max of the key and value halves: out[0]=min_hi out[1]=max_hi out[2]=min_lo out[3]=max_lo.
+// Each workgroup folds a grid-stride share through the wave and LDS and
+// stores its four partials (part[4 b ..]); minmax_final_k folds the
+// workgroups' partials.  Device atomics on the four shared words serialised:
+// ~100 µs for 2,048 workgroups.
+__device__ __forceinline__ void minmax_block(u32 &mnh, u32 &mxh, u32 &mnl, u32 &mxl) {
 #pragma unroll
     for (int d = WAVE / 2; d > 0; d >>= 1) {
         mnh = min(mnh, (u32)__shfl_xor((int)mnh, d, WAVE));
@@ -365,7 +356,6 @@ __global__ __launch_bounds__(256) void minmax_k(const u64 *in, u64 n, u32 *out) 
         mnl = min(mnl, (u32)__shfl_xor((int)mnl, d, WAVE));
         mxl = max(mxl, (u32)__shfl_xor((int)mxl, d, WAVE));
     }
-    // one set of atomics per block: the per-wave form contended on 4 addresses
     __shared__ u32 sm[4][256 / WAVE];
     const int wave = threadIdx.x / WAVE;
     if ((threadIdx.x & (WAVE - 1)) == 0) {
@@ -377,8 +367,35 @@ __global__ __launch_bounds__(256) void minmax_k(const u64 *in, u64 n, u32 *out) 
             mnh = min(mnh, sm[0][w]); mxh = max(mxh, sm[1][w]);
             mnl = min(mnl, sm[2][w]); mxl = max(mxl, sm[3][w]);
         }
-        atomicMin(&out[0], mnh); atomicMax(&out[1], mxh);
-        atomicMin(&out[2], mnl); atomicMax(&out[3], mxl);
+    }
+}
+
+__global__ __launch_bounds__(256) void minmax_k(const u64 *in, u64 n, u32 *part) {
+    u32 mnh = 0xffffffffu, mxh = 0, mnl = 0xffffffffu, mxl = 0;
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        u64 x = in[i];
+        u32 h = tkey(x), l = tval(x);
+        mnh = min(mnh, h); mxh = max(mxh, h); mnl = min(mnl, l); mxl = max(mxl, l);
+    }
+    minmax_block(mnh, mxh, mnl, mxl);
+    if (threadIdx.x == 0) {
+        u32 *p = part + 4 * (u64)blockIdx.x;
+        p[0] = mnh; p[1] = mxh; p[2] = mnl; p[3] = mxl;
+    }
+}
+
+// One workgroup: the fold of nb workgroups' partials (none: the empty bounds).
+__global__ __launch_bounds__(256) void minmax_final_k(const u32 *part, u32 nb, u32 *out) {
+    u32 mnh = 0xffffffffu, mxh = 0, mnl = 0xffffffffu, mxl = 0;
+    for (u32 b = threadIdx.x; b < nb; b += blockDim.x) {
+        const u32 *p = part + 4 * (u64)b;
+        mnh = min(mnh, p[0]); mxh = max(mxh, p[1]); mnl = min(mnl, p[2]); mxl = max(mxl, p[3]);
+    }
+    minmax_block(mnh, mxh, mnl, mxl);
+    if (threadIdx.x == 0) {
+        out[0] = mnh; out[1] = mxh; out[2] = mnl; out[3] = mxl;
     }
 }
 
@@ -783,12 +800,10 @@ static void radix_passes_sized(dlg_ctx *ctx, u64 *&src, u64 *&dst, u64 n, RadixK
 }
 
 void tuple_bounds(dlg_ctx *ctx, const u64 *data, u64 n, u32 *h) {
-    DBuf<u32> mm(ctx, 4);
-    DLG_LAUNCH(ctx, DLG_K_MISC, 0.0, minmax_init_k, 1, 64, 0, mm.p);
-    if (n) {
-        unsigned g = (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 8);
-        DLG_LAUNCH(ctx, DLG_K_SORT_HIST, 8.0 * n, minmax_k, g, 256, 0, data, n, mm.p);
-    }
+    const unsigned g = n ? (unsigned)std::min<u64>((n + 255) / 256, (u64)ctx->num_cus * 8) : 0u;
+    DBuf<u32> mm(ctx, 4 + 4 * (u64)g);   // the bounds, then the workgroups' partials
+    if (g) DLG_LAUNCH(ctx, DLG_K_SORT_HIST, 8.0 * n, minmax_k, g, 256, 0, data, n, mm.p + 4);
+    DLG_LAUNCH(ctx, DLG_K_MISC, 16.0 * g, minmax_final_k, 1, 256, 0, (const u32 *)(mm.p + 4), g, mm.p);
     HIPCHK(hipMemcpyAsync(ctx->h_scalars, mm.p, 16, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     std::memcpy(h, ctx->h_scalars, 16);
