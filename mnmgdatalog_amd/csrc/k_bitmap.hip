@@ -959,8 +959,10 @@ __global__ __launch_bounds__(256) void sj_row_raw_k(const u32 *col, const u64 *d
             m = s_m[0][w] > m ? s_m[0][w] : m;
             mb = s_m[1][w] > mb ? s_m[1][w] : mb;
         }
-        if (m) atomicMax(mx, (unsigned long long)m);
-        if (mb) atomicMax(mx + 1, (unsigned long long)mb);
+        // an atomic only where it can still raise the maximum (a stale read only costs an atomic)
+        if (m > __hip_atomic_load(mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(mx, (unsigned long long)m);
+        if (mb > __hip_atomic_load(mx + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(mx + 1, (unsigned long long)mb);
     }
 }
 

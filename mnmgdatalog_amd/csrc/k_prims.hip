@@ -103,8 +103,12 @@ __global__ __launch_bounds__(256) void pack_sym_scan_k(const int2 *rows, u64 n, 
             mnb = min(mnb, sm[2][w]); mxb = max(mxb, sm[3][w]);
             dc += sd[w];
         }
-        atomicMin(&mm[0], mna); atomicMax(&mm[1], mxa);
-        atomicMin(&mm[2], mnb); atomicMax(&mm[3], mxb);
+        // an atomic only where it can still move the bound: after the first
+        // workgroups most skip it (a stale read only costs an atomic)
+        if (mna < __hip_atomic_load(&mm[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&mm[0], mna);
+        if (mxa > __hip_atomic_load(&mm[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&mm[1], mxa);
+        if (mnb < __hip_atomic_load(&mm[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&mm[2], mnb);
+        if (mxb > __hip_atomic_load(&mm[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&mm[3], mxb);
         if (dc) atomicAdd(desc, (unsigned long long)dc);
     }
 }
