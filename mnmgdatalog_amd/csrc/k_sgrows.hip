@@ -25,6 +25,13 @@
 namespace dlg {
 
 constexpr int SR_T = 256;   // threads per workgroup of the row OR
+// The round's totals are added by every workgroup; workgroup b adds into
+// slice b % SG_SLICES of the counters (one 64-byte line each), and the host
+// sums the slices: device atomics from 2,048 workgroups on one line serialise.
+constexpr int SG_SLICES = 64, SG_SLICE_WORDS = 8;
+__device__ __forceinline__ unsigned long long *sg_slice(unsigned long long *counters) {
+    return counters + SG_SLICE_WORDS * (blockIdx.x % SG_SLICES);
+}
 
 // predecessor lists: pred[poff[y] ..] = the b of every edge(b, y), relative to lo
 __global__ void sg_pred_hist_k(const u64 *rows, u64 n, u32 lo, u32 *cnt) {
@@ -58,8 +65,8 @@ __global__ void sg_succ_fill_k(const u64 *rows, u64 n, u32 lo, u32 *succ) {
 // list (SG_CAP at a time), and the threads take the list's a's in turn, two
 // at a time, so a word dense in a's is spread over the workgroup.  ELL: a's
 // successors are one row of SG_ELL_K ids (SG_EMPTY past the degree), else
-// a CSR list.  counters[4] += Σ deg(a) (the join's raw rows), counters[5] +=
-// the words of D read.
+// a CSR list.  In the workgroup's slice of the counters: [4] += Σ deg(a)
+// (the join's raw rows), [5] += the words of D read.
 constexpr int SG_ELL_K = 16;
 constexpr u32 SG_EMPTY = 0xffffffffu;   // ids relative to lo stay below it
 constexpr int SG_CAP = 2048;
@@ -160,8 +167,8 @@ __global__ __launch_bounds__(SR_T) void sg_tjoin_k(u32 *D, const u64 *soff, cons
     block_exclusive_scan(raw, s_scan, &rt);
     block_exclusive_scan(dwords, s_scan, &dt);
     if (t == 0) {
-        if (rt) atomicAdd(counters + 4, (unsigned long long)rt);
-        if (dt) atomicAdd(counters + 5, (unsigned long long)dt);
+        if (rt) atomicAdd(sg_slice(counters) + 4, (unsigned long long)rt);
+        if (dt) atomicAdd(sg_slice(counters) + 5, (unsigned long long)dt);
     }
 }
 
@@ -186,9 +193,9 @@ __global__ void bm_clear_rows_k(u32 *words, u64 W, u64 wpr, u32 *flag) {
 }
 
 // N row y = OR of the flagged R rows of y's predecessors, minus F row y;
-// new bits go into F and D.  counters[0] += the raw join rows (popcounts of
-// the R words read), counters[1] += R words read, counters[2] += F words
-// read, counters[3] += new bits.
+// new bits go into F and D.  In the workgroup's slice of the counters:
+// [0] += the raw join rows (popcounts of the R words read), [1] += R words
+// read, [2] += F words read, [3] += new bits.
 __global__ __launch_bounds__(SR_T) void sg_rowor_k(const u32 *R, const u32 *rflag, const u64 *poff,
                                                    const u32 *pred, u64 W, u64 wpr, u32 *F, u32 *D,
                                                    unsigned long long *counters) {
@@ -257,7 +264,7 @@ __global__ __launch_bounds__(SR_T) void sg_rowor_k(const u32 *R, const u32 *rfla
     if (threadIdx.x < 4) {
         u64 t = 0;
         for (int w = 0; w < SR_T / WAVE; w++) t += s_red[threadIdx.x][w];
-        if (t) atomicAdd(counters + threadIdx.x, (unsigned long long)t);
+        if (t) atomicAdd(sg_slice(counters) + threadIdx.x, (unsigned long long)t);
     }
 }
 
@@ -304,7 +311,7 @@ void sg_rows_setup(dlg_ctx *ctx, const Index &ix, const Bitmap &F, SgRows &sr) {
     }
     sr.rflag.ensure(W);
     fill_u32(ctx, sr.rflag.p, W, 0u);
-    sr.counters.ensure(6);
+    sr.counters.ensure(SG_SLICES * SG_SLICE_WORDS);
     ctx->sync();   // cnt goes out of scope
 }
 
@@ -314,7 +321,7 @@ u64 sg_rows_round(dlg_ctx *ctx, SgRows &sr, Bitmap &R, Bitmap &F, Bitmap &D, u64
     const u64 W = R.W, wpr = R.wpr;
     if ((u64)ctx->lds_per_block < 4 * wpr + 4 * SG_CAP + 4096)
         fail(DLG_INVALID_INPUT, "sg_rows_round: a row of the map does not fit the LDS of a workgroup");
-    fill_u64(ctx, sr.counters.p, 6, 0ull);
+    fill_u64(ctx, sr.counters.p, SG_SLICES * SG_SLICE_WORDS, 0ull);
     const unsigned g = (unsigned)std::min<u64>(W, (u64)ctx->num_cus * 8);
     if (sr.ell.p)
         DLG_LAUNCH(ctx, DLG_K_BITMAP_INSERT, 0.0, sg_tjoin_k<true>, g, SR_T, (unsigned)(4 * wpr), D.words.p,
@@ -329,8 +336,11 @@ u64 sg_rows_round(dlg_ctx *ctx, SgRows &sr, Bitmap &R, Bitmap &F, Bitmap &D, u64
                D.words.p, (unsigned long long *)sr.counters.p);
     const unsigned gw = (unsigned)std::min<u64>((W + 3) / 4, (u64)ctx->num_cus * 16);   // a wave per row
     DLG_LAUNCH(ctx, DLG_K_MISC, 4.0 * W, bm_clear_rows_k, gw, 256, 0, R.words.p, W, wpr, sr.rflag.p);
-    u64 h[6];
-    ctx->read_scalars(sr.counters.p, 6, h);
+    u64 sl[SG_SLICES * SG_SLICE_WORDS], h[6] = {0, 0, 0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(sl, sr.counters.p, sizeof(sl), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < SG_SLICES; k++)
+        for (int q = 0; q < 6; q++) h[q] += sl[SG_SLICE_WORDS * k + q];
     // algorithmic bytes: D words read, 4 B per successor, R words read, touched F words read (+ written)
     ctx->prof_add_bytes(DLG_K_BITMAP_INSERT, 4.0 * h[5] + 4.0 * h[4] + 4.0 * h[1] + 8.0 * h[2]);
     if (raw) *raw += h[4] + h[0];
