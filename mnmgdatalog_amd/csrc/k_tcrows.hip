@@ -588,16 +588,17 @@ __global__ __launch_bounds__(TA_T) void tc_rows_compact_k(const u32 *ax, const u
     }
     // the blocks before this one (all of them, for the last block's totals)
     u64 pc = 0, ps = 0, pn = 0, pr = 0, pt = 0, pw = 0;
-    for (u32 j = threadIdx.x; j < (last ? gridDim.x : blockIdx.x); j += TA_T) {
-        const bool before = j < blockIdx.x;
-        if (before) {
-            pc += bpart[TR_PART * j];
-            ps += bpart[TR_PART * j + 1];
+    for (u32 j = threadIdx.x; j < blockIdx.x; j += TA_T) {   // the two columns every block needs
+        pc += bpart[TR_PART * j];
+        ps += bpart[TR_PART * j + 1];
+    }
+    if (last) {   // uniform: the other four only for the round's totals
+        for (u32 j = threadIdx.x; j < gridDim.x; j += TA_T) {
+            pn += bpart[TR_PART * j + 2];
+            pr += bpart[TR_PART * j + 3];
+            pt += bpart[TR_PART * j + 4];
+            pw += bpart[TR_PART * j + 5];
         }
-        pn += bpart[TR_PART * j + 2];
-        pr += bpart[TR_PART * j + 3];
-        pt += bpart[TR_PART * j + 4];
-        pw += bpart[TR_PART * j + 5];
     }
     u64 ec0, es0, tn = 0, trw = 0, ttw = 0, tcw = 0;
     block_exclusive_scan(pc, scr, &ec0);
